@@ -1,0 +1,184 @@
+/*
+ * tensormesh_amd.h -- C-ABI of libtmhip.so, the MI355X (gfx950) kernels of the
+ * tensorized element-assembly path.
+ *
+ * The reference (kilickursat/TensorMesh) is pure Python/PyTorch and has no
+ * FFI; its "operator API" is the Python classes ElementAssembler,
+ * NodeAssembler, ReduceProjector, Condenser and SparseMatrix.  Each entry
+ * point below replaces one step of that path and cites the reference lines
+ * it stands for.  The Python package tensormesh_amd binds these with ctypes
+ * (tensormesh_amd/_native.py).
+ *
+ * Conventions
+ *  - Device pointers unless a parameter says "host".  All memory is owned by
+ *    the caller; no entry point allocates, frees or synchronises, except the
+ *    host-side pattern builders, which run on the calling thread.
+ *  - Every device entry point is stream-ordered on the hipStream_t it gets.
+ *  - Return 0 on success, a negative tm_status otherwise; tm_last_error()
+ *    gives a thread-local message.  No C++ exception crosses the ABI.
+ *  - Indices are int32; entry points reject sizes that do not fit.
+ *  - dtype TM_F64: points, coefficients and values are double.  TM_F32: they
+ *    are float; element geometry (edge vectors, cofactors, determinant) is
+ *    still evaluated in double, contraction and accumulation in float.
+ */
+#ifndef TENSORMESH_AMD_H
+#define TENSORMESH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* tm_stream_t; /* == hipStream_t */
+
+#define TM_ABI_VERSION 1
+
+enum tm_status {
+  TM_OK = 0,
+  TM_ERR_ARG = -1,         /* bad pointer / shape / enum value */
+  TM_ERR_SIZE = -2,        /* size beyond int32 indexing or a kernel limit */
+  TM_ERR_HIP = -3,         /* HIP runtime error (launch / memcpy) */
+  TM_ERR_UNSUPPORTED = -4, /* element / form / dtype combination not built */
+  TM_ERR_TOPOLOGY = -5     /* repeated node inside one element */
+};
+
+enum tm_dtype { TM_F32 = 0, TM_F64 = 1 };
+
+enum tm_cell {
+  TM_TRI3 = 0,  /* "triangle"  B=3  D=2 */
+  TM_TET4 = 1,  /* "tetra"     B=4  D=3 */
+  TM_TRI6 = 2,  /* "triangle6" B=6  D=2 */
+  TM_TET10 = 3  /* "tetra10"   B=10 D=3 */
+};
+
+enum tm_form {
+  TM_FORM_LAPLACE = 0,        /* gradu @ gradv                 builtin.py:61-62 */
+  TM_FORM_DIFFUSION_ELEM = 1, /* kappa[e] * (gradu @ gradv)    thermal_topology.py:48-52 */
+  TM_FORM_DIFFUSION_NODE = 2, /* kappa_h(x) * (gradu @ gradv)  forms.rst:172-184 */
+  TM_FORM_MASS = 3,           /* u * v                         builtin.py:99-100 */
+  TM_FORM_LINEAR_ELASTICITY = 4 /* Ba^T C Bb (3x3 blocks)      builtin.py:172-178 */
+};
+
+/* Element block on the device. */
+typedef struct tm_mesh_view {
+  int32_t cell;          /* enum tm_cell */
+  int32_t dtype;         /* enum tm_dtype of points / coefficients / values */
+  int64_t n_elem;
+  int64_t n_points;
+  const int32_t* conn;   /* [n_elem, B] row-major */
+  const void* points;    /* [n_points, D] row-major */
+} tm_mesh_view;
+
+/*
+ * Reference-element tables, float64, for Q quadrature points, B basis
+ * functions and D dimensions:
+ *   [0, Q)                          w[q]
+ *   [Q, Q + Q*B)                    phi[q][b]
+ *   [Q + Q*B, Q + Q*B + Q*D*B)      dphi/dxi[q][d][b]
+ */
+typedef struct tm_form_desc {
+  int32_t form;              /* enum tm_form */
+  int32_t n_quad;            /* Q */
+  const double* tab_host;    /* host copy of the tables (read during the call) */
+  const double* tab_dev;     /* device copy (read by P2 kernels) */
+  const void* coeff;         /* DIFFUSION_ELEM: [n_elem]; DIFFUSION_NODE: [n_points]; else NULL */
+  double young;              /* LINEAR_ELASTICITY: E  */
+  double poisson;            /* LINEAR_ELASTICITY: nu */
+} tm_form_desc;
+
+/* Sparsity pattern plus the slot -> nnz permutation in segment form. */
+typedef struct tm_pattern_view {
+  int64_t nnz;               /* node-pair entries */
+  const int32_t* row_ptr;    /* [n_points + 1] CSR */
+  const int32_t* col_idx;    /* [nnz] */
+  const int32_t* seg_ptr;    /* [nnz + 1] start of each entry's contributions */
+  const int32_t* seg_slot;   /* [n_elem*B*B] slot ids s = e*B*B + i*B + j, ascending per entry */
+} tm_pattern_view;
+
+/* Node -> (element, local index) incidence, ascending element id per node. */
+typedef struct tm_incidence_view {
+  const int32_t* inc_ptr;    /* [n_points + 1] */
+  const int32_t* inc;        /* [n_elem*B] packed e*B + i */
+} tm_incidence_view;
+
+int tm_abi_version(void);
+const char* tm_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Pattern (host).  Replaces the edge / projector construction of
+ * ElementAssembler.from_mesh (tensormesh/assemble/element_assembler.py:804-853):
+ * CSR of the sorted unique (conn[e,i], conn[e,j]) pairs -- the order scipy
+ * coo->csr->coo gives -- and, per entry, the ascending list of slots
+ * s = e*B*B + i*B + j that ReduceProjector.index_add_ would sum into it.
+ * All pointers are HOST pointers.  tm_pattern_nnz sizes the outputs;
+ * tm_pattern_build fills them (slot2nnz may be NULL).
+ * ------------------------------------------------------------------- */
+int tm_pattern_nnz(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                   int64_t* nnz_out);
+int tm_pattern_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                     int32_t* row_ptr, int32_t* col_idx, int32_t* seg_ptr, int32_t* seg_slot,
+                     int32_t* slot2nnz);
+
+/* Node incidence (host), the scatter map of NodeAssembler's ReduceProjector
+ * (tensormesh/assemble/node_assembler.py:665-670) in gather form. */
+int tm_incidence_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                       int32_t* inc_ptr, int32_t* inc);
+
+/* ---------------------------------------------------------------------
+ * Bilinear assembly, two fixed stages on one stream.
+ * Replaces ElementAssembler.__call__ for the built-in forms
+ * (element_assembler.py:231-445): geometry Transformation.batch_shape_grad_jxw
+ * (transformation.py:1219-1276, element.py:874-968), form + _integrate
+ * (element_assembler.py:190-209, 416-438), ReduceProjector.__call__
+ * (projector.py:95-123).
+ *   stage G: per element -> slot_ws[n_elem*B*B*H*H] (H = 3 for elasticity, else 1)
+ *   stage S: segmented reduction slot_ws -> values[nnz*H*H] in entry order,
+ *            each entry summed in ascending slot order (no atomics).
+ * values is the edata of SparseMatrix (scalar) or the block data handed to
+ * SparseMatrix.from_block_coo (matrix.py:331-381) for H > 1.
+ * ------------------------------------------------------------------- */
+int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc* form,
+                         const tm_pattern_view* pat, void* slot_ws, void* values,
+                         tm_stream_t stream);
+
+/* Stage G alone (element matrices into slot_ws).  Same citation as above. */
+int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
+                        tm_stream_t stream);
+
+/* Stage S alone: out[k*HH + t] = sum over s in seg(k) of slot[s*HH + t].
+ * The deterministic form of ReduceProjector.__call__ (projector.py:95-123). */
+int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
+                        const int32_t* seg_slot, const void* slot, void* out,
+                        tm_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Linear-form assembly (load vector), two stages on one stream.
+ * Replaces NodeAssembler.__call__ (tensormesh/assemble/node_assembler.py:357-409)
+ * for forward(v, f) = f * v with nodal f (form->coeff, [n_points]) or
+ * forward(v) = c * v (const_value != NULL, builtin.py:730-770):
+ *   b[n] = sum_{(e,i): conn[e,i]=n} sum_q w_q |det J_q| phi_i(q) f_h(q)
+ * slot_ws: [n_elem*B]; out: [n_points]; form->tab_dev and n_quad required.
+ * ------------------------------------------------------------------- */
+int tm_assemble_linear(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
+                       const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Condensation values.  Replaces the per-call part of Condenser.__call__
+ * (tensormesh/operator/condense.py:278-296).  Topology arrays come from the
+ * once-per-pattern layout (condense.py:182-220), all int32, device:
+ *   inner_rp[n_inner+1], inner_idx[nnz_ii]   entries of K_ii per inner row
+ *   ou_rp[n_inner+1], ou_idx[nnz_io], ou_col[nnz_io]  entries / outer dof of K_io
+ *   inner_dof[n_inner]                        global dof of each inner dof
+ * Outputs kii[nnz_ii] = values[inner_idx], f_i[n_inner] = rhs[inner_dof] - K_io u_o.
+ * ------------------------------------------------------------------- */
+int tm_condense(int32_t dtype, int64_t n_inner, const void* values, const int32_t* inner_rp,
+                const int32_t* inner_idx, const int32_t* ou_rp, const int32_t* ou_idx, const int32_t* ou_col,
+                const int32_t* inner_dof, const void* rhs, const void* u_o, void* kii, void* f_i,
+                tm_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TENSORMESH_AMD_H */

@@ -1,0 +1,372 @@
+"""float64 CPU restatement of the reference assembly chain.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Each function names the reference lines it restates.  Floating-point work
+is done with torch-CPU in float64 using the same operation sequence as the
+reference (einsum, batched ``torch.inverse`` / ``torch.linalg.det``,
+``index_add_``), so that autograd through it reproduces the reference's
+gradients; integer pattern work uses numpy/scipy exactly as
+``ElementAssembler.from_mesh`` does.
+"""
+
+from __future__ import annotations
+
+import itertools
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+import scipy.sparse
+import torch
+
+from tensormesh_amd.element import TET_QUAD, TRI_QUAD  # data tables only
+
+FAMILY = {"triangle": ("tri", 2, 1), "triangle6": ("tri", 2, 2),
+          "tetra": ("tet", 3, 1), "tetra10": ("tet", 3, 2)}
+TRI_EDGE = [[1, 2], [0, 2], [0, 1]]                         # element.py:1684
+TET_EDGE = [[2, 3], [1, 3], [1, 2], [0, 3], [0, 2], [0, 1]]  # element.py:1949
+
+
+# ---------------------------------------------------------------- pattern ---
+def pattern(conn: np.ndarray, n_points: int):
+    """Edges and slot->edge map of ``ElementAssembler.from_mesh``.
+
+    Restates tensormesh/assemble/element_assembler.py:804-836: slot
+    ``s = e*B*B + i*B + j`` pairs ``(conn[e,i], conn[e,j])``; scipy
+    ``coo -> csr -> coo`` yields the sorted unique pairs (CSR order); a second
+    CSR with data ``arange(nnz)`` is the lookup table for ``elem_eids``.
+    Returns (row[nnz], col[nnz], eids[S]) as int64 numpy arrays.
+    """
+    conn = np.asarray(conn, dtype=np.int64)
+    n_elem, nb = conn.shape
+    elem_u = np.repeat(conn, nb, axis=1).reshape(-1)          # conn[e, i] for j fast
+    elem_v = np.tile(conn, (1, nb)).reshape(-1)               # conn[e, j]
+    tmp = scipy.sparse.coo_matrix((np.ones_like(elem_u), (elem_u, elem_v)),
+                                  shape=(n_points, n_points)).tocsr().tocoo()
+    row, col = tmp.row.astype(np.int64), tmp.col.astype(np.int64)
+    eids_csr = scipy.sparse.coo_matrix((np.arange(row.size), (row, col)),
+                                       shape=(n_points, n_points)).tocsr()
+    eids = np.asarray(eids_csr[elem_u, elem_v]).ravel().astype(np.int64)
+    return row, col, eids
+
+
+# ----------------------------------------------------- reference element ---
+def quadrature(etype: str, order: int):
+    """Quadrature table (tensormesh/element/quadrature.py:91-160)."""
+    fam = FAMILY[etype][0]
+    w, p = (TRI_QUAD if fam == "tri" else TET_QUAD)[order]
+    return torch.tensor(w, dtype=torch.float64), torch.tensor(p, dtype=torch.float64)
+
+
+def lagrange_nodes(etype: str) -> np.ndarray:
+    """Basis nodes: vertices then edge midpoints (basis.py:639-696,750-859)."""
+    fam, dim, order = FAMILY[etype]
+    verts = np.vstack([np.zeros(dim), np.eye(dim)])
+    if order == 1:
+        return verts
+    edges = TRI_EDGE if fam == "tri" else TET_EDGE
+    return np.vstack([verts] + [0.5 * (verts[a] + verts[b]) for a, b in edges])
+
+
+def _exponents(dim: int, order: int):
+    return [e for e in itertools.product(range(order + 1), repeat=dim) if sum(e) <= order]
+
+
+def vandermonde_coefficients(etype: str) -> Tuple[np.ndarray, list]:
+    """coef[B, M] with phi_n = sum_m coef[n, m] * monomial_m.
+
+    Restates ``Element.get_basis_fns`` (tensormesh/element/element.py:636-659):
+    ``V = monomials(basis nodes)``, ``coef = inv(V).T``.
+    """
+    fam, dim, order = FAMILY[etype]
+    nodes = lagrange_nodes(etype)
+    exps = _exponents(dim, order)
+    V = np.array([[np.prod(node ** np.array(e)) for e in exps] for node in nodes])
+    return np.linalg.inv(V).T, exps
+
+
+def shape_values(etype: str, pts: np.ndarray) -> np.ndarray:
+    coef, exps = vandermonde_coefficients(etype)
+    mono = np.array([[np.prod(p ** np.array(e)) for e in exps] for p in pts])  # [Q, M]
+    return mono @ coef.T                                                          # [Q, B]
+
+
+def shape_ref_grads(etype: str, pts: np.ndarray) -> np.ndarray:
+    """dphi[Q, D, B] (layout of ``Polynomials.map`` in element.py:960)."""
+    coef, exps = vandermonde_coefficients(etype)
+    dim = pts.shape[1]
+    out = np.zeros((pts.shape[0], dim, coef.shape[0]))
+    for d in range(dim):
+        dm = np.zeros((pts.shape[0], len(exps)))
+        for m, e in enumerate(exps):
+            if e[d] == 0:
+                continue
+            e2 = list(e)
+            e2[d] -= 1
+            dm[:, m] = e[d] * np.prod(pts ** np.array(e2), axis=1)
+        out[:, d, :] = dm @ coef.T
+    return out
+
+
+def shape_grad_jxw(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder: int):
+    """Physical shape gradients and JxW.
+
+    Restates ``Element.eval_shape_grad`` (tensormesh/element/element.py:956-968)
+    and ``Transformation.batch_shape_grad_jxw`` (transformation.py:1265-1272):
+    ``J = einsum('ebj,qib->eqij')``, ``inv = torch.inverse(J)``,
+    ``grad = einsum('qib,eqji->eqbj')``, ``jxw = w * |det J|``.
+    Returns (phi[Q,B], grad[E,Q,B,D], jxw[E,Q], J[E,Q,D,D]).
+    """
+    w, q = quadrature(etype, qorder)
+    qn = q.numpy()
+    phi = torch.tensor(shape_values(etype, qn), dtype=points.dtype)
+    dphi = torch.tensor(shape_ref_grads(etype, qn), dtype=points.dtype)
+    coords = points[conn]                                           # [E, B, D]
+    J = torch.einsum("ebj,qib->eqij", coords, dphi)
+    inv = torch.inverse(J)
+    grad = torch.einsum("qib,eqji->eqbj", dphi, inv)
+    jxw = torch.einsum("q,eq->eq", w.to(points.dtype), torch.linalg.det(J).abs())
+    return phi, grad, jxw, J
+
+
+# -------------------------------------------------------------- integrands ---
+def voigt_B(g: torch.Tensor) -> torch.Tensor:
+    """Voigt strain-displacement rows for one basis gradient g[..., 3].
+
+    Restates ``voigt_shape_grad`` (tensormesh/functional/elasticity.py:132-200):
+    rows [diag(g); skew(g, sign=False)] = [xx, yy, zz, yz, xz, xy].
+    """
+    z = torch.zeros_like(g[..., 0])
+    g0, g1, g2 = g[..., 0], g[..., 1], g[..., 2]
+    rows = [torch.stack([g0, z, z], -1), torch.stack([z, g1, z], -1), torch.stack([z, z, g2], -1),
+            torch.stack([z, g2, g1], -1), torch.stack([g2, z, g0], -1), torch.stack([g1, g0, z], -1)]
+    return torch.stack(rows, -2)                                    # [..., 6, 3]
+
+
+def voigt_B2(g: torch.Tensor) -> torch.Tensor:
+    """2-D Voigt rows [xx, yy, xy] (elasticity.py:132-200, dim 2)."""
+    z = torch.zeros_like(g[..., 0])
+    g0, g1 = g[..., 0], g[..., 1]
+    return torch.stack([torch.stack([g0, z], -1), torch.stack([z, g1], -1),
+                        torch.stack([g1, g0], -1)], -2)             # [..., 3, 2]
+
+
+def voigt_C(E: float, nu: float, dim: int) -> torch.Tensor:
+    """Isotropic Voigt stiffness (elasticity.py:202-284)."""
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    n = 3 if dim == 2 else 6
+    C = torch.zeros(n, n, dtype=torch.float64)
+    C[:dim, :dim] = lam
+    for k in range(dim):
+        C[k, k] = lam + 2.0 * mu
+    for k in range(dim, n):
+        C[k, k] = mu
+    return C
+
+
+def element_integrals(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder: int,
+                      form: str, kappa: Optional[torch.Tensor] = None,
+                      E: float = 1.0, nu: float = 0.3) -> torch.Tensor:
+    """Per-element integrals [E,B,B] or [E,B,B,H,H].
+
+    Restates the vmap'd form evaluation and ``_integrate``
+    (tensormesh/assemble/element_assembler.py:359-441 and :190-209) for the
+    built-in forms: Laplace ``gradu @ gradv`` (builtin.py:61-62), mass
+    ``u * v`` (:99-100), element-coefficient diffusion ``kappa * (gradu @
+    gradv)`` with ``element_data={'kappa': [E]}`` (thermal_topology.py:48-52),
+    node-coefficient diffusion with ``point_data={'kappa': [N]}``
+    (forms.rst:172-184), linear elasticity ``Ba.T @ C @ Bb`` (builtin.py:172-178).
+    """
+    phi, grad, jxw, _ = shape_grad_jxw(points, conn, etype, qorder)
+    dim = points.shape[1]
+    if form == "mass":
+        integrand = torch.einsum("qi,qj->qij", phi, phi)
+        return torch.einsum("qij,eq->eij", integrand, jxw)
+    if form == "laplace":
+        integrand = torch.einsum("eqid,eqjd->eqij", grad, grad)
+    elif form == "diffusion_elem":
+        integrand = kappa[:, None, None, None] * torch.einsum("eqid,eqjd->eqij", grad, grad)
+    elif form == "diffusion_node":
+        kq = torch.einsum("eb,qb->eq", kappa[conn], phi)
+        integrand = kq[:, :, None, None] * torch.einsum("eqid,eqjd->eqij", grad, grad)
+    elif form == "linear_elasticity":
+        C = voigt_C(E, nu, dim).to(points.dtype)
+        B = voigt_B(grad) if dim == 3 else voigt_B2(grad)          # [E,Q,B,V,D]
+        integrand = torch.einsum("eqiva,vw,eqjwb->eqijab", B, C, B)
+    else:
+        raise ValueError(form)
+    return torch.einsum("eqij...,eq->eij...", integrand, jxw)
+
+
+# -------------------------------------------------------------- projector ---
+def reduce_project(x: torch.Tensor, indices: torch.Tensor, n_out: int) -> torch.Tensor:
+    """``ReduceProjector.__call__`` (tensormesh/assemble/projector.py:95-123):
+    ``zeros(n_out).index_add_(0, indices, x.reshape(S, ...))``."""
+    return _index_add(x, indices, n_out)
+
+
+def _index_add(x: torch.Tensor, indices: torch.Tensor, n_out: int) -> torch.Tensor:
+    """Flatten the leading dims of ``x`` that cover ``indices.numel()`` slots
+    and accumulate them into ``n_out`` rows in slot order."""
+    S = indices.numel()
+    lead = 1
+    k = 0
+    while lead < S:
+        lead *= x.shape[k]
+        k += 1
+    tail = x.shape[k:]
+    flat = x.reshape(S, *tail)
+    out = torch.zeros(n_out, *tail, dtype=x.dtype)
+    return out.index_add(0, indices, flat)
+
+
+def block_coo(values: torch.Tensor, row: torch.Tensor, col: torch.Tensor):
+    """``SparseMatrix.from_block_coo`` (tensormesh/sparse/matrix.py:331-381):
+    value index ``edge*H*H + a*H + b``, row ``edge_u*H + a``, col ``edge_v*H + b``."""
+    H = values.shape[1]
+    a = torch.arange(H).repeat_interleave(H)
+    b = torch.arange(H).repeat(H)
+    r = (row[:, None] * H + a[None]).reshape(-1)
+    c = (col[:, None] * H + b[None]).reshape(-1)
+    return values.reshape(-1), r, c
+
+
+def assemble_bilinear(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder: int, form: str,
+                      kappa: Optional[torch.Tensor] = None, E: float = 1.0, nu: float = 0.3,
+                      pat=None):
+    """Full ``ElementAssembler.__call__`` for one element block.
+
+    Returns (values, row, col, shape) of the output COO; block forms are
+    expanded with ``block_coo`` (element_assembler.py:211-229).
+    """
+    n = points.shape[0]
+    if pat is None:
+        pat = pattern(conn.numpy(), n)
+    row, col, eids = (torch.from_numpy(a) for a in pat)
+    loc = element_integrals(points, conn, etype, qorder, form, kappa, E, nu)
+    vals = _index_add(loc, eids, row.numel())
+    if vals.dim() == 1:
+        return vals, row, col, (n, n)
+    v, r, c = block_coo(vals, row, col)
+    H = vals.shape[1]
+    return v, r, c, (n * H, n * H)
+
+
+def assemble_linear(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder: int,
+                    f_nodal: torch.Tensor) -> torch.Tensor:
+    """Load vector of ``NodeAssembler`` with ``forward(v, f) = f * v``.
+
+    Restates tensormesh/assemble/node_assembler.py:357-409: nodal ``f`` is
+    interpolated to the quadrature points (``einsum('eb,qb->eq')``), the
+    integrand ``f_h * phi`` is integrated against ``jxw`` one quadrature point
+    at a time (default ``batch_size=1``) and scattered with ``index_add_`` over
+    ``conn.flatten()`` (node_assembler.py:665-670).
+    """
+    phi, grad, jxw, _ = shape_grad_jxw(points, conn, etype, qorder)
+    fq = torch.einsum("eb,qb->eq", f_nodal[conn], phi)
+    elem = None
+    for q in range(phi.shape[0]):
+        part = torch.einsum("eqb,eq->eb", (fq[:, q:q + 1, None] * phi[None, q:q + 1, :]), jxw[:, q:q + 1])
+        elem = part if elem is None else elem + part
+    return torch.zeros(points.shape[0], dtype=points.dtype).index_add(0, conn.reshape(-1), elem.reshape(-1))
+
+
+# -------------------------------------------------------------- condenser ---
+def condense(values, row, col, n_dof, mask, dvalue=None, rhs=None):
+    """``Condenser._compute_layout`` + ``__call__`` (tensormesh/operator/condense.py:182-296).
+
+    Returns dict with K_ii (values,row,col,shape), K_io, f_i and the local
+    numbering used by ``recover``.
+    """
+    inner = ~mask
+    is_inner_edge = inner[row] & inner[col]
+    is_ou2in_edge = inner[row] & mask[col]
+    n_in, n_out = int(inner.sum()), int(mask.sum())
+    local = torch.full((n_dof,), -1, dtype=torch.long)
+    local[inner] = torch.arange(n_in)
+    local[mask] = torch.arange(n_out)
+    if dvalue is None:
+        dvalue = torch.zeros(n_out, dtype=values.dtype)
+    elif dvalue.shape[0] == n_dof:
+        dvalue = dvalue[mask]
+    if rhs is None:
+        rhs = torch.zeros(n_dof, dtype=values.dtype)
+    kio_v = values[is_ou2in_edge]
+    kio_r = local[row[is_ou2in_edge]]
+    kio_c = local[col[is_ou2in_edge]]
+    minus = torch.zeros(n_in, dtype=values.dtype).index_add(0, kio_r, kio_v * dvalue[kio_c])
+    return {
+        "K_ii": (values[is_inner_edge], local[row[is_inner_edge]], local[col[is_inner_edge]], (n_in, n_in)),
+        "K_io": (kio_v, kio_r, kio_c, (n_in, n_out)),
+        "f_i": rhs[inner] - minus,
+        "dvalue": dvalue,
+    }
+
+
+def recover(u_inner, mask, dvalue):
+    """``Condenser.recover`` (condense.py:339-368)."""
+    out = torch.zeros(mask.shape[0], dtype=u_inner.dtype)
+    out[~mask] = u_inner
+    out[mask] = dvalue.to(u_inner.dtype)
+    return out
+
+
+# ----------------------------------------------------------- neo-hookean ---
+def cofactor_det(F: torch.Tensor) -> torch.Tensor:
+    """det by cofactor expansion; same value as torch.linalg.det, but with a
+    finite second derivative at F = I (torch.linalg.det's double backward
+    returns NaN there)."""
+    if F.shape[-1] == 2:
+        return F[..., 0, 0] * F[..., 1, 1] - F[..., 0, 1] * F[..., 1, 0]
+    return (F[..., 0, 0] * (F[..., 1, 1] * F[..., 2, 2] - F[..., 1, 2] * F[..., 2, 1])
+            - F[..., 0, 1] * (F[..., 1, 0] * F[..., 2, 2] - F[..., 1, 2] * F[..., 2, 0])
+            + F[..., 0, 2] * (F[..., 1, 0] * F[..., 2, 1] - F[..., 1, 1] * F[..., 2, 0]))
+
+
+def neohookean_psi(F: torch.Tensor, mu: float, lam: float, det=torch.linalg.det) -> torch.Tensor:
+    """psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2 (builtin.py:301-314)."""
+    d = F.shape[-1]
+    J = det(F)
+    I1 = (F * F).sum((-2, -1))
+    lj = torch.log(J)
+    return 0.5 * mu * (I1 - d) - mu * lj + 0.5 * lam * lj * lj
+
+
+def lame(E: float, nu: float):
+    return E / (2.0 * (1.0 + nu)), E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+
+
+def neohookean_energy(points, conn, etype, qorder, u, E=1.0, nu=0.3):
+    """``NeoHookeanModel.energy`` via ``ElementAssembler.energy``
+    (builtin.py:316-333, element_assembler.py:631-657):
+    ``graddisplacement = einsum('eb...,eqbd->eq...d', u[conn], shape_grad)``,
+    ``F = I + grad``, energy ``sum(psi * jxw)``."""
+    mu, lam = lame(E, nu)
+    _, grad, jxw, _ = shape_grad_jxw(points, conn, etype, qorder)
+    gu = torch.einsum("eba,eqbd->eqad", u[conn], grad)
+    F = torch.eye(points.shape[1], dtype=u.dtype) + gu
+    return (neohookean_psi(F, mu, lam) * jxw).sum()
+
+
+def neohookean_element_tangent(points, conn, etype, qorder, u, E=1.0, nu=0.3):
+    """Per-element residual [E,B,H] and tangent [E,B,B,H,H] by autograd.
+
+    The reference has no in-tree tangent: its residual is ``autograd`` of
+    the energy (element_assembler.py:447-659) and a Newton solve would take
+    the autograd Jacobian.  This computes exactly those derivatives element by
+    element with ``torch.func`` (small meshes only).
+    """
+    mu, lam = lame(E, nu)
+    _, grad, jxw, _ = shape_grad_jxw(points, conn, etype, qorder)
+    ue = u[conn]                                                    # [E,B,H]
+    dim = points.shape[1]
+    eye = torch.eye(dim, dtype=u.dtype)
+
+    def e_energy(ue_, g_, w_):
+        F = eye + torch.einsum("ba,qbd->qad", ue_, g_)
+        return (neohookean_psi(F, mu, lam, det=cofactor_det) * w_).sum()
+
+    res = torch.func.vmap(torch.func.grad(e_energy))(ue, grad, jxw)
+    hes = torch.func.vmap(torch.func.hessian(e_energy))(ue, grad, jxw)  # [E,B,H,B,H]
+    return res, hes.permute(0, 1, 3, 2, 4).contiguous()

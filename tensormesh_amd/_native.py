@@ -1,0 +1,131 @@
+"""ctypes binding of libtmhip.so (the C-ABI declared in include/tensormesh_amd.h).
+
+The library is loaded from the package directory (built in-tree by
+``tensormesh_amd.build``).  There is no fallback: if the library is missing
+or does not export the expected ABI version, ``lib()`` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Optional
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libtmhip.so"
+ABI_VERSION = 1
+
+# enum values (include/tensormesh_amd.h)
+TM_F32, TM_F64 = 0, 1
+CELL = {"triangle": 0, "tetra": 1, "triangle6": 2, "tetra10": 3}
+FORM = {"laplace": 0, "diffusion_elem": 1, "diffusion_node": 2, "mass": 3, "linear_elasticity": 4}
+
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_dblp = ctypes.POINTER(ctypes.c_double)
+
+
+class MeshView(ctypes.Structure):
+    _fields_ = [("cell", ctypes.c_int32), ("dtype", ctypes.c_int32),
+                ("n_elem", ctypes.c_int64), ("n_points", ctypes.c_int64),
+                ("conn", ctypes.c_void_p), ("points", ctypes.c_void_p)]
+
+
+class FormDesc(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int32), ("n_quad", ctypes.c_int32),
+                ("tab_host", ctypes.c_void_p), ("tab_dev", ctypes.c_void_p),
+                ("coeff", ctypes.c_void_p), ("young", ctypes.c_double),
+                ("poisson", ctypes.c_double)]
+
+
+class PatternView(ctypes.Structure):
+    _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("col_idx", ctypes.c_void_p),
+                ("seg_ptr", ctypes.c_void_p), ("seg_slot", ctypes.c_void_p)]
+
+
+class IncidenceView(ctypes.Structure):
+    _fields_ = [("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/tensormesh_amd.h
+SIGNATURES = {
+    "tm_abi_version": (ctypes.c_int, []),
+    "tm_last_error": (ctypes.c_char_p, []),
+    "tm_pattern_nnz": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, c_i64p]),
+    "tm_pattern_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "tm_incidence_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_assemble_bilinear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                            ctypes.POINTER(PatternView), ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
+    "tm_element_matrices": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_segmented_reduce": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_assemble_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                          ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 12),
+}
+
+_LIB: Optional[ctypes.CDLL] = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise NativeError(
+            f"{LIB_PATH} is missing: build it with `python -m tensormesh_amd.build` "
+            "(the HIP kernels are the only implementation of the built-in forms)")
+    handle = ctypes.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | getattr(os, "RTLD_LOCAL", 0))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    ver = handle.tm_abi_version()
+    if ver != ABI_VERSION:
+        raise NativeError(f"libtmhip.so ABI version {ver}, expected {ABI_VERSION}")
+    _LIB = handle
+    return handle
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = lib().tm_last_error().decode(errors="replace")
+        raise NativeError(f"{what} failed with status {status}: {msg}")
+
+
+def ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def require_device(t: torch.Tensor, what: str) -> None:
+    if t.device.type != "cuda":
+        raise NativeError(
+            f"{what} is on {t.device}; the built-in forms run on the ROCm device "
+            "(move the mesh / data with .cuda())")
+
+
+def stream_of(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float64:
+        return TM_F64
+    if dtype == torch.float32:
+        return TM_F32
+    raise NativeError(f"dtype {dtype} is not supported (float32 / float64)")

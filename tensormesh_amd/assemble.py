@@ -1,0 +1,563 @@
+"""ElementAssembler / NodeAssembler with the reference's API.
+
+Mirrors tensormesh/assemble/element_assembler.py:35-879 and
+tensormesh/assemble/node_assembler.py:33-741 for simplex meshes
+(triangle, tetra, triangle6, tetra10):
+
+* ``from_mesh(mesh, quadrature_order=2, project='reduce', *args, **kwargs)``
+  builds the CSR pattern and the slot permutation once (libtmhip host
+  builder) and forwards ``*args, **kwargs`` to ``__post_init__``;
+* ``from_assembler(obj, *args, **kwargs)`` shares that topology;
+* ``__call__(points=None, func=None, point_data=None, element_data=None,
+  scalar_data=None, batch_size=-1)`` returns a ``SparseMatrix`` whose
+  entries are in the reference's edge order (sorted unique (row, col)), or
+  for ``[B, B, H, H]`` integrands the ``from_block_coo`` expansion.
+
+Dispatch: a ``forward`` (or ``func``) recognised as a built-in integrand
+(``tensormesh_amd.forms``) runs on the HIP kernels and requires the data on
+the ROCm device -- there is no CPU or torch substitute for it.  Any other
+integrand runs through the reference's generic vmap path, restated in
+``_torch_bilinear`` / ``_torch_linear`` below; so do built-in integrands when
+gradients are requested for a form that has no hand-written adjoint yet.
+
+Geometry is recomputed on every call from the points passed in; the
+reference's cached-geometry quirk (transformation.py:137-161, stale
+``_shape_grad`` after ``update_points``) is deliberately not reproduced.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import inspect
+import math
+from typing import Callable, Dict, Mapping, Optional, Union
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from . import _native as nat
+from .element import ELEMENT_SHAPE, reference_tables, torch_tables
+from .forms import FormMatch, match_bilinear, match_linear
+from .pattern import Pattern, layout_hash
+from .sparse import SparseMatrix
+
+
+# ------------------------------------------------------------ geometry ---
+def shape_grad_jxw(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder: int,
+                   q_start: int = 0, q_batch: int = -1):
+    """Torch evaluation of the reference geometry (element.py:956-968,
+    transformation.py:1265-1272) for the generic form path."""
+    w, phi, dphi = torch_tables(etype, qorder, points.dtype, points.device)
+    stop = None if q_batch in (-1, None) else q_start + q_batch
+    w, phi, dphi = w[q_start:stop], phi[q_start:stop], dphi[q_start:stop]
+    coords = points[conn]
+    J = torch.einsum("ebj,qib->eqij", coords, dphi)
+    inv = torch.inverse(J)
+    grad = torch.einsum("qib,eqji->eqbj", dphi, inv)
+    jxw = torch.einsum("q,eq->eq", w, torch.linalg.det(J).abs())
+    return phi, grad, jxw
+
+
+def _n_quad(etype: str, qorder: int) -> int:
+    from .element import quadrature
+    return int(quadrature(etype, qorder)[0].shape[0])
+
+
+class _Broadcast:
+    def __init__(self, element, quadrature, u, v):
+        self.element = 0 if element else None
+        self.quadrature = 0 if quadrature else None
+        self.u = 0 if u else None
+        self.v = 0 if v else None
+
+
+# --------------------------------------------------------- base module ---
+class _AssemblerBase(nn.Module):
+    def _init_topology(self, elements: Dict[str, torch.Tensor], points: torch.Tensor, quadrature_order: int):
+        if len(elements) != 1:
+            raise NotImplementedError("tensormesh_amd assembles one element type per mesh")
+        etype = next(iter(elements))
+        if etype not in ELEMENT_SHAPE:
+            raise NotImplementedError(f"element type {etype} is not supported (triangle, tetra, triangle6, tetra10)")
+        self.element_types = [etype]
+        self.elements = {etype: elements[etype].long()}
+        self.dimension = ELEMENT_SHAPE[etype][0]
+        self.quadrature_order = quadrature_order
+        self.n_points = int(points.shape[0])
+        self._points = points
+        self._dev_cache: Dict[str, object] = {}
+
+    @property
+    def etype(self) -> str:
+        return self.element_types[0]
+
+    @property
+    def device(self) -> torch.device:
+        return self._points.device
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self._points.dtype
+
+    def type(self, dtype: torch.dtype):
+        if dtype not in (torch.float32, torch.float64):
+            raise Exception(f"the dtype {dtype} is not supported")
+        self._points = self._points.to(dtype)
+        return self
+
+    def _cached(self, key: str, device: torch.device, make):
+        k = f"{key}@{device}"
+        if k not in self._dev_cache:
+            self._dev_cache[k] = make()
+        return self._dev_cache[k]
+
+    def _conn32(self, device) -> torch.Tensor:
+        return self._cached("conn32", device,
+                            lambda: self.elements[self.etype].to(device=device, dtype=torch.int32).contiguous())
+
+    def _tables(self, device):
+        tab = reference_tables(self.etype, self.quadrature_order)
+        dev = self._cached(f"tab{self.quadrature_order}", device,
+                           lambda: torch.from_numpy(tab).to(device))
+        return tab, dev
+
+    def _mesh_view(self, points: torch.Tensor) -> nat.MeshView:
+        conn = self._conn32(points.device)
+        return nat.MeshView(nat.CELL[self.etype], nat.dtype_code(points.dtype), conn.shape[0], points.shape[0],
+                            conn.data_ptr(), points.data_ptr())
+
+
+# ----------------------------------------------------- ElementAssembler ---
+class ElementAssembler(_AssemblerBase):
+    """Bilinear-form assembler; see the module docstring."""
+
+    def __init__(self, elements: Dict[str, torch.Tensor], points: torch.Tensor, pattern: Pattern,
+                 quadrature_order: int = 2, *args, **kwargs):
+        super().__init__()
+        self._init_topology(elements, points, quadrature_order)
+        self._pattern = pattern
+        self.edges = torch.stack([pattern.row, pattern.col], 0)
+        self.__post_init__(*args, **kwargs)
+
+    def __post_init__(self, *args, **kwargs):
+        pass
+
+    def forward(self, *args, **kwargs):
+        raise NotImplementedError("forward is not implemented")
+
+    # ---- constructors (element_assembler.py:741-878) ----
+    @classmethod
+    def from_mesh(cls, mesh, quadrature_order: int = 2, project: str = "reduce", *args, **kwargs):
+        assert project in ["reduce", "sparse"]
+        elements = mesh.elements()
+        if isinstance(elements, torch.Tensor):
+            elements = {mesh.default_element_type: elements}
+        if len(elements) != 1:
+            raise NotImplementedError("tensormesh_amd assembles one element type per mesh")
+        conn = next(iter(elements.values()))
+        pattern = Pattern.build(conn, mesh.points.shape[0], device=mesh.points.device)
+        return cls(elements, mesh.points, pattern, quadrature_order, *args, **kwargs)
+
+    @classmethod
+    def from_assembler(cls, obj, *args, **kwargs):
+        assert isinstance(obj, ElementAssembler), f"obj must be an instance of ElementAssembler, but got {type(obj)}"
+        return cls(obj.elements, obj._points, obj._pattern, obj.quadrature_order, *args, **kwargs)
+
+    @property
+    def pattern(self) -> Pattern:
+        return self._pattern
+
+    def _pattern_on(self, device) -> Pattern:
+        if self._pattern.row_ptr.device == torch.device(device) or (
+                torch.device(device).type == "cuda" and self._pattern.row_ptr.device.type == "cuda"):
+            return self._pattern
+        return self._cached("pattern", torch.device(device), lambda: self._pattern.to(device))
+
+    # ---- __call__ (element_assembler.py:231-445) ----
+    def __call__(self, points: Optional[torch.Tensor] = None, func: Optional[Callable] = None,
+                 point_data: Optional[Mapping[str, torch.Tensor]] = None,
+                 element_data: Optional[Mapping] = None,
+                 scalar_data: Optional[Mapping] = None, batch_size: int = -1) -> SparseMatrix:
+        assert isinstance(point_data, dict) or point_data is None, \
+            f"point_data should be a dict, but got {type(point_data)}. Please pass  in extra parameter using key-value pairs"
+        point_data = {} if point_data is None else dict(point_data)
+        E = self.elements[self.etype].shape[0]
+        if element_data is None:
+            element_data = {}
+        else:
+            element_data = dict(element_data)
+            for key, value in element_data.items():
+                if isinstance(value, dict):
+                    value = value[self.etype]
+                    element_data[key] = value
+                assert value.shape[0] == E, f"the shape of {key} should be [{E}, ...], but got {value.shape[0]}"
+        scalar_data = {} if scalar_data is None else {k: torch.as_tensor(v) for k, v in scalar_data.items()}
+        if points is None:
+            points = self._points
+        else:
+            assert points.shape[1] == self.dimension, \
+                f"the dimension of points should be {self.dimension}, but got {points.shape[1]}"
+        if points.dtype not in (torch.float32, torch.float64):
+            raise Exception(f"the dtype {points.dtype} is not supported")
+        for key, value in point_data.items():
+            assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
+
+        fn = self.forward if func is None else func
+        match = match_bilinear(fn, self.dimension, element_data, point_data, scalar_data,
+                               owner=self if func is None else None)
+        if match is not None:
+            coeff = None
+            if match.coeff_key is not None:
+                coeff = (element_data if match.form == "diffusion_elem" else point_data)[match.coeff_key]
+            needs_grad = torch.is_grad_enabled() and (points.requires_grad or (coeff is not None and coeff.requires_grad))
+            if not needs_grad:
+                return self._native_bilinear(match, points, coeff)
+            from .autograd import adjoint_supported, assemble_with_adjoint
+            if adjoint_supported(self, match):
+                return self._wrap_output(assemble_with_adjoint(self, match, points, coeff), points.device)
+        point_data["x"] = points
+        return self._torch_bilinear(fn, points, point_data, element_data, scalar_data, batch_size)
+
+    # ---- native path ----
+    def _native_values(self, match: FormMatch, points: torch.Tensor, coeff: Optional[torch.Tensor]) -> torch.Tensor:
+        nat.require_device(points, "points")
+        L = nat.lib()
+        dev = points.device
+        points = points.detach().contiguous()
+        dtype = points.dtype
+        pat = self._pattern_on(dev)
+        conn = self._conn32(dev)
+        E, B = conn.shape
+        H = self.dimension if match.form == "linear_elasticity" else 1
+        if coeff is not None:
+            nat.require_device(coeff, match.coeff_key)
+            coeff = coeff.detach().to(dtype).contiguous()
+        tab, tab_dev = self._tables(dev)
+        slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
+        values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
+        mesh = self._mesh_view(points)
+        form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
+                            tab.ctypes.data, tab_dev.data_ptr(),
+                            None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
+        pv = pat.view()
+        nat.check(L.tm_assemble_bilinear(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(pv),
+                                         slot.data_ptr(), values.data_ptr(), nat.stream_of(dev)),
+                  "tm_assemble_bilinear")
+        return values if H == 1 else values.reshape(pat.nnz, H, H)
+
+    def _native_bilinear(self, match, points, coeff) -> SparseMatrix:
+        return self._wrap_output(self._native_values(match, points, coeff), points.device)
+
+    def _wrap_output(self, values: torch.Tensor, device) -> SparseMatrix:
+        pat = self._pattern_on(device)
+        if values.dim() == 1:
+            return SparseMatrix(values, pat.row, pat.col, (self.n_points, self.n_points),
+                                layout_hash=pat.hash, row_ptr=pat.row_ptr)
+        H = values.shape[1]
+
+        def make_block():
+            # from_block_coo index expansion (matrix.py:367-378), once per pattern
+            a = torch.arange(H, device=pat.row.device).repeat_interleave(H)
+            b = torch.arange(H, device=pat.row.device).repeat(H)
+            r = (pat.row[:, None] * H + a[None]).reshape(-1)
+            c = (pat.col[:, None] * H + b[None]).reshape(-1)
+            return r, c, layout_hash(r.cpu().numpy(), c.cpu().numpy())
+
+        r, c, h = self._cached(f"block{H}", torch.device(device), make_block)
+        return SparseMatrix(values.reshape(-1), r, c, (self.n_points * H, self.n_points * H), layout_hash=h)
+
+    # ---- generic torch path (element_assembler.py:320-445) ----
+    def _torch_bilinear(self, fn, points, point_data, element_data, scalar_data, batch_size) -> SparseMatrix:
+        etype = self.etype
+        conn = self.elements[etype].to(points.device)
+        params = list(inspect.signature(fn).parameters)
+        rules = [
+            (lambda x: x == "u", _Broadcast(False, True, True, False)),
+            (lambda x: x == "v", _Broadcast(False, True, False, True)),
+            (lambda x: x == "gradu", _Broadcast(True, True, True, False)),
+            (lambda x: x == "gradv", _Broadcast(True, True, False, True)),
+            (lambda x: x in element_data, _Broadcast(True, False, False, False)),
+            (lambda x: x in scalar_data, _Broadcast(True, True, True, True)),
+            (lambda x: x in point_data, _Broadcast(True, True, False, False)),
+            (lambda x: x in {"grad" + k for k in point_data}, _Broadcast(True, True, False, False)),
+        ]
+        dims = []
+        for key in params:
+            for cond, bc in rules:
+                if cond(key):
+                    dims.append(bc)
+                    break
+            else:
+                raise ValueError(f"{key} is not supported, please use `u`, `v`, `gradu`, `gradv` or more keys "
+                                 f"provided by point_data, element_data or scalar_data")
+        e_dims = tuple(d.element for d in dims)
+        q_dims = tuple(d.quadrature for d in dims)
+        u_dims = tuple(d.u for d in dims)
+        v_dims = tuple(d.v for d in dims)
+        inner = torch.vmap(torch.vmap(torch.vmap(fn, in_dims=v_dims), in_dims=u_dims), in_dims=q_dims)
+        element_parallel = not all(d is None for d in e_dims)
+        pfn = torch.vmap(inner, in_dims=e_dims) if element_parallel else inner
+        nq = _n_quad(etype, self.quadrature_order)
+        if batch_size in (-1, None):
+            n_batch, bs = 1, nq
+        else:
+            bs, n_batch = batch_size, math.ceil(nq / batch_size)
+        ele_pd = {k: v[conn] for k, v in point_data.items()}
+        total = None
+        for i in range(n_batch):
+            phi, grad, jxw = shape_grad_jxw(points, conn, etype, self.quadrature_order, i * bs, bs)
+            args = []
+            for key in params:
+                if key in ("u", "v"):
+                    args.append(phi)
+                elif key in ("gradu", "gradv"):
+                    args.append(grad)
+                elif key in ele_pd:
+                    args.append(torch.einsum("eb...,qb->eq...", ele_pd[key], phi))
+                elif key.startswith("grad") and key[4:] in ele_pd:
+                    args.append(torch.einsum("eb...,eqbd->eq...d", ele_pd[key[4:]], grad))
+                elif key in element_data:
+                    args.append(element_data[key])
+                elif key in scalar_data:
+                    args.append(scalar_data[key])
+            out = pfn(*args)
+            if element_parallel:
+                out = torch.einsum("eqij...,eq->eij...", out, jxw)
+            else:
+                out = torch.einsum("qij...,eq->eij...", out, jxw)
+            total = out if total is None else total + out
+        pat = self._pattern_on(points.device)
+        S = total.shape[0] * total.shape[1] * total.shape[2]
+        flat = total.reshape(S, *total.shape[3:])
+        values = torch.zeros(pat.nnz, *total.shape[3:], dtype=flat.dtype, device=flat.device)
+        values = values.index_add(0, pat.slot2nnz.long(), flat)
+        if values.dim() == 1:
+            return self._wrap_output(values, points.device)
+        if values.dim() == 3:
+            return self._wrap_output(values, points.device)
+        raise Exception(f"the shape of integral is supposed to be  1D or 3D, but got {values.shape}")
+
+    # ---- energy (element_assembler.py:447-659), torch path ----
+    def energy(self, points=None, func=None, point_data=None, element_data=None, scalar_data=None,
+               batch_size: int = -1):
+        point_data = {} if point_data is None else dict(point_data)
+        element_data = {} if element_data is None else dict(element_data)
+        scalar_data = {} if scalar_data is None else {k: torch.as_tensor(v) for k, v in scalar_data.items()}
+        points = self._points if points is None else points
+        point_data["x"] = points
+        fn = self.element_energy if func is None else func
+        params = list(inspect.signature(fn).parameters)
+        conn = self.elements[self.etype].to(points.device)
+        e_dims, q_dims = [], []
+        nq = _n_quad(self.etype, self.quadrature_order)
+        for key in params:
+            if key in element_data:
+                d = element_data[key]
+                e_dims.append(0)
+                q_dims.append(0 if (d.dim() >= 2 and d.shape[1] == nq) else None)
+            elif key in scalar_data:
+                e_dims.append(None)
+                q_dims.append(None)
+            elif key in point_data or (key.startswith("grad") and key[4:] in point_data):
+                e_dims.append(0)
+                q_dims.append(0)
+            else:
+                raise ValueError(f"{key} is not supported for energy calculation.")
+        pfn = torch.vmap(torch.vmap(fn, in_dims=tuple(q_dims)), in_dims=tuple(e_dims))
+        phi, grad, jxw = shape_grad_jxw(points, conn, self.etype, self.quadrature_order)
+        ele_pd = {k: v[conn] for k, v in point_data.items()}
+        args = []
+        for key in params:
+            if key in ele_pd:
+                args.append(torch.einsum("eb...,qb->eq...", ele_pd[key], phi))
+            elif key.startswith("grad") and key[4:] in ele_pd:
+                args.append(torch.einsum("eb...,eqbd->eq...d", ele_pd[key[4:]], grad))
+            elif key in element_data:
+                args.append(element_data[key])
+            else:
+                args.append(scalar_data[key])
+        return (pfn(*args) * jxw).sum()
+
+    def element_energy(self, **kwargs):
+        raise NotImplementedError("element_energy is not implemented")
+
+    def __repr__(self):
+        return (f"{self.__class__.__name__}(element_types={self.element_types}, "
+                f"n_element={self.elements[self.etype].shape[0]}, n_point={self.n_points}, "
+                f"n_dim={self.dimension}, nnz={self._pattern.nnz})")
+
+
+# -------------------------------------------------------- NodeAssembler ---
+class NodeAssembler(_AssemblerBase):
+    """Linear-form assembler (node_assembler.py:33-741)."""
+
+    def __init__(self, elements: Dict[str, torch.Tensor], points: torch.Tensor, incidence, quadrature_order: int = 2,
+                 *args, **kwargs):
+        super().__init__()
+        self._init_topology(elements, points, quadrature_order)
+        self._incidence = incidence  # (inc_ptr, inc) int32 on the mesh device
+        self._compile = False
+        self.__post_init__(*args, **kwargs)
+
+    def __post_init__(self, *args, **kwargs):
+        pass
+
+    def forward(self, *args):
+        raise NotImplementedError("forward is not implemented")
+
+    @classmethod
+    def from_mesh(cls, mesh, quadrature_order: int = 2, project: str = "reduce", *args, **kwargs):
+        elements = mesh.elements()
+        if isinstance(elements, torch.Tensor):
+            elements = {mesh.default_element_type: elements}
+        return cls.from_elements(mesh.points, elements, quadrature_order, mesh.device, mesh.dtype, project,
+                                 *args, **kwargs)
+
+    @classmethod
+    def from_elements(cls, points, elements, quadrature_order: int = 2, device="cpu", dtype=torch.float32,
+                      project: str = "reduce", *args, **kwargs):
+        if project not in ("reduce", "sparse"):
+            raise ValueError(f"project should be either 'reduce' or 'sparse', but got {project}")
+        if len(elements) != 1:
+            raise NotImplementedError("tensormesh_amd assembles one element type per mesh")
+        conn = next(iter(elements.values()))
+        L = nat.lib()
+        c = np.ascontiguousarray(conn.detach().cpu().numpy().astype(np.int32))
+        E, B = c.shape
+        n = int(points.shape[0])
+        inc_ptr = np.empty(n + 1, np.int32)
+        inc = np.empty(E * B, np.int32)
+        nat.check(L.tm_incidence_build(c.ctypes.data, E, B, n, inc_ptr.ctypes.data, inc.ctypes.data),
+                  "tm_incidence_build")
+        dev = torch.device(device)
+        incidence = (torch.from_numpy(inc_ptr).to(dev), torch.from_numpy(inc).to(dev))
+        return cls(elements, points.to(device=dev, dtype=dtype), incidence, quadrature_order, *args, **kwargs)
+
+    @classmethod
+    def from_assembler(cls, obj, *args, **kwargs):
+        assert isinstance(obj, NodeAssembler), f"the object {obj} should inheritate from NodeAssembler"
+        return cls(obj.elements, obj._points, obj._incidence, obj.quadrature_order, *args, **kwargs)
+
+    def compile(self, mode: str = "disable", **kwargs) -> "NodeAssembler":
+        """Accepted for API compatibility (node_assembler.py:416-500): the
+        built-in loads are compiled HIP kernels already."""
+        self._compile = True
+        return self
+
+    def flat_mode(self) -> "NodeAssembler":
+        return self.compile()
+
+    def reset_compile(self) -> "NodeAssembler":
+        self._compile = False
+        return self
+
+    @property
+    def is_compiled(self) -> bool:
+        return self._compile
+
+    def __call__(self, points: Optional[torch.Tensor] = None, func: Optional[Callable] = None,
+                 point_data: Optional[Dict[str, torch.Tensor]] = None,
+                 scalar_data: Optional[Dict[str, torch.Tensor]] = None, batch_size: int = 1) -> torch.Tensor:
+        point_data = {} if point_data is None else dict(point_data)
+        scalar_data = {} if scalar_data is None else dict(scalar_data)
+        if points is None:
+            points = self._points
+        else:
+            assert points.shape[1] == self.dimension, \
+                f"the dimension of the points should be {self.dimension}, but got {points.shape[1]}"
+        for key, value in point_data.items():
+            assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
+        fn = self.forward if func is None else func
+        match = match_linear(fn, point_data, scalar_data)
+        if match is not None:
+            f = None if match.key is None else point_data[match.key]
+            needs_grad = torch.is_grad_enabled() and (points.requires_grad or (f is not None and f.requires_grad))
+            if not needs_grad:
+                return self._native_linear(points, f, match.const)
+        point_data["x"] = points
+        return self._torch_linear(fn, points, point_data, scalar_data, batch_size)
+
+    def _native_linear(self, points, f, const) -> torch.Tensor:
+        nat.require_device(points, "points")
+        L = nat.lib()
+        dev = points.device
+        points = points.detach().contiguous()
+        conn = self._conn32(dev)
+        E, B = conn.shape
+        if f is not None:
+            nat.require_device(f, "point data")
+            f = f.detach().to(points.dtype).contiguous()
+        tab, tab_dev = self._tables(dev)
+        inc_ptr, inc = (t.to(dev) for t in self._incidence)
+        slot = torch.empty(E * B, dtype=points.dtype, device=dev)
+        out = torch.empty(points.shape[0], dtype=points.dtype, device=dev)
+        mesh = self._mesh_view(points)
+        form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
+                            None if f is None else f.data_ptr(), 0.0, 0.0)
+        iv = nat.IncidenceView(inc_ptr.data_ptr(), inc.data_ptr())
+        cval = ctypes.c_double(const)
+        nat.check(L.tm_assemble_linear(ctypes.byref(mesh), ctypes.byref(form),
+                                       ctypes.byref(cval) if f is None else None, ctypes.byref(iv),
+                                       slot.data_ptr(), out.data_ptr(), nat.stream_of(dev)),
+                  "tm_assemble_linear")
+        return out
+
+    def _torch_linear(self, fn, points, point_data, scalar_data, batch_size) -> torch.Tensor:
+        etype = self.etype
+        conn = self.elements[etype].to(points.device)
+        params = list(inspect.signature(fn).parameters)
+        rules = [
+            (lambda x: x == "v", (None, 0, 0)),
+            (lambda x: x == "gradv", (0, 0, 0)),
+            (lambda x: x in scalar_data, (None, None, None)),
+            (lambda x: x in point_data, (0, 0, None)),
+            (lambda x: x in {"grad" + k for k in point_data}, (0, 0, None)),
+        ]
+        dims = []
+        for key in params:
+            for cond, d in rules:
+                if cond(key):
+                    dims.append(d)
+                    break
+            else:
+                raise ValueError(f"{key} is not supported, please use  `v`, `gradv` or more keys provided by "
+                                 f"point_data, element_data or scalar_data")
+        e_dims = tuple(d[0] for d in dims)
+        q_dims = tuple(d[1] for d in dims)
+        v_dims = tuple(d[2] for d in dims)
+        element_parallel = not all(d is None for d in e_dims)
+        inner = torch.vmap(torch.vmap(fn, in_dims=v_dims), in_dims=q_dims)
+        pfn = torch.vmap(inner, in_dims=e_dims) if element_parallel else inner
+        nq = _n_quad(etype, self.quadrature_order)
+        if batch_size in (-1, None):
+            n_batch, bs = 1, nq
+        else:
+            bs, n_batch = batch_size, math.ceil(nq / batch_size)
+        ele_pd = {k: v[conn] for k, v in point_data.items()}
+        total = None
+        for i in range(n_batch):
+            phi, grad, jxw = shape_grad_jxw(points, conn, etype, self.quadrature_order, i * bs, bs)
+            args = []
+            for key in params:
+                if key == "v":
+                    args.append(phi)
+                elif key == "gradv":
+                    args.append(grad)
+                elif key in ele_pd:
+                    args.append(torch.einsum("eb...,qb->eq...", ele_pd[key], phi))
+                elif key.startswith("grad") and key[4:] in ele_pd:
+                    args.append(torch.einsum("eb...,eqbd->eq...d", ele_pd[key[4:]], grad))
+                elif key in scalar_data:
+                    args.append(scalar_data[key])
+            out = pfn(*args)
+            out = torch.einsum("eqb...,eq->eb...", out, jxw) if element_parallel else \
+                torch.einsum("qb...,eq->eb...", out, jxw)
+            total = out if total is None else total + out
+        flat = total.reshape(conn.numel(), *total.shape[2:])
+        res = torch.zeros(self.n_points, *total.shape[2:], dtype=flat.dtype, device=flat.device)
+        return res.index_add(0, conn.reshape(-1), flat).flatten()
+
+    def __repr__(self):
+        return (f"{self.__class__.__name__}(element_types={self.element_types}, "
+                f"n_element={self.elements[self.etype].shape[0]}, n_point={self.n_points}, n_dim={self.dimension})")

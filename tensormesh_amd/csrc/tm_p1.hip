@@ -1,0 +1,320 @@
+// P1 simplex assembly: stage G (element geometry + local matrix) and stage S
+// (deterministic segmented reduction into CSR entry order).
+//
+// Stage G replaces, for linear triangles and tetrahedra, the chain
+//   Element.eval_shape_grad      tensormesh/element/element.py:956-968
+//   Transformation jxw = w|detJ| tensormesh/element/transformation.py:1265-1272
+//   vmap'd form + _integrate     tensormesh/assemble/element_assembler.py:190-209,416-438
+// For P1 the reference gradients are constant, so J, J^-1 and |det J| are
+// per element: with edge vectors a_k = x_{k+1} - x_0 (the rows of the
+// reference's J[i][j] = sum_b x[b][j] dphi_b/dxi_i), the columns of J^-1 are
+// the cofactor vectors n_k / det, and grad(phi_{k+1}) = n_k / det,
+// grad(phi_0) = -sum_k grad(phi_{k+1}).  The quadrature sum of a constant
+// integrand is (sum_q w_q) * integrand, so
+//   laplace / diffusion:  K_ij = c_e * W * |det| * grad_i . grad_j
+//   mass:                 M_ij = |det| * sum_q w_q phi_i(q) phi_j(q)
+// where c_e is kappa[e], 1, or sum_b kappa[n_b] sum_q w_q phi_b(q) for a
+// nodal coefficient.  The local matrix is written to the slot buffer at
+// s = e*B*B + i*B + j (the reference's slot numbering,
+// element_assembler.py:811-819).
+//
+// Stage S replaces ReduceProjector.__call__ (projector.py:95-123): entry k sums
+// slot[seg_slot[t]] for t in [seg_ptr[k], seg_ptr[k+1]) from zero, in
+// ascending slot order, i.e. the order of the reference's sequential CPU
+// index_add_.  No atomics: the result is bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include "tm_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct P1Consts {
+  double W;           // sum_q w_q
+  double wphi[4];     // sum_q w_q phi_b(q)
+  double mref[16];    // sum_q w_q phi_i(q) phi_j(q), row-major B x B
+};
+
+template <int D>
+struct Simplex {
+  static constexpr int B = D + 1;
+};
+
+// Gather the B vertex coordinates of element e.
+template <typename T, int D>
+__device__ __forceinline__ void gather(const int32_t* __restrict__ conn, const T* __restrict__ pts,
+                                       int64_t e, int (&v)[D + 1], double (&x)[D + 1][D]) {
+  constexpr int B = D + 1;
+  if constexpr (D == 3) {
+    const int4 c = reinterpret_cast<const int4*>(conn)[e];
+    v[0] = c.x; v[1] = c.y; v[2] = c.z; v[3] = c.w;
+  } else {
+#pragma unroll
+    for (int b = 0; b < B; ++b) v[b] = conn[e * B + b];
+  }
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[b][d] = (double)pts[(int64_t)v[b] * D + d];
+  }
+}
+
+// Cofactor vectors n[k] (columns of adj(J)) and det J for the P1 simplex.
+template <int D>
+__device__ __forceinline__ double cofactors(const double (&x)[D + 1][D], double (&n)[D][D]) {
+  if constexpr (D == 3) {
+    double a[3], b[3], c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      a[d] = x[1][d] - x[0][d];
+      b[d] = x[2][d] - x[0][d];
+      c[d] = x[3][d] - x[0][d];
+    }
+    // columns of adj(J) for J with rows a, b, c
+    n[0][0] = b[1] * c[2] - b[2] * c[1];
+    n[0][1] = b[2] * c[0] - b[0] * c[2];
+    n[0][2] = b[0] * c[1] - b[1] * c[0];
+    n[1][0] = c[1] * a[2] - c[2] * a[1];
+    n[1][1] = c[2] * a[0] - c[0] * a[2];
+    n[1][2] = c[0] * a[1] - c[1] * a[0];
+    n[2][0] = a[1] * b[2] - a[2] * b[1];
+    n[2][1] = a[2] * b[0] - a[0] * b[2];
+    n[2][2] = a[0] * b[1] - a[1] * b[0];
+    return a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
+  } else {
+    const double a0 = x[1][0] - x[0][0], a1 = x[1][1] - x[0][1];
+    const double b0 = x[2][0] - x[0][0], b1 = x[2][1] - x[0][1];
+    n[0][0] = b1;  n[0][1] = -b0;
+    n[1][0] = -a1; n[1][1] = a0;
+    return a0 * b1 - a1 * b0;
+  }
+}
+
+// Local matrix of one element in registers (double), B*B entries.
+template <typename T, int D, int FORM>
+__device__ __forceinline__ void p1_local(const int32_t* __restrict__ conn, const T* __restrict__ pts,
+                                         const T* __restrict__ coeff, const P1Consts& c, int64_t e,
+                                         double (&K)[(D + 1) * (D + 1)]) {
+  constexpr int B = D + 1;
+  int v[B];
+  double x[B][D];
+  gather<T, D>(conn, pts, e, v, x);
+  double n[D][D];
+  const double det = cofactors<D>(x, n);
+  const double adet = fabs(det);
+  if constexpr (FORM == TM_FORM_MASS) {
+#pragma unroll
+    for (int k = 0; k < B * B; ++k) K[k] = adet * c.mref[k];
+    return;
+  } else {
+    // grad(phi) * det: g[0] = -sum n[k], g[k+1] = n[k]
+    double g[B][D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        g[k + 1][d] = n[k][d];
+        s += n[k][d];
+      }
+      g[0][d] = -s;
+    }
+    double coef = c.W;
+    if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
+      coef *= (double)coeff[e];
+    } else if constexpr (FORM == TM_FORM_DIFFUSION_NODE) {
+      double kq = 0.0;
+#pragma unroll
+      for (int b = 0; b < B; ++b) kq += (double)coeff[v[b]] * c.wphi[b];
+      coef = kq;
+    }
+    // c * W * |det| * (n_i/det).(n_j/det) = c * W * (n_i . n_j) / |det|
+    const double scale = coef / adet;
+#pragma unroll
+    for (int i = 0; i < B; ++i) {
+#pragma unroll
+      for (int j = i; j < B; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) s += g[i][d] * g[j][d];
+        K[i * B + j] = scale * s;
+        K[j * B + i] = scale * s;
+      }
+    }
+  }
+}
+
+template <typename T, int D, int FORM>
+__global__ void __launch_bounds__(kBlock) p1_element_kernel(const int32_t* __restrict__ conn,
+                                                            const T* __restrict__ pts,
+                                                            const T* __restrict__ coeff, P1Consts c,
+                                                            int64_t n_elem, T* __restrict__ slot) {
+  constexpr int B = D + 1;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n_elem) return;
+  double K[B * B];
+  p1_local<T, D, FORM>(conn, pts, coeff, c, e, K);
+  T* out = slot + e * (B * B);
+#pragma unroll
+  for (int k = 0; k < B * B; ++k) out[k] = (T)K[k];
+}
+
+template <typename T, int HH>
+__global__ void __launch_bounds__(kBlock) segmented_reduce_kernel(const int32_t* __restrict__ seg_ptr,
+                                                                  const int32_t* __restrict__ seg_slot,
+                                                                  const T* __restrict__ slot,
+                                                                  int64_t nnz, T* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nnz) return;
+  const int32_t b = seg_ptr[k], e = seg_ptr[k + 1];
+  T acc[HH];
+#pragma unroll
+  for (int h = 0; h < HH; ++h) acc[h] = T(0);
+  for (int32_t t = b; t < e; ++t) {
+    const int64_t s = seg_slot[t];
+#pragma unroll
+    for (int h = 0; h < HH; ++h) acc[h] += slot[s * HH + h];
+  }
+#pragma unroll
+  for (int h = 0; h < HH; ++h) out[k * HH + h] = acc[h];
+}
+
+int fill_consts(const tm_form_desc* f, int B, int D, P1Consts& c) {
+  c = P1Consts{};
+  if (!f->tab_host || f->n_quad <= 0) return tmh::fail(TM_ERR_ARG, "form tables missing");
+  const int Q = f->n_quad;
+  const double* w = f->tab_host;
+  const double* phi = f->tab_host + Q;
+  for (int q = 0; q < Q; ++q) {
+    c.W += w[q];
+    for (int b = 0; b < B; ++b) c.wphi[b] += w[q] * phi[q * B + b];
+    for (int i = 0; i < B; ++i)
+      for (int j = 0; j < B; ++j) c.mref[i * B + j] += w[q] * phi[q * B + i] * phi[q * B + j];
+  }
+  (void)D;
+  return TM_OK;
+}
+
+template <typename T, int D>
+int launch_p1(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream) {
+  constexpr int B = D + 1;
+  P1Consts c;
+  int st = fill_consts(f, B, D, c);
+  if (st) return st;
+  const int64_t grid = (m->n_elem + kBlock - 1) / kBlock;
+  if (grid == 0) return TM_OK;
+  const T* pts = static_cast<const T*>(m->points);
+  const T* coeff = static_cast<const T*>(f->coeff);
+  T* slot = static_cast<T*>(slot_ws);
+  switch (f->form) {
+    case TM_FORM_LAPLACE:
+      p1_element_kernel<T, D, TM_FORM_LAPLACE><<<grid, kBlock, 0, stream>>>(m->conn, pts, coeff, c, m->n_elem, slot);
+      break;
+    case TM_FORM_DIFFUSION_ELEM:
+      if (!coeff) return tmh::fail(TM_ERR_ARG, "diffusion form needs a coefficient");
+      p1_element_kernel<T, D, TM_FORM_DIFFUSION_ELEM><<<grid, kBlock, 0, stream>>>(m->conn, pts, coeff, c, m->n_elem, slot);
+      break;
+    case TM_FORM_DIFFUSION_NODE:
+      if (!coeff) return tmh::fail(TM_ERR_ARG, "diffusion form needs a coefficient");
+      p1_element_kernel<T, D, TM_FORM_DIFFUSION_NODE><<<grid, kBlock, 0, stream>>>(m->conn, pts, coeff, c, m->n_elem, slot);
+      break;
+    case TM_FORM_MASS:
+      p1_element_kernel<T, D, TM_FORM_MASS><<<grid, kBlock, 0, stream>>>(m->conn, pts, coeff, c, m->n_elem, slot);
+      break;
+    default:
+      return tmh::fail(TM_ERR_UNSUPPORTED, "form not available for P1 scalar assembly");
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("p1 element kernel launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+}
+
+template <typename T>
+int launch_reduce(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot, const void* slot,
+                  void* out, hipStream_t stream) {
+  const int64_t grid = (nnz + kBlock - 1) / kBlock;
+  if (grid == 0) return TM_OK;
+  const T* s = static_cast<const T*>(slot);
+  T* o = static_cast<T*>(out);
+  switch (hh) {
+    case 1: segmented_reduce_kernel<T, 1><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o); break;
+    case 4: segmented_reduce_kernel<T, 4><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o); break;
+    case 9: segmented_reduce_kernel<T, 9><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o); break;
+    default: return tmh::fail(TM_ERR_UNSUPPORTED, "block size must be 1, 2x2 or 3x3");
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("segmented reduce launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+}
+
+int check_mesh(const tm_mesh_view* m) {
+  if (!m) return tmh::fail(TM_ERR_ARG, "mesh is NULL");
+  if (tmh::cell_basis(m->cell) < 0) return tmh::fail(TM_ERR_ARG, "unknown cell type");
+  if (m->dtype != TM_F32 && m->dtype != TM_F64) return tmh::fail(TM_ERR_ARG, "unknown dtype");
+  if (m->n_elem < 0 || m->n_points < 0) return tmh::fail(TM_ERR_ARG, "negative size");
+  const int64_t B = tmh::cell_basis(m->cell);
+  if (m->n_elem * B * B * 9 >= tmh::kMaxIndex * 4 || m->n_points >= tmh::kMaxIndex)
+    return tmh::fail(TM_ERR_SIZE, "mesh too large for int32 indexing");
+  if (m->n_elem > 0 && (!m->conn || !m->points)) return tmh::fail(TM_ERR_ARG, "NULL conn / points");
+  return TM_OK;
+}
+
+}  // namespace
+
+// Defined in tm_p2.hip (P2 and block forms).
+int tm_p2_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
+                           hipStream_t stream, int* hh_out);
+
+extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
+                                   tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  int st = check_mesh(mesh);
+  if (st) return st;
+  if (!form) return tmh::fail(TM_ERR_ARG, "form is NULL");
+  if (mesh->n_elem > 0 && !slot_ws) return tmh::fail(TM_ERR_ARG, "slot workspace is NULL");
+  const bool p1_scalar = (mesh->cell == TM_TRI3 || mesh->cell == TM_TET4) && form->form != TM_FORM_LINEAR_ELASTICITY;
+  if (!p1_scalar) {
+    int hh = 0;
+    return tm_p2_element_matrices(mesh, form, slot_ws, stream, &hh);
+  }
+  if (mesh->dtype == TM_F64) {
+    return mesh->cell == TM_TET4 ? launch_p1<double, 3>(mesh, form, slot_ws, stream)
+                                 : launch_p1<double, 2>(mesh, form, slot_ws, stream);
+  }
+  return mesh->cell == TM_TET4 ? launch_p1<float, 3>(mesh, form, slot_ws, stream)
+                               : launch_p1<float, 2>(mesh, form, slot_ws, stream);
+  TM_TRY_END
+}
+
+extern "C" int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
+                                   const int32_t* seg_slot, const void* slot, void* out,
+                                   tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (nnz < 0 || nnz >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "nnz out of range");
+  if (nnz > 0 && (!seg_ptr || !seg_slot || !slot || !out)) return tmh::fail(TM_ERR_ARG, "NULL pointer");
+  if (dtype == TM_F64) return launch_reduce<double>(nnz, hh, seg_ptr, seg_slot, slot, out, stream);
+  if (dtype == TM_F32) return launch_reduce<float>(nnz, hh, seg_ptr, seg_slot, slot, out, stream);
+  return tmh::fail(TM_ERR_ARG, "unknown dtype");
+  TM_TRY_END
+}
+
+extern "C" int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc* form,
+                                    const tm_pattern_view* pat, void* slot_ws, void* values,
+                                    tm_stream_t stream) {
+  TM_TRY_BEGIN
+  if (!pat) return tmh::fail(TM_ERR_ARG, "pattern is NULL");
+  int st = tm_element_matrices(mesh, form, slot_ws, stream);
+  if (st) return st;
+  const int hh = form->form == TM_FORM_LINEAR_ELASTICITY ? tmh::cell_dim(mesh->cell) * tmh::cell_dim(mesh->cell) : 1;
+  return tm_segmented_reduce(mesh->dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, slot_ws, values, stream);
+  TM_TRY_END
+}

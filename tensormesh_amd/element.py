@@ -1,0 +1,156 @@
+"""Reference-element data for the simplices on the assembly path.
+
+* Quadrature rules on the reference triangle / tetrahedron, restated from
+  the reference tables (tensormesh/element/quadrature.py:91-160) for rule
+  orders 1-4.  ``quadrature_order`` is a rule *order*, not a point count:
+  triangle 1/2/3/4 -> 1/3/4/6 points, tetrahedron 1/2/3/4 -> 1/4/5/11 points;
+  the order-3 rules carry a negative weight.
+* Lagrange shape functions in closed form for P1 and P2, in the reference's
+  internal node order (vertices, then edge midpoints in the order of
+  ``Triangle.edge = [[1,2],[0,2],[0,1]]`` / ``Tetrahedron.edge =
+  [[2,3],[1,3],[1,2],[0,3],[0,2],[0,1]]``, tensormesh/element/element.py:1684
+  and :1949).  The reference obtains the same polynomials by inverting a
+  Vandermonde matrix (tensormesh/element/element.py:555-659).
+
+``reference_tables`` packs weights, values and reference gradients into the
+flat float64 layout that the HIP kernels read (documented in
+``include/tensormesh_amd.h``).
+"""
+
+from __future__ import annotations
+
+import functools
+from typing import Tuple
+
+import numpy as np
+import torch
+
+TRI_QUAD = {
+    1: ([0.5], [[0.3333333333333333, 0.3333333333333333]]),
+    2: ([0.16666666666666666] * 3,
+        [[0.16666666666666666, 0.16666666666666666],
+         [0.6666666666666666, 0.16666666666666666],
+         [0.16666666666666666, 0.6666666666666666]]),
+    3: ([-0.28125, 0.260416666666667, 0.260416666666667, 0.260416666666667],
+        [[0.333333333333333, 0.333333333333333], [0.2, 0.6], [0.6, 0.2], [0.2, 0.2]]),
+    4: ([0.111690794839006, 0.054975871827661, 0.111690794839006,
+         0.054975871827661, 0.111690794839006, 0.054975871827661],
+        [[0.445948490915965, 0.10810301816807], [0.0915762135097699, 0.816847572980459],
+         [0.10810301816807, 0.445948490915965], [0.816847572980459, 0.091576213509771],
+         [0.445948490915965, 0.445948490915965], [0.091576213509771, 0.0915762135097699]]),
+}
+
+TET_QUAD = {
+    1: ([0.16666666666666666], [[0.25, 0.25, 0.25]]),
+    2: ([0.041666666666666664] * 4,
+        [[0.5854101966249685, 0.1381966011250105, 0.1381966011250105],
+         [0.1381966011250105, 0.1381966011250105, 0.1381966011250105],
+         [0.1381966011250105, 0.1381966011250105, 0.5854101966249685],
+         [0.1381966011250105, 0.5854101966249685, 0.1381966011250105]]),
+    3: ([-0.13333333333333333, 0.075, 0.075, 0.075, 0.075],
+        [[0.25, 0.25, 0.25], [0.5, 0.1666666666666667, 0.1666666666666667],
+         [0.1666666666666667, 0.1666666666666667, 0.1666666666666667],
+         [0.1666666666666667, 0.1666666666666667, 0.5],
+         [0.1666666666666667, 0.5, 0.1666666666666667]]),
+    4: ([-0.01315555555555555] + [0.007622222222222217] * 4 + [0.02488888888888888] * 6,
+        [[0.25, 0.25, 0.25],
+         [0.7857142857142857, 0.0714285714285714, 0.0714285714285714],
+         [0.0714285714285714, 0.0714285714285714, 0.0714285714285714],
+         [0.0714285714285714, 0.0714285714285714, 0.7857142857142857],
+         [0.0714285714285714, 0.7857142857142857, 0.0714285714285714],
+         [0.1005964238332008, 0.3994035761667992, 0.3994035761667992],
+         [0.3994035761667992, 0.1005964238332008, 0.3994035761667992],
+         [0.3994035761667992, 0.3994035761667992, 0.1005964238332008],
+         [0.3994035761667992, 0.1005964238332008, 0.1005964238332008],
+         [0.1005964238332008, 0.3994035761667992, 0.1005964238332008],
+         [0.1005964238332008, 0.1005964238332008, 0.3994035761667992]]),
+}
+
+ELEMENT_SHAPE = {
+    # element type: (dim, order, n_basis, simplex family)
+    "triangle": (2, 1, 3, "tri"),
+    "triangle6": (2, 2, 6, "tri"),
+    "tetra": (3, 1, 4, "tet"),
+    "tetra10": (3, 2, 10, "tet"),
+}
+
+
+def quadrature(element_type: str, order: int) -> Tuple[np.ndarray, np.ndarray]:
+    """(weights[Q], points[Q, D]) as float64 numpy arrays."""
+    fam = ELEMENT_SHAPE[element_type][3]
+    table = TRI_QUAD if fam == "tri" else TET_QUAD
+    if order not in table:
+        raise NotImplementedError(
+            f"quadrature_order={order} for {element_type}: supported orders are {sorted(table)}")
+    w, p = table[order]
+    return np.asarray(w, dtype=np.float64), np.asarray(p, dtype=np.float64)
+
+
+def _barycentric(points: np.ndarray) -> np.ndarray:
+    return np.concatenate([1.0 - points.sum(1, keepdims=True), points], 1)  # [Q, D+1]
+
+
+def shape_values(element_type: str, points: np.ndarray) -> np.ndarray:
+    """phi[Q, B] at reference points."""
+    dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    lam = _barycentric(points)
+    if order == 1:
+        return lam.copy()
+    edges = ((1, 2), (0, 2), (0, 1)) if fam == "tri" else ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))
+    vert = lam * (2.0 * lam - 1.0)
+    mid = np.stack([4.0 * lam[:, a] * lam[:, b] for a, b in edges], 1)
+    return np.concatenate([vert, mid], 1)
+
+
+def shape_ref_grads(element_type: str, points: np.ndarray) -> np.ndarray:
+    """dphi/dxi [Q, D, B] at reference points (layout of the reference's
+    ``Polynomials.map`` output, tensormesh/element/element.py:960)."""
+    dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    q = points.shape[0]
+    lam = _barycentric(points)
+    # d lambda_k / d xi_i : lambda_0 = 1 - sum(xi), lambda_k = xi_{k-1}
+    dlam = np.zeros((dim, dim + 1))
+    dlam[:, 0] = -1.0
+    for i in range(dim):
+        dlam[i, i + 1] = 1.0
+    out = np.zeros((q, dim, nb))
+    if order == 1:
+        out[:] = dlam[None]
+        return out
+    edges = ((1, 2), (0, 2), (0, 1)) if fam == "tri" else ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))
+    for k in range(dim + 1):
+        out[:, :, k] = (4.0 * lam[:, k:k + 1] - 1.0) * dlam[None, :, k]
+    for m, (a, b) in enumerate(edges):
+        out[:, :, dim + 1 + m] = 4.0 * (lam[:, b:b + 1] * dlam[None, :, a] + lam[:, a:a + 1] * dlam[None, :, b])
+    return out
+
+
+@functools.lru_cache(maxsize=None)
+def reference_tables(element_type: str, qorder: int) -> np.ndarray:
+    """Flat float64 table read by the kernels.
+
+    Layout (Q quadrature points, B basis functions, D dims)::
+
+        [0, Q)                 w[q]
+        [Q, Q + Q*B)           phi[q][b]
+        [Q + Q*B, Q + Q*B + Q*D*B)   dphi[q][d][b]   (d/dxi_d)
+    """
+    w, p = quadrature(element_type, qorder)
+    phi = shape_values(element_type, p)
+    dphi = shape_ref_grads(element_type, p)
+    return np.concatenate([w, phi.ravel(), dphi.ravel()]).astype(np.float64)
+
+
+def mass_reference(element_type: str, qorder: int) -> np.ndarray:
+    """Mref[B, B] = sum_q w_q phi_i(q) phi_j(q) on the reference element."""
+    w, p = quadrature(element_type, qorder)
+    phi = shape_values(element_type, p)
+    return np.einsum("q,qi,qj->ij", w, phi, phi)
+
+
+def torch_tables(element_type: str, qorder: int, dtype, device):
+    """(w[Q], phi[Q,B], dphi[Q,D,B]) as torch tensors for the torch form path."""
+    w, p = quadrature(element_type, qorder)
+    return (torch.tensor(w, dtype=dtype, device=device),
+            torch.tensor(shape_values(element_type, p), dtype=dtype, device=device),
+            torch.tensor(shape_ref_grads(element_type, p), dtype=dtype, device=device))

@@ -1,0 +1,91 @@
+"""Sparsity pattern + slot permutation of one element block.
+
+``Pattern.build(conn, n_points)`` is the setup half of
+``ElementAssembler.from_mesh`` (tensormesh/assemble/element_assembler.py:804-853):
+the CSR of the sorted unique ``(conn[e,i], conn[e,j])`` pairs and, for each
+entry, the ascending list of slots ``s = e*B*B + i*B + j`` that the
+reference's ``ReduceProjector`` sums into it.  The integer work runs in
+libtmhip's host pattern builder; results are moved to the mesh device once
+and cached by the assembler.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import hashlib
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+
+def layout_hash(row: np.ndarray, col: np.ndarray) -> str:
+    """SHA-256 of the int64 row bytes followed by the int64 col bytes, the
+    digest ``SparseMatrix.__init__`` computes (tensormesh/sparse/matrix.py:81-86)."""
+    return hashlib.sha256(np.ascontiguousarray(row, dtype=np.int64).tobytes()
+                          + np.ascontiguousarray(col, dtype=np.int64).tobytes()).hexdigest()
+
+
+@dataclass
+class Pattern:
+    n_points: int
+    n_elem: int
+    n_basis: int
+    nnz: int
+    row_ptr: torch.Tensor      # int32 [N+1]
+    col_idx: torch.Tensor      # int32 [nnz]
+    seg_ptr: torch.Tensor      # int32 [nnz+1]
+    seg_slot: torch.Tensor     # int32 [S]
+    slot2nnz: torch.Tensor     # int32 [S]
+    row: torch.Tensor          # int64 [nnz]
+    col: torch.Tensor          # int64 [nnz]
+    hash: str
+    inc_ptr: Optional[torch.Tensor] = None   # int32 [N+1]
+    inc: Optional[torch.Tensor] = None       # int32 [E*B]
+
+    @staticmethod
+    def build(conn: torch.Tensor, n_points: int, device=None, incidence: bool = True) -> "Pattern":
+        L = nat.lib()
+        c = np.ascontiguousarray(conn.detach().cpu().numpy().astype(np.int32, copy=False))
+        E, B = c.shape
+        nnz = ctypes.c_int64(0)
+        nat.check(L.tm_pattern_nnz(c.ctypes.data, E, B, n_points, ctypes.byref(nnz)), "tm_pattern_nnz")
+        nnz = int(nnz.value)
+        S = E * B * B
+        row_ptr = np.empty(n_points + 1, np.int32)
+        col_idx = np.empty(nnz, np.int32)
+        seg_ptr = np.empty(nnz + 1, np.int32)
+        seg_slot = np.empty(S, np.int32)
+        slot2nnz = np.empty(S, np.int32)
+        nat.check(L.tm_pattern_build(c.ctypes.data, E, B, n_points, row_ptr.ctypes.data, col_idx.ctypes.data,
+                                     seg_ptr.ctypes.data, seg_slot.ctypes.data, slot2nnz.ctypes.data),
+                  "tm_pattern_build")
+        row = np.repeat(np.arange(n_points, dtype=np.int64), np.diff(row_ptr))
+        col = col_idx.astype(np.int64)
+        h = layout_hash(row, col)
+        inc_ptr = inc = None
+        if incidence:
+            inc_ptr_np = np.empty(n_points + 1, np.int32)
+            inc_np = np.empty(E * B, np.int32)
+            nat.check(L.tm_incidence_build(c.ctypes.data, E, B, n_points, inc_ptr_np.ctypes.data,
+                                           inc_np.ctypes.data), "tm_incidence_build")
+            inc_ptr, inc = torch.from_numpy(inc_ptr_np), torch.from_numpy(inc_np)
+        dev = conn.device if device is None else torch.device(device)
+        t = lambda a: torch.from_numpy(a).to(dev)
+        return Pattern(n_points, E, B, nnz, t(row_ptr), t(col_idx), t(seg_ptr), t(seg_slot), t(slot2nnz),
+                       t(row), t(col), h,
+                       None if inc_ptr is None else inc_ptr.to(dev), None if inc is None else inc.to(dev))
+
+    def to(self, device) -> "Pattern":
+        dev = torch.device(device)
+        mv = lambda x: None if x is None else x.to(dev)
+        return Pattern(self.n_points, self.n_elem, self.n_basis, self.nnz, mv(self.row_ptr), mv(self.col_idx),
+                       mv(self.seg_ptr), mv(self.seg_slot), mv(self.slot2nnz), mv(self.row), mv(self.col),
+                       self.hash, mv(self.inc_ptr), mv(self.inc))
+
+    def view(self) -> nat.PatternView:
+        return nat.PatternView(self.nnz, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
+                               self.seg_ptr.data_ptr(), self.seg_slot.data_ptr())

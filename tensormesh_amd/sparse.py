@@ -1,0 +1,215 @@
+"""COO sparse matrix with the FEM helpers of the reference's ``SparseMatrix``.
+
+Mirrors tensormesh/sparse/matrix.py:26-643 (``edata``/``row``/``col``/
+``shape``, ``layout_hash``, ``has_same_layout``, ``from_block_coo``,
+``to_dense``, ``@``, ``solve``) without its ``torch_sla.SparseTensor`` base.
+Differences that matter for the hot path:
+
+* the SHA-256 ``layout_hash`` is the same digest as the reference's
+  (int64 row bytes + int64 col bytes) but is computed lazily, or handed over
+  by the assembler that built the pattern once, instead of copying
+  ``row``/``col`` to the host on every construction (matrix.py:81-86);
+* ``row_ptr`` (int32 CSR) travels with matrices whose entries are in CSR
+  order, which the native SpMV and solver use.
+"""
+
+from __future__ import annotations
+
+import hashlib
+from typing import Optional, Tuple
+
+import numpy as np
+import scipy.sparse
+import torch
+
+
+class SparseMatrix:
+    def __init__(self, edata: torch.Tensor, row: torch.Tensor, col: torch.Tensor,
+                 shape: Tuple[int, int], *, layout_hash: Optional[str] = None,
+                 row_ptr: Optional[torch.Tensor] = None):
+        assert edata.dim() == 1, f"edata must be 1D, got {tuple(edata.shape)}"
+        assert row.shape == edata.shape and col.shape == edata.shape, \
+            f"row/col/edata must have the same length, got {row.shape}, {col.shape}, {edata.shape}"
+        self.values = edata
+        self.row_indices = row.long()
+        self.col_indices = col.long()
+        self.shape = (int(shape[0]), int(shape[1]))
+        self._layout_hash = layout_hash
+        self.row_ptr = row_ptr
+
+    # ---- reference-compatible accessors (matrix.py:88-120) ----
+    @property
+    def edata(self) -> torch.Tensor:
+        return self.values
+
+    @property
+    def row(self) -> torch.Tensor:
+        return self.row_indices
+
+    @property
+    def col(self) -> torch.Tensor:
+        return self.col_indices
+
+    @property
+    def edges(self) -> torch.Tensor:
+        return torch.stack([self.row_indices, self.col_indices], 0)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.values.numel())
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self.values.dtype
+
+    @property
+    def device(self) -> torch.device:
+        return self.values.device
+
+    @property
+    def layout_hash(self) -> str:
+        if self._layout_hash is None:
+            self._layout_hash = hashlib.sha256(
+                self.row_indices.detach().cpu().numpy().astype(np.int64).tobytes()
+                + self.col_indices.detach().cpu().numpy().astype(np.int64).tobytes()).hexdigest()
+        return self._layout_hash
+
+    def has_same_layout(self, other) -> bool:
+        if isinstance(other, str):
+            return self.layout_hash == other
+        if isinstance(other, SparseMatrix):
+            return self.layout_hash == other.layout_hash
+        raise TypeError(f"Expected str or SparseMatrix, got {type(other)}")
+
+    @property
+    def grad(self) -> Optional["SparseMatrix"]:
+        if self.values.grad is None:
+            return None
+        return self._like(self.values.grad)
+
+    def _like(self, values: torch.Tensor) -> "SparseMatrix":
+        return SparseMatrix(values, self.row_indices, self.col_indices, self.shape,
+                            layout_hash=self._layout_hash, row_ptr=self.row_ptr)
+
+    # ---- conversions ----
+    def to_dense(self) -> torch.Tensor:
+        out = torch.zeros(self.shape, dtype=self.dtype, device=self.device)
+        return out.index_put((self.row_indices, self.col_indices), self.values, accumulate=True)
+
+    def to_scipy_coo(self) -> scipy.sparse.coo_matrix:
+        return scipy.sparse.coo_matrix((self.values.detach().cpu().numpy(),
+                                        (self.row_indices.cpu().numpy(), self.col_indices.cpu().numpy())),
+                                       shape=self.shape)
+
+    def to_sparse_coo(self) -> torch.Tensor:
+        return torch.sparse_coo_tensor(self.edges, self.values, self.shape)
+
+    @property
+    def layout_mask(self) -> torch.Tensor:
+        mask = torch.zeros(self.shape, device=self.device, dtype=self.dtype)
+        mask[self.row_indices, self.col_indices] = 1
+        return mask
+
+    def degree(self, axis: int = 0) -> torch.Tensor:
+        idx = self.row_indices if axis == 0 else self.col_indices
+        return torch.bincount(idx, minlength=self.shape[axis])
+
+    def transpose(self) -> "SparseMatrix":
+        return SparseMatrix(self.values, self.col_indices, self.row_indices, (self.shape[1], self.shape[0]))
+
+    @property
+    def T(self) -> "SparseMatrix":
+        return self.transpose()
+
+    # ---- device / dtype (type-preserving, matrix.py:174-214) ----
+    def to(self, *args, **kwargs) -> "SparseMatrix":
+        v = self.values.to(*args, **kwargs)
+        dev = v.device
+        rp = None if self.row_ptr is None else self.row_ptr.to(dev)
+        return SparseMatrix(v, self.row_indices.to(dev), self.col_indices.to(dev), self.shape,
+                            layout_hash=self._layout_hash, row_ptr=rp)
+
+    def cuda(self, device=None) -> "SparseMatrix":
+        return self.to("cuda" if device is None else device)
+
+    def cpu(self) -> "SparseMatrix":
+        return self.to("cpu")
+
+    def double(self) -> "SparseMatrix":
+        return self._like(self.values.double())
+
+    def float(self) -> "SparseMatrix":
+        return self._like(self.values.float())
+
+    def detach(self) -> "SparseMatrix":
+        return self._like(self.values.detach())
+
+    # ---- arithmetic on a shared layout ----
+    def _check_same(self, other: "SparseMatrix") -> None:
+        if self.shape != other.shape or not self.has_same_layout(other):
+            raise ValueError("arithmetic between SparseMatrix objects needs an identical layout")
+
+    def __add__(self, other):
+        if isinstance(other, SparseMatrix):
+            self._check_same(other)
+            return self._like(self.values + other.values)
+        return NotImplemented
+
+    def __sub__(self, other):
+        if isinstance(other, SparseMatrix):
+            self._check_same(other)
+            return self._like(self.values - other.values)
+        return NotImplemented
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float)) or (torch.is_tensor(other) and other.dim() == 0):
+            return self._like(self.values * other)
+        return NotImplemented
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return self._like(-self.values)
+
+    def __matmul__(self, x: torch.Tensor) -> torch.Tensor:
+        from .solve import spmv
+        return spmv(self, x)
+
+    def solve(self, b: torch.Tensor, **kwargs) -> torch.Tensor:
+        """Solve ``A x = b`` for a symmetric positive definite ``A``.
+
+        Jacobi-preconditioned conjugate gradients on the device; see
+        ``tensormesh_amd.solve.cg``.  Differentiable in ``values`` and ``b``.
+        """
+        from .solve import solve
+        return solve(self, b, **kwargs)
+
+    # ---- constructors (matrix.py:331-381) ----
+    @staticmethod
+    def from_block_coo(edata: torch.Tensor, row: torch.Tensor, col: torch.Tensor,
+                       shape: Tuple[int, int]) -> "SparseMatrix":
+        """Expand block-COO ``[n, H, H]`` into scalar COO: value index
+        ``edge*H*H + a*H + b`` at (``row*H + a``, ``col*H + b``)."""
+        H = edata.shape[1]
+        assert edata.shape[2] == H, f"from_block_coo expects square blocks, got {tuple(edata.shape)}"
+        a = torch.arange(H, device=row.device).repeat_interleave(H)
+        b = torch.arange(H, device=row.device).repeat(H)
+        r = (row.long()[:, None] * H + a[None]).reshape(-1)
+        c = (col.long()[:, None] * H + b[None]).reshape(-1)
+        return SparseMatrix(edata.reshape(-1), r, c, (shape[0] * H, shape[1] * H))
+
+    @staticmethod
+    def from_dense(tensor: torch.Tensor) -> "SparseMatrix":
+        assert tensor.dim() == 2
+        rows, cols = torch.where(tensor != 0)
+        return SparseMatrix(tensor[rows, cols], rows, cols, tuple(tensor.shape))
+
+    @staticmethod
+    def from_scipy_coo(matrix, device="cpu", dtype=torch.float64) -> "SparseMatrix":
+        m = matrix.tocoo()
+        return SparseMatrix(torch.tensor(m.data, dtype=dtype, device=device),
+                            torch.tensor(m.row.astype(np.int64), device=device),
+                            torch.tensor(m.col.astype(np.int64), device=device), m.shape)
+
+    def __repr__(self) -> str:
+        return f"SparseMatrix(shape={self.shape}, nnz={self.nnz}, dtype={self.dtype}, device={self.device})"

@@ -1,0 +1,183 @@
+"""Parity of the HIP path with the CPU oracle and the golden fixtures.
+
+Tolerances (float64 path): the sparsity pattern is bit-exact; values agree
+entrywise within rtol 1e-12 plus an absolute floor of 1e-14 * max|K| for
+entries that are sums with cancellation (structural near-zeros).
+"""
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import tensormesh_amd as tm
+from oracle import fem
+from tensormesh_amd import _native as nat
+from tensormesh_amd.mesh import Mesh, lognormal_coefficient, quadratic_arrays, unit_cube_tet, unit_cube_tet_arrays
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+DEV = "cuda"
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def close64(a, b, rtol=1e-12):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    scale = b.abs().max().item()
+    err = (a - b).abs()
+    ok = bool((err <= rtol * b.abs() + 1e-14 * scale).all())
+    return ok, (err.max().item() / max(scale, 1e-300))
+
+
+def test_native_library_is_loaded():
+    L = nat.lib()
+    assert Path(L._name).resolve() == nat.LIB_PATH
+
+
+def test_cfg1_laplace_load_condense_against_golden():
+    say("case cfg1 tri20 laplace/load/condense")
+    d = np.load(GOLD / "cfg1_tri20.npz")
+    mesh = Mesh(torch.from_numpy(d["points"]), {"triangle": torch.from_numpy(d["cells"])},
+                torch.from_numpy(d["boundary"])).to(DEV)
+
+    class LaplaceAssembler(tm.ElementAssembler):
+        def forward(self, gradu, gradv):
+            return gradu @ gradv
+
+    class SourceAssembler(tm.NodeAssembler):
+        def forward(self, v, f):
+            return f * v
+
+    K = LaplaceAssembler.from_mesh(mesh)()
+    assert K.device.type == "cuda"
+    assert np.array_equal(K.row.cpu().numpy(), d["row"]) and np.array_equal(K.col.cpu().numpy(), d["col"])
+    ok, err = close64(K.values, torch.from_numpy(d["K"]))
+    assert ok, err
+    x, y = mesh.points[:, 0], mesh.points[:, 1]
+    f = 2 * math.pi ** 2 * torch.sin(math.pi * x) * torch.sin(math.pi * y)
+    b = SourceAssembler.from_mesh(mesh)(point_data={"f": f})
+    ok, err = close64(b, torch.from_numpy(d["b"]))
+    assert ok, err
+    cond = tm.Condenser(mesh.boundary_mask)
+    Ki, fi = cond(K, b)
+    assert np.array_equal(Ki.row.cpu().numpy(), d["Kii_row"]) and np.array_equal(Ki.col.cpu().numpy(), d["Kii_col"])
+    assert torch.equal(Ki.values.cpu(), K.values.cpu()[cond.inner_idx.cpu()])
+    ok, err = close64(Ki.values, torch.from_numpy(d["Kii"]))
+    assert ok, err
+    ok, err = close64(fi, torch.from_numpy(d["f_i"]))
+    assert ok, err
+    u = cond.recover(Ki.solve(fi, rtol=1e-12))
+    u_exact = torch.sin(math.pi * x) * torch.sin(math.pi * y)
+    rel = ((u - u_exact).norm() / u_exact.norm()).item()
+    say(f"cfg1 L2 error {rel:.3e}")
+    assert rel < 1e-2
+
+
+@pytest.mark.parametrize("q", [1, 2])
+def test_tet_diffusion_and_mass_against_golden(q):
+    say(f"case tet3 diffusion/mass qorder {q}")
+    d = np.load(GOLD / "tet3_diffusion.npz")
+    mesh = Mesh(torch.from_numpy(d["points"]), {"tetra": torch.from_numpy(d["cells"])}).to(DEV)
+    ke = torch.from_numpy(d["kappa_e"]).to(DEV)
+    kn = torch.from_numpy(d["kappa_n"]).to(DEV)
+    A = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=q)
+    Ke = A(element_data={"kappa": ke})
+    assert np.array_equal(Ke.row.cpu().numpy(), d["row"]) and np.array_equal(Ke.col.cpu().numpy(), d["col"])
+    ok, err = close64(Ke.values, torch.from_numpy(d[f"diff_e_q{q}"]))
+    assert ok, err
+    Kn = A(point_data={"kappa": kn})
+    ok, err = close64(Kn.values, torch.from_numpy(d[f"diff_n_q{q}"]))
+    assert ok, err
+    M = tm.MassElementAssembler.from_mesh(mesh, quadrature_order=q)()
+    ok, err = close64(M.values, torch.from_numpy(d[f"mass_q{q}"]))
+    assert ok, err
+
+
+def test_tet10_elasticity_against_golden():
+    say("case tet10 linear elasticity")
+    d = np.load(GOLD / "tet10_elastic.npz")
+    mesh = Mesh(torch.from_numpy(d["points"]), {"tetra10": torch.from_numpy(d["cells"])}).to(DEV)
+    K = tm.LinearElasticityElementAssembler.from_mesh(mesh, quadrature_order=2, E=1.0, nu=0.3)()
+    assert np.array_equal(K.row.cpu().numpy(), d["row"]) and np.array_equal(K.col.cpu().numpy(), d["col"])
+    ok, err = close64(K.values, torch.from_numpy(d["values"]))
+    assert ok, err
+
+
+@pytest.mark.parametrize("etype,form,q", [("triangle6", "laplace", 2), ("triangle6", "mass", 4),
+                                          ("tetra10", "laplace", 2), ("tetra10", "mass", 3),
+                                          ("tetra", "linear_elasticity", 2),
+                                          ("tetra10", "diffusion_node", 2)])
+def test_p2_and_block_forms_against_oracle(etype, form, q):
+    say(f"case {etype} {form} q{q}")
+    order = 2 if etype.endswith(("6", "10")) else 1
+    if etype.startswith("tri"):
+        from tensormesh_amd.mesh import unit_square_tri
+        m = unit_square_tri(4, 0.15, 11, order=order)
+    else:
+        m = unit_cube_tet(2, 0.1, 11, order=order)
+    kap = torch.from_numpy(lognormal_coefficient(m.n_points, 3))
+    ref = fem.assemble_bilinear(m.points, m.elements(), etype, q, form, kappa=kap, E=1.0, nu=0.3)
+    md = m.to(DEV)
+    if form == "laplace":
+        K = tm.LaplaceElementAssembler.from_mesh(md, quadrature_order=q)()
+    elif form == "mass":
+        K = tm.MassElementAssembler.from_mesh(md, quadrature_order=q)()
+    elif form == "diffusion_node":
+        K = tm.DiffusionElementAssembler.from_mesh(md, quadrature_order=q)(point_data={"kappa": kap.to(DEV)})
+    else:
+        K = tm.LinearElasticityElementAssembler.from_mesh(md, quadrature_order=q, E=1.0, nu=0.3)()
+    assert torch.equal(K.row.cpu(), ref[1]) and torch.equal(K.col.cpu(), ref[2])
+    ok, err = close64(K.values, ref[0])
+    assert ok, err
+
+
+def _cfg2(dtype=torch.float64):
+    p, c, bnd = unit_cube_tet_arrays(64, 0.15, 0)
+    kap = lognormal_coefficient(c.shape[0], 1)
+    return p, c, kap
+
+
+def test_cfg2_full_size_fp64_against_oracle():
+    say("case cfg2 64^3 Kuhn tets fp64 vs oracle")
+    p, c, kap = _cfg2()
+    P, C, Kp = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(kap)
+    ref_v, ref_r, ref_c, _ = fem.assemble_bilinear(P, C, "tetra", 1, "diffusion_elem", kappa=Kp)
+    mesh = Mesh(P, {"tetra": C}).to(DEV)
+    A = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=1)
+    K = A(element_data={"kappa": Kp.to(DEV)})
+    assert K.nnz == 4018753
+    assert torch.equal(K.row.cpu(), ref_r) and torch.equal(K.col.cpu(), ref_c)
+    ok, err = close64(K.values, ref_v)
+    say(f"cfg2 fp64 max rel err {err:.3e}")
+    assert ok, err
+    K2 = A(element_data={"kappa": Kp.to(DEV)})
+    assert torch.equal(K.values, K2.values), "assembly is not bitwise reproducible"
+    # row sums vanish (constants are in the kernel of the diffusion operator)
+    rs = torch.zeros(K.shape[0], dtype=torch.float64, device=DEV).index_add(0, K.row, K.values)
+    assert rs.abs().max().item() < 1e-12 * K.values.abs().max().item()
+
+
+def test_cfg2_fp32_error_bound_against_fp64_oracle():
+    say("case cfg2 fp32 vs fp64 oracle")
+    p, c, kap = _cfg2()
+    # the fp32 path sees fp32 inputs; the fp64 oracle runs on the same (fp32-representable) inputs
+    p32, k32 = p.astype(np.float32), kap.astype(np.float32)
+    P, C, Kp = torch.from_numpy(p32.astype(np.float64)), torch.from_numpy(c), torch.from_numpy(k32.astype(np.float64))
+    ref_v, _, _, _ = fem.assemble_bilinear(P, C, "tetra", 1, "diffusion_elem", kappa=Kp)
+    mesh = Mesh(torch.from_numpy(p32), {"tetra": C}).to(DEV)
+    K = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=1)(element_data={"kappa": torch.from_numpy(k32).to(DEV)})
+    assert K.dtype == torch.float32
+    d = (K.values.double().cpu() - ref_v)
+    normwise = (d.norm() / ref_v.norm()).item()
+    maxrel = (d.abs().max() / ref_v.abs().max()).item()
+    big = ref_v.abs() > 1e-3 * ref_v.abs().max()
+    entry = (d[big].abs() / ref_v[big].abs()).max().item()
+    say(f"cfg2 fp32: normwise {normwise:.2e}  max|d|/max|K| {maxrel:.2e}  entrywise(|K|>1e-3max) {entry:.2e}")
+    assert normwise < 1e-6 and maxrel < 1e-6 and entry < 1e-5

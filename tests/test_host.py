@@ -1,0 +1,180 @@
+"""CPU tests of the host logic: the C-ABI library, the host pattern builder,
+form recognition, the generic torch form path and the API containers."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import tensormesh_amd as tm
+from oracle import fem
+from tensormesh_amd import _native as nat
+from tensormesh_amd.forms import match_bilinear, match_linear
+from tensormesh_amd.mesh import quadratic_arrays, unit_cube_tet, unit_cube_tet_arrays, unit_square_tri
+from tensormesh_amd.pattern import Pattern, layout_hash
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_header_symbol():
+    L = nat.lib()
+    header = (ROOT / "include" / "tensormesh_amd.h").read_text()
+    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(tm_\w+)\s*\(", header, flags=re.M))
+    assert names, "no declarations parsed"
+    assert names == set(nat.SIGNATURES), names ^ set(nat.SIGNATURES)
+    for n in names:
+        assert hasattr(L, n)
+    assert L.tm_abi_version() == nat.ABI_VERSION
+
+
+def test_errors_cross_the_abi_as_status():
+    L = nat.lib()
+    conn = np.array([[0, 1, 5]], dtype=np.int32)
+    nnz = ctypes.c_int64(0)
+    st = L.tm_pattern_nnz(conn.ctypes.data, 1, 3, 3, ctypes.byref(nnz))
+    assert st == -1 and b"outside" in L.tm_last_error()
+    with pytest.raises(nat.NativeError):
+        nat.check(st, "tm_pattern_nnz")
+
+
+@pytest.mark.parametrize("kind,n", [("tri", 7), ("tet", 3), ("tet10", 2)])
+def test_pattern_builder_matches_reference_construction(kind, n):
+    if kind == "tri":
+        m = unit_square_tri(n, 0.2, 1)
+    elif kind == "tet":
+        m = unit_cube_tet(n, 0.15, 1)
+    else:
+        m = unit_cube_tet(n, 0.0, 1, order=2)
+    conn = m.elements()
+    pat = Pattern.build(conn, m.n_points)
+    row, col, eids = fem.pattern(conn.numpy(), m.n_points)
+    assert np.array_equal(pat.row.numpy(), row) and np.array_equal(pat.col.numpy(), col)
+    assert np.array_equal(pat.slot2nnz.numpy(), eids)
+    # segments: ascending slots, exactly the slots mapped to each entry
+    seg_ptr, seg_slot = pat.seg_ptr.numpy(), pat.seg_slot.numpy()
+    assert seg_ptr[-1] == eids.size
+    for k in range(0, pat.nnz, max(1, pat.nnz // 200)):
+        s = seg_slot[seg_ptr[k]:seg_ptr[k + 1]]
+        assert np.all(np.diff(s) > 0)
+        assert np.array_equal(np.sort(np.nonzero(eids == k)[0]), s)
+    assert pat.hash == layout_hash(row, col)
+
+
+def test_incidence_is_index_add_order():
+    m = unit_cube_tet(2, 0.1, 0)
+    nd = tm.NodeAssembler.from_mesh(m)
+    inc_ptr, inc = (t.numpy() for t in nd._incidence)
+    flat = m.elements().numpy().reshape(-1)
+    for n in range(m.n_points):
+        assert np.array_equal(inc[inc_ptr[n]:inc_ptr[n + 1]], np.nonzero(flat == n)[0])
+
+
+def test_form_recognition():
+    class Lap(tm.ElementAssembler):
+        def forward(self, gradu, gradv):
+            return gradu @ gradv
+
+    class Scaled(tm.ElementAssembler):
+        def forward(self, gradu, gradv):
+            return 2.5 * (gradu @ gradv)
+
+    class Weird(tm.ElementAssembler):
+        def forward(self, gradu, gradv, k):
+            return k * gradu[0] * gradv[0]
+
+    E = torch.ones(4)
+    assert match_bilinear(Lap.forward.__get__(object()), 2, {}, {}, {}).form == "laplace"
+    assert match_bilinear(Scaled.forward.__get__(object()), 2, {}, {}, {}) is None
+    assert match_bilinear(Weird.forward.__get__(object()), 2, {"k": E}, {}, {}) is None
+    f = lambda gradu, gradv, kappa: kappa * (gradu @ gradv)
+    assert match_bilinear(f, 3, {"kappa": E}, {}, {}).form == "diffusion_elem"
+    assert match_bilinear(f, 3, {}, {"kappa": torch.ones(5)}, {}).form == "diffusion_node"
+    assert match_bilinear(lambda u, v: u * v, 3, {}, {}, {}).form == "mass"
+    el = tm.LinearElasticityElementAssembler.__new__(tm.LinearElasticityElementAssembler)
+    el.E, el.nu = 2.0, 0.25
+    m = match_bilinear(el.forward, 3, {}, {}, {}, owner=el)
+    assert m.form == "linear_elasticity" and (m.young, m.poisson) == (2.0, 0.25)
+    assert match_linear(lambda v, f: f * v, {"f": torch.ones(3)}, {}).key == "f"
+    lm = match_linear(tm.const_node_assembler(3.0)().forward if False else (lambda v: 3.0 * v), {}, {})
+    assert lm.key is None and abs(lm.const - 3.0) < 1e-15
+    assert match_linear(lambda x, v: x[0] * v, {"x": torch.ones(3, 2)}, {}) is None
+
+
+def test_builtin_forms_refuse_cpu_tensors():
+    m = unit_square_tri(4, 0.1, 0)
+    K = tm.LaplaceElementAssembler.from_mesh(m)
+    with pytest.raises(nat.NativeError, match="ROCm device"):
+        K()
+
+
+def test_user_form_torch_path_matches_oracle():
+    # a form that is not built in runs through the generic vmap path
+    m = unit_square_tri(5, 0.2, 2)
+
+    class Aniso(tm.ElementAssembler):
+        def forward(self, gradu, gradv):
+            return 2.0 * gradu[0] * gradv[0] + gradu[1] * gradv[1]
+
+    K = Aniso.from_mesh(m)()
+    v, r, c, _ = fem.assemble_bilinear(m.points, m.elements(), "triangle", 2, "laplace")
+    # oracle of the same anisotropic form through element_integrals' ingredients
+    phi, grad, jxw, _ = fem.shape_grad_jxw(m.points, m.elements(), "triangle", 2)
+    integ = 2.0 * grad[..., :, None, 0] * grad[..., None, :, 0] + grad[..., :, None, 1] * grad[..., None, :, 1]
+    loc = torch.einsum("eqij,eq->eij", integ, jxw)
+    ref = torch.zeros(r.numel(), dtype=torch.float64).index_add(0, torch.from_numpy(fem.pattern(m.elements().numpy(), m.n_points)[2]), loc.reshape(-1))
+    assert torch.equal(K.row, r) and torch.equal(K.col, c)
+    assert torch.allclose(K.values, ref, rtol=1e-13, atol=1e-15)
+
+
+def test_user_load_torch_path_matches_oracle():
+    m = unit_square_tri(6, 0.2, 3)
+    Src = tm.func_node_assembler(lambda x: torch.sin(x[..., 0]) + x[..., 1])
+    b = Src.from_mesh(m)(m.points)
+    f = torch.sin(m.points[:, 0]) + m.points[:, 1]
+    # f is linear in y but not in x: compare with the oracle's interpolated form only on the linear part
+    Lin = tm.func_node_assembler(lambda x: x[..., 1])
+    b2 = Lin.from_mesh(m)(m.points)
+    ref = fem.assemble_linear(m.points, m.elements(), "triangle", 2, m.points[:, 1])
+    assert torch.allclose(b2, ref, rtol=1e-13, atol=1e-16)
+    assert b.shape == (m.n_points,)
+
+
+def test_sparse_matrix_api():
+    edata = torch.stack([torch.ones(2, 2), 2 * torch.ones(2, 2)]).double()
+    M = tm.SparseMatrix.from_block_coo(edata, torch.tensor([0, 1]), torch.tensor([0, 1]), (2, 2))
+    assert torch.equal(M.to_dense(), torch.tensor([[1., 1, 0, 0], [1, 1, 0, 0], [0, 0, 2, 2], [0, 0, 2, 2]]).double())
+    x = torch.arange(4.0).double()
+    assert torch.allclose(M @ x, M.to_dense() @ x)
+    assert M.has_same_layout(M.layout_hash) and (M + M).has_same_layout(M)
+    assert torch.equal(M.T.to_dense(), M.to_dense().T)
+
+
+def test_condenser_cpu_known_answers():
+    # tests/operator/test_condenser.py:48-138 (reference)
+    row = torch.tensor([0, 0, 1, 1, 1, 2, 2, 2, 3, 3])
+    col = torch.tensor([0, 1, 0, 1, 2, 1, 2, 3, 2, 3])
+    val = torch.tensor([2., -1., -1., 2., -1., -1., 2., -1., -1., 2.]).double()
+    K = tm.SparseMatrix(val, row, col, (4, 4))
+    f = torch.tensor([1., 0., 0., 1.]).double()
+    mask = torch.tensor([True, False, False, True])
+    cd = tm.Condenser(mask, torch.tensor([1.0, 2.0]).double())
+    Ki, fi = cd(K, f)
+    assert Ki.shape == (2, 2) and torch.equal(fi, torch.tensor([1., 2.]).double())
+    u = cd.recover(torch.tensor([5., 6.]).double())
+    assert torch.equal(u, torch.tensor([1., 5., 6., 2.]).double())
+    cd2 = tm.Condenser(mask, torch.tensor([99.0, 99.0]))
+    assert torch.equal(cd2.restrict(f), torch.tensor([0., 0.]).double())
+    assert torch.equal(cd2.prolong(torch.tensor([1., 2.]).double()), torch.tensor([0., 1., 2., 0.]).double())
+    assert cd2.is_inner_dof is None
+
+
+def test_cg_solves_condensed_poisson_cpu():
+    d = np.load(ROOT / "tests" / "golden" / "cfg1_tri20.npz")
+    Ki = tm.SparseMatrix(torch.from_numpy(d["Kii"]), torch.from_numpy(d["Kii_row"]),
+                         torch.from_numpy(d["Kii_col"]), (361, 361))
+    fi = torch.from_numpy(d["f_i"])
+    u = Ki.solve(fi, rtol=1e-12)
+    ref = torch.linalg.solve(Ki.to_dense(), fi)
+    assert torch.allclose(u, ref, rtol=1e-9, atol=1e-12)

@@ -1,0 +1,284 @@
+"""Pins of the CPU oracle (oracle/fem.py) against facts that do not come
+from the oracle itself: closed forms, invariants, brute force and the
+known answers of the reference's own tests."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem
+from tensormesh_amd.mesh import quadratic_arrays, unit_cube_tet_arrays, unit_square_tri_arrays
+
+torch.set_default_dtype(torch.float64)
+
+
+def _dense(v, r, c, shape):
+    return torch.sparse_coo_tensor(torch.stack([r, c]), v, shape).to_dense()
+
+
+# ---------------------------------------------------------------- pattern
+@pytest.mark.parametrize("n,kind", [(3, "tri"), (2, "tet"), (1, "tet10")])
+def test_pattern_bruteforce(n, kind):
+    if kind == "tri":
+        p, c, _ = unit_square_tri_arrays(n, 0.1, 3)
+    elif kind == "tet":
+        p, c, _ = unit_cube_tet_arrays(n, 0.1, 3)
+    else:
+        p, c, _ = quadratic_arrays(*unit_cube_tet_arrays(n, 0.0, 3))
+    row, col, eids = fem.pattern(c, p.shape[0])
+    pairs = sorted({(int(a), int(b)) for e in c for a in e for b in e})
+    assert list(zip(row.tolist(), col.tolist())) == pairs
+    B = c.shape[1]
+    for e in range(c.shape[0]):
+        for i in range(B):
+            for j in range(B):
+                k = eids[e * B * B + i * B + j]
+                assert (row[k], col[k]) == (c[e, i], c[e, j])
+
+
+def test_pattern_sizes_of_the_workloads():
+    p, c, bnd = unit_square_tri_arrays(20, 0.2, 0)
+    assert c.shape == (800, 3) and p.shape == (441, 2) and bnd.sum() == 80
+    assert fem.pattern(c, 441)[0].size == 2921
+    p, c, _ = unit_cube_tet_arrays(64, 0.15, 0)
+    assert c.shape == (1572864, 4) and p.shape[0] == 274625
+    assert fem.pattern(c, p.shape[0])[0].size == 4018753
+
+
+# ------------------------------------------------------ reference element
+@pytest.mark.parametrize("etype,order,vol", [("triangle", o, 0.5) for o in (1, 2, 3, 4)] +
+                         [("tetra", o, 1 / 6) for o in (1, 2, 3, 4)])
+def test_quadrature_weights_and_exactness(etype, order, vol):
+    w, q = fem.quadrature(etype, order)
+    assert abs(w.sum().item() - vol) < 1e-12
+    dim = q.shape[1]
+    # exact integral of x^a y^b (z^c) over the reference simplex: a! b! c! / (d + a + b + c)!
+    for e in itertools.product(range(order + 1), repeat=dim):
+        if sum(e) > order:
+            continue
+        exact = math.prod(math.factorial(k) for k in e) / math.factorial(dim + sum(e))
+        approx = (w * torch.prod(q ** torch.tensor(e, dtype=torch.float64), 1)).sum().item()
+        assert abs(approx - exact) < 1e-12, (e, approx, exact)
+
+
+@pytest.mark.parametrize("etype", ["triangle", "triangle6", "tetra", "tetra10"])
+def test_basis_kronecker_and_partition_of_unity(etype):
+    nodes = fem.lagrange_nodes(etype)
+    phi = fem.shape_values(etype, nodes)
+    assert np.allclose(phi, np.eye(nodes.shape[0]), atol=1e-12)
+    pts = np.random.default_rng(0).random((7, nodes.shape[1])) * 0.3
+    assert np.allclose(fem.shape_values(etype, pts).sum(1), 1.0)
+    assert np.allclose(fem.shape_ref_grads(etype, pts).sum(2), 0.0, atol=1e-12)
+
+
+def test_p2_triangle_values_match_reference_test():
+    # closed form of tests/element/test_transformation.py:66-98 (reference)
+    w, q = fem.quadrature("triangle", 4)
+    q = q.numpy()
+    xi, eta = q[:, 0], q[:, 1]
+    exp = np.zeros((q.shape[0], 6))
+    exp[:, 0] = (1 - xi - eta) * (1 - 2 * xi - 2 * eta)
+    exp[:, 1] = xi * (2 * xi - 1)
+    exp[:, 2] = eta * (2 * eta - 1)
+    exp[:, 5] = 4 * xi * (1 - xi - eta)
+    exp[:, 3] = 4 * xi * eta
+    exp[:, 4] = 4 * eta * (1 - xi - eta)
+    assert np.allclose(fem.shape_values("triangle6", q), exp, atol=1e-13)
+
+
+def test_product_tables_equal_oracle_vandermonde():
+    from tensormesh_amd import element as el
+    for etype in ("triangle", "triangle6", "tetra", "tetra10"):
+        w, q = el.quadrature(etype, 2)
+        assert np.allclose(el.shape_values(etype, q), fem.shape_values(etype, q), atol=1e-13)
+        assert np.allclose(el.shape_ref_grads(etype, q), fem.shape_ref_grads(etype, q), atol=1e-12)
+
+
+# -------------------------------------------------------------- geometry
+def test_jacobian_determinant_is_measure():
+    p, c, _ = unit_square_tri_arrays(5, 0.2, 1)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    _, grad, jxw, _ = fem.shape_grad_jxw(P, C, "triangle", 1)
+    x = p[c]
+    area = 0.5 * np.abs((x[:, 1, 0] - x[:, 0, 0]) * (x[:, 2, 1] - x[:, 0, 1])
+                        - (x[:, 2, 0] - x[:, 0, 0]) * (x[:, 1, 1] - x[:, 0, 1]))
+    assert np.allclose(jxw[:, 0].numpy(), area, rtol=1e-13)
+    p, c, _ = unit_cube_tet_arrays(3, 0.15, 1)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    _, grad, jxw, _ = fem.shape_grad_jxw(P, C, "tetra", 2)
+    x = p[c]
+    vol = np.abs(np.einsum("ei,ei->e", x[:, 1] - x[:, 0], np.cross(x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]))) / 6
+    assert np.allclose(jxw.sum(1).numpy(), vol, rtol=1e-13)
+    # gradients of a linear function are exact
+    a = np.array([0.3, -1.2, 2.0])
+    u = torch.from_numpy(p @ a + 0.7)
+    gu = torch.einsum("eb,eqbd->eqd", u[C], grad)
+    assert torch.allclose(gu, torch.tensor(a).expand_as(gu), atol=1e-11)
+
+
+# ---------------------------------------------------------- bilinear forms
+def test_laplace_patch_test_and_properties():
+    p, c, bnd = unit_square_tri_arrays(8, 0.2, 5)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    K = _dense(*fem.assemble_bilinear(P, C, "triangle", 2, "laplace"))
+    assert torch.allclose(K, K.T, atol=1e-14)
+    assert K.sum(1).abs().max() < 1e-12
+    assert torch.linalg.eigvalsh(K).min() > -1e-10
+    u = P @ torch.tensor([0.4, -0.9]) + 0.1
+    assert (K @ u)[~torch.from_numpy(bnd)].abs().max() < 1e-12
+
+
+def test_diffusion_element_closed_form():
+    p, c, _ = unit_cube_tet_arrays(2, 0.15, 2)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    kap = torch.rand(c.shape[0]) + 0.5
+    loc = fem.element_integrals(P, C, "tetra", 1, "diffusion_elem", kappa=kap)
+    x = p[c]
+    for e in range(c.shape[0]):
+        X = np.stack([x[e, 1] - x[e, 0], x[e, 2] - x[e, 0], x[e, 3] - x[e, 0]])
+        G = np.linalg.inv(X)           # columns = grad(lambda_1..3)
+        g = np.concatenate([-G.sum(1, keepdims=True), G], 1)
+        vol = abs(np.linalg.det(X)) / 6
+        assert np.allclose(loc[e].numpy(), kap[e].item() * vol * g.T @ g, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("etype,D", [("triangle", 2), ("tetra", 3)])
+def test_mass_closed_form(etype, D):
+    gen = unit_square_tri_arrays if D == 2 else unit_cube_tet_arrays
+    p, c, _ = gen(3, 0.1, 4)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    loc = fem.element_integrals(P, C, etype, 2, "mass")
+    _, _, jxw, _ = fem.shape_grad_jxw(P, C, etype, 1)
+    vol = jxw[:, 0]
+    exact = vol[:, None, None] / ((D + 1) * (D + 2)) * (torch.ones(D + 1, D + 1) + torch.eye(D + 1))
+    assert torch.allclose(loc, exact, rtol=1e-12)
+    M = _dense(*fem.assemble_bilinear(P, C, etype, 2, "mass"))
+    assert abs(M.sum().item() - 1.0) < 1e-12
+
+
+def test_voigt_known_answers():
+    # tests/functional/test_elasticity.py:19-84 (reference)
+    B = fem.voigt_B(torch.tensor([1., 2., 3.]))
+    assert torch.equal(B, torch.tensor([[1., 0, 0], [0, 2, 0], [0, 0, 3], [0, 3, 2], [3, 0, 1], [2, 1, 0]]))
+    B2 = fem.voigt_B2(torch.tensor([1., 2.]))
+    assert torch.equal(B2, torch.tensor([[1., 0], [0, 2], [2, 1]]))
+    C = fem.voigt_C(70.0, 0.3, 3)
+    mu, lam = 70 / 2.6, 70 * 0.3 / (1.3 * 0.4)
+    exp = torch.zeros(6, 6)
+    exp[:3, :3] = lam
+    exp[[0, 1, 2], [0, 1, 2]] = lam + 2 * mu
+    exp[[3, 4, 5], [3, 4, 5]] = mu
+    assert torch.allclose(C, exp)
+
+
+def test_elasticity_rigid_body_modes():
+    p, c, _ = quadratic_arrays(*unit_cube_tet_arrays(2, 0.1, 6))
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    v, r, cc, shape = fem.assemble_bilinear(P, C, "tetra10", 2, "linear_elasticity", E=2.0, nu=0.25)
+    K = _dense(v, r, cc, shape)
+    assert torch.allclose(K, K.T, atol=1e-12)
+    n = p.shape[0]
+    modes = []
+    for d in range(3):
+        t = torch.zeros(n, 3)
+        t[:, d] = 1
+        modes.append(t.reshape(-1))
+    for a, b in ((0, 1), (1, 2), (0, 2)):
+        t = torch.zeros(n, 3)
+        t[:, a], t[:, b] = -P[:, b], P[:, a]
+        modes.append(t.reshape(-1))
+    for m in modes:
+        assert (K @ m).abs().max() < 1e-11 * K.abs().max()
+
+
+# ----------------------------------------------------- projector / blocks
+def test_reduce_project_bruteforce():
+    g = torch.Generator().manual_seed(0)
+    for n_src, n_dst, h in ((100, 10, 1), (1000, 100, 8)):
+        dst = torch.randint(0, n_dst, (n_src,), generator=g)
+        x = torch.randn(n_src, h, generator=g)
+        out = fem.reduce_project(x, dst, n_dst)
+        ref = torch.zeros(n_dst, h)
+        for s in range(n_src):
+            ref[dst[s]] += x[s]
+        assert torch.allclose(out, ref, atol=1e-12)
+
+
+def test_block_coo_known_answer():
+    # tests/sparse/test_block_coo.py:10-25 (reference)
+    edata = torch.stack([torch.ones(2, 2), 2 * torch.ones(2, 2)])
+    v, r, c = fem.block_coo(edata, torch.tensor([0, 1]), torch.tensor([0, 1]))
+    dense = _dense(v, r, c, (4, 4))
+    assert torch.equal(dense, torch.tensor([[1., 1, 0, 0], [1, 1, 0, 0], [0, 0, 2, 2], [0, 0, 2, 2]]))
+
+
+# ------------------------------------------------------------------ loads
+def test_load_closed_form_and_integral():
+    p, c, _ = unit_square_tri_arrays(6, 0.2, 9)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    f = torch.sin(P[:, 0]) + P[:, 1] ** 2
+    b = fem.assemble_linear(P, C, "triangle", 2, f)
+    # exact for P1 f_h * phi (quadratic) with the order-2 rule: |T|/12 (2 f_i + f_j + f_k)
+    _, _, jxw, _ = fem.shape_grad_jxw(P, C, "triangle", 1)
+    exp = torch.zeros(p.shape[0])
+    for e in range(c.shape[0]):
+        fe = f[C[e]]
+        for i in range(3):
+            exp[C[e, i]] += jxw[e, 0] / 12 * (fe.sum() + fe[i])
+    assert torch.allclose(b, exp, rtol=1e-12)
+    ones = fem.assemble_linear(P, C, "triangle", 2, torch.ones(p.shape[0]))
+    assert abs(ones.sum().item() - 1.0) < 1e-13
+
+
+# ------------------------------------------------------------- condenser
+def test_condenser_known_answers():
+    # tests/operator/test_condenser.py:48-100 (reference): 4x4 tridiagonal
+    row = torch.tensor([0, 0, 1, 1, 1, 2, 2, 2, 3, 3])
+    col = torch.tensor([0, 1, 0, 1, 2, 1, 2, 3, 2, 3])
+    val = torch.tensor([2., -1., -1., 2., -1., -1., 2., -1., -1., 2.])
+    f = torch.tensor([1., 0., 0., 1.])
+    mask = torch.tensor([True, False, False, True])
+    out = fem.condense(val, row, col, 4, mask, torch.tensor([1.0, 2.0]), f)
+    kv, kr, kc, shape = out["K_ii"]
+    assert shape == (2, 2)
+    assert torch.equal(_dense(kv, kr, kc, shape), torch.tensor([[2., -1.], [-1., 2.]]))
+    # f_i = f_inner - K_io u_o = [0 - (-1*1), 0 - (-1*2)]
+    assert torch.equal(out["f_i"], torch.tensor([1., 2.]))
+    u = fem.recover(torch.tensor([5., 6.]), mask, out["dvalue"])
+    assert torch.equal(u, torch.tensor([1., 5., 6., 2.]))
+
+
+# ------------------------------------------------------------ neo-hookean
+def test_neohookean_zero_state_and_linearisation():
+    p, c, _ = unit_cube_tet_arrays(1, 0.1, 2)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    u0 = torch.zeros_like(P)
+    assert abs(fem.neohookean_energy(P, C, "tetra", 2, u0).item()) < 1e-14
+    res, tan = fem.neohookean_element_tangent(P, C, "tetra", 2, u0, E=1.0, nu=0.3)
+    assert res.abs().max() < 1e-14
+    lin = fem.element_integrals(P, C, "tetra", 2, "linear_elasticity", E=1.0, nu=0.3)
+    assert torch.allclose(tan, lin, atol=1e-13)
+
+
+def test_neohookean_finite_differences():
+    p, c, _ = unit_cube_tet_arrays(1, 0.1, 2)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    g = torch.Generator().manual_seed(3)
+    u = 0.05 * torch.randn(P.shape, generator=g)
+    res, tan = fem.neohookean_element_tangent(P, C, "tetra", 2, u)
+    h = 1e-6
+    # residual == d energy / du (element 0, dof (1, 2))
+    e = 0
+    def elem_energy(uu):
+        return fem.neohookean_energy(P, C[e:e + 1], "tetra", 2, uu)
+    du = torch.zeros_like(u)
+    du[C[e, 1], 2] = h
+    fd = (elem_energy(u + du) - elem_energy(u - du)) / (2 * h)
+    assert abs(fd.item() - res[e, 1, 2].item()) < 1e-8
+    # tangent == d residual / du
+    res_p, _ = fem.neohookean_element_tangent(P, C, "tetra", 2, u + du)
+    res_m, _ = fem.neohookean_element_tangent(P, C, "tetra", 2, u - du)
+    fd_t = (res_p - res_m)[e] / (2 * h)               # d R[e,i,a] / d u[node1, 2]
+    assert torch.allclose(fd_t, tan[e, :, 1, :, 2], atol=1e-7)

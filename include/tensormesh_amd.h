@@ -95,12 +95,16 @@ typedef struct tm_pattern_view {
   const int32_t* col_idx;    /* [nnz] */
   const int32_t* seg_ptr;    /* [nnz + 1] start of each entry's contributions */
   const int32_t* seg_slot;   /* [n_elem*B*B] slot ids s = e*B*B + i*B + j, ascending per entry */
+  int64_t n_blocks;          /* reduction plan (tm_reduce_plan), or 0 for one thread per entry */
+  const int32_t* blk_ptr;    /* [n_blocks + 1] first entry of each reduction block */
 } tm_pattern_view;
 
 /* Node -> (element, local index) incidence, ascending element id per node. */
 typedef struct tm_incidence_view {
   const int32_t* inc_ptr;    /* [n_points + 1] */
   const int32_t* inc;        /* [n_elem*B] packed e*B + i */
+  int64_t n_blocks;          /* reduction plan over nodes, or 0 */
+  const int32_t* blk_ptr;    /* [n_blocks + 1] */
 } tm_incidence_view;
 
 int tm_abi_version(void);
@@ -120,6 +124,14 @@ int tm_pattern_nnz(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t
 int tm_pattern_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
                      int32_t* row_ptr, int32_t* col_idx, int32_t* seg_ptr, int32_t* seg_slot,
                      int32_t* slot2nnz);
+
+/* Reduction plan (host): split entries [0, n_entries) into blocks of whole
+ * entries holding at most `cap` contributions each (an entry with more gets a
+ * block of its own).  blk_ptr_out needs n_entries + 1 slots; *n_blocks gets
+ * the count.  Used by the segmented reduction to stage each block's
+ * contributions in LDS. */
+int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t cap, int32_t* blk_ptr_out,
+                   int64_t* n_blocks);
 
 /* Node incidence (host), the scatter map of NodeAssembler's ReduceProjector
  * (tensormesh/assemble/node_assembler.py:665-670) in gather form. */
@@ -147,11 +159,14 @@ int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc* form,
 int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
                         tm_stream_t stream);
 
-/* Stage S alone: out[k*HH + t] = sum over s in seg(k) of slot[s*HH + t].
- * The deterministic form of ReduceProjector.__call__ (projector.py:95-123). */
+/* Stage S alone: out[k*HH + t] = sum over s in seg(k) of slot[s*HH + t],
+ * added left to right in segment order.  The deterministic form of
+ * ReduceProjector.__call__ (projector.py:95-123).  With a plan (n_blocks > 0)
+ * each block gathers its contributions into LDS first; without one, one
+ * thread walks each entry. */
 int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
-                        const int32_t* seg_slot, const void* slot, void* out,
-                        tm_stream_t stream);
+                        const int32_t* seg_slot, int64_t n_blocks, const int32_t* blk_ptr,
+                        const void* slot, void* out, tm_stream_t stream);
 
 /* ---------------------------------------------------------------------
  * Linear-form assembly (load vector), two stages on one stream.

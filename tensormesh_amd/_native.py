@@ -42,11 +42,13 @@ class FormDesc(ctypes.Structure):
 
 class PatternView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("col_idx", ctypes.c_void_p),
-                ("seg_ptr", ctypes.c_void_p), ("seg_slot", ctypes.c_void_p)]
+                ("seg_ptr", ctypes.c_void_p), ("seg_slot", ctypes.c_void_p),
+                ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
 
 
 class IncidenceView(ctypes.Structure):
-    _fields_ = [("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p)]
+    _fields_ = [("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p),
+                ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/tensormesh_amd.h
@@ -65,7 +67,9 @@ SIGNATURES = {
     "tm_element_matrices": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                            ctypes.c_void_p, ctypes.c_void_p]),
     "tm_segmented_reduce": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
-                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+                                           ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_reduce_plan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, c_i64p]),
     "tm_assemble_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                           ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),
@@ -121,6 +125,27 @@ def require_device(t: torch.Tensor, what: str) -> None:
 
 def stream_of(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
+
+
+LDS_BYTES = 32768   # LDS staging buffer of the planned segmented reduction (tm_reduce.hip)
+
+
+def reduce_cap(dtype: torch.dtype, hh: int) -> int:
+    """Contributions per reduction block that fit the kernel's LDS buffer."""
+    size = 8 if dtype == torch.float64 else 4
+    return max(1, min(4096, LDS_BYTES // (size * hh)))
+
+
+def reduce_plan(seg_ptr_host, cap: int):
+    """blk_ptr (int32 numpy) of tm_reduce_plan for a host seg_ptr array."""
+    import numpy as np
+    seg_ptr_host = np.ascontiguousarray(seg_ptr_host, dtype=np.int32)
+    n = seg_ptr_host.shape[0] - 1
+    out = np.empty(n + 1, np.int32)
+    nb = ctypes.c_int64(0)
+    check(lib().tm_reduce_plan(seg_ptr_host.ctypes.data, n, cap, out.ctypes.data, ctypes.byref(nb)),
+          "tm_reduce_plan")
+    return out[: nb.value + 1].copy()
 
 
 def dtype_code(dtype: torch.dtype) -> int:

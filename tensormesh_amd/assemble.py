@@ -64,6 +64,13 @@ def _n_quad(etype: str, qorder: int) -> int:
     return int(quadrature(etype, qorder)[0].shape[0])
 
 
+def _scalar_state(obj) -> tuple:
+    """Plain scalar attributes of a module (e.g. E, nu, c): part of the form
+    recognition cache key, so changing one re-probes the integrand."""
+    return tuple(sorted((k, v) for k, v in obj.__dict__.items()
+                        if isinstance(v, (int, float, bool, str)) and not k.startswith("_")))
+
+
 class _Broadcast:
     def __init__(self, element, quadrature, u, v):
         self.element = 0 if element else None
@@ -204,8 +211,7 @@ class ElementAssembler(_AssemblerBase):
             assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
 
         fn = self.forward if func is None else func
-        match = match_bilinear(fn, self.dimension, element_data, point_data, scalar_data,
-                               owner=self if func is None else None)
+        match = self._match(fn, func is None, element_data, point_data, scalar_data)
         if match is not None:
             coeff = None
             if match.coeff_key is not None:
@@ -218,6 +224,28 @@ class ElementAssembler(_AssemblerBase):
                 return self._wrap_output(assemble_with_adjoint(self, match, points, coeff), points.device)
         point_data["x"] = points
         return self._torch_bilinear(fn, points, point_data, element_data, scalar_data, batch_size)
+
+    def _match(self, fn, is_forward: bool, element_data, point_data, scalar_data) -> Optional[FormMatch]:
+        """Form recognition, cached per (integrand, data layout): the probe runs
+        once, not on every call."""
+        func_id = getattr(fn, "__func__", fn)
+        owner_state = _scalar_state(self) if is_forward else None
+
+        def sig(d):
+            return tuple(sorted((k, tuple(v.shape[1:]) if torch.is_tensor(v) else None) for k, v in d.items()))
+
+        key = (func_id, owner_state, sig(element_data), sig(point_data), tuple(sorted(scalar_data)))
+        cache = self.__dict__.setdefault("_match_cache", {})
+        try:
+            if key in cache:
+                return cache[key]
+        except TypeError:   # unhashable callable
+            return match_bilinear(fn, self.dimension, element_data, point_data, scalar_data,
+                                  owner=self if is_forward else None)
+        m = match_bilinear(fn, self.dimension, element_data, point_data, scalar_data,
+                           owner=self if is_forward else None)
+        cache[key] = m
+        return m
 
     # ---- native path ----
     def _native_values(self, match: FormMatch, points: torch.Tensor, coeff: Optional[torch.Tensor]) -> torch.Tensor:
@@ -240,7 +268,7 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
                             tab.ctypes.data, tab_dev.data_ptr(),
                             None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
-        pv = pat.view()
+        pv = pat.view(dtype, H * H)
         nat.check(L.tm_assemble_bilinear(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(pv),
                                          slot.data_ptr(), values.data_ptr(), nat.stream_of(dev)),
                   "tm_assemble_bilinear")
@@ -469,7 +497,13 @@ class NodeAssembler(_AssemblerBase):
         for key, value in point_data.items():
             assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
         fn = self.forward if func is None else func
-        match = match_linear(fn, point_data, scalar_data)
+        key = (getattr(fn, "__func__", fn), _scalar_state(self) if func is None else None,
+               tuple(sorted((k, tuple(v.shape[1:])) for k, v in point_data.items())), tuple(sorted(scalar_data)))
+        cache = self.__dict__.setdefault("_match_cache", {})
+        try:
+            match = cache[key] if key in cache else cache.setdefault(key, match_linear(fn, point_data, scalar_data))
+        except TypeError:
+            match = match_linear(fn, point_data, scalar_data)
         if match is not None:
             f = None if match.key is None else point_data[match.key]
             needs_grad = torch.is_grad_enabled() and (points.requires_grad or (f is not None and f.requires_grad))
@@ -490,12 +524,15 @@ class NodeAssembler(_AssemblerBase):
             f = f.detach().to(points.dtype).contiguous()
         tab, tab_dev = self._tables(dev)
         inc_ptr, inc = (t.to(dev) for t in self._incidence)
+        cap = nat.reduce_cap(points.dtype, 1)
+        plan = self._cached(f"nodeplan{cap}", dev,
+                            lambda: torch.from_numpy(nat.reduce_plan(inc_ptr.cpu().numpy(), cap)).to(dev))
         slot = torch.empty(E * B, dtype=points.dtype, device=dev)
         out = torch.empty(points.shape[0], dtype=points.dtype, device=dev)
         mesh = self._mesh_view(points)
         form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
                             None if f is None else f.data_ptr(), 0.0, 0.0)
-        iv = nat.IncidenceView(inc_ptr.data_ptr(), inc.data_ptr())
+        iv = nat.IncidenceView(inc_ptr.data_ptr(), inc.data_ptr(), plan.numel() - 1, plan.data_ptr())
         cval = ctypes.c_double(const)
         nat.check(L.tm_assemble_linear(ctypes.byref(mesh), ctypes.byref(form),
                                        ctypes.byref(cval) if f is None else None, ctypes.byref(iv),

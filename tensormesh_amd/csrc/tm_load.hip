@@ -10,7 +10,7 @@
 // per-element vectors are summed into nodes.  The scatter of the reference
 // (ReduceProjector over conn.flatten(), node_assembler.py:665-670) becomes a
 // gather over the node incidence lists (ascending e*B + i), done by the
-// segmented reduction of tm_p1.hip with seg_ptr = inc_ptr, seg_slot = inc.
+// segmented reduction of tm_reduce.hip with seg_ptr = inc_ptr, seg_slot = inc.
 #include <hip/hip_runtime.h>
 
 #include "tm_internal.h"
@@ -118,6 +118,7 @@ extern "C" int tm_assemble_linear(const tm_mesh_view* mesh, const tm_form_desc* 
   int st = mesh->dtype == TM_F64 ? dispatch<double>(mesh, form, const_value, slot_ws, stream)
                                  : dispatch<float>(mesh, form, const_value, slot_ws, stream);
   if (st) return st;
-  return tm_segmented_reduce(mesh->dtype, mesh->n_points, 1, inc->inc_ptr, inc->inc, slot_ws, out, stream_);
+  return tm_segmented_reduce(mesh->dtype, mesh->n_points, 1, inc->inc_ptr, inc->inc, inc->n_blocks, inc->blk_ptr,
+                             slot_ws, out, stream_);
   TM_TRY_END
 }

@@ -1,5 +1,5 @@
-// P1 simplex assembly: stage G (element geometry + local matrix) and stage S
-// (deterministic segmented reduction into CSR entry order).
+// P1 simplex assembly, stage G (element geometry + local matrix), and the
+// two-stage bilinear entry point.
 //
 // Stage G replaces, for linear triangles and tetrahedra, the chain
 //   Element.eval_shape_grad      tensormesh/element/element.py:956-968
@@ -18,10 +18,8 @@
 // s = e*B*B + i*B + j (the reference's slot numbering,
 // element_assembler.py:811-819).
 //
-// Stage S replaces ReduceProjector.__call__ (projector.py:95-123): entry k sums
-// slot[seg_slot[t]] for t in [seg_ptr[k], seg_ptr[k+1]) from zero, in
-// ascending slot order, i.e. the order of the reference's sequential CPU
-// index_add_.  No atomics: the result is bitwise reproducible.
+// Stage S (ReduceProjector.__call__, projector.py:95-123) lives in
+// tm_reduce.hip.
 #include <hip/hip_runtime.h>
 
 #include "tm_internal.h"
@@ -145,39 +143,37 @@ __device__ __forceinline__ void p1_local(const int32_t* __restrict__ conn, const
   }
 }
 
+// One thread per element; the block's 256 local matrices are staged in LDS
+// and written out as contiguous 16-byte vectors (a per-thread store of B*B
+// values would hit B*B different cache lines per wave instruction).
 template <typename T, int D, int FORM>
 __global__ void __launch_bounds__(kBlock) p1_element_kernel(const int32_t* __restrict__ conn,
                                                             const T* __restrict__ pts,
                                                             const T* __restrict__ coeff, P1Consts c,
                                                             int64_t n_elem, T* __restrict__ slot) {
   constexpr int B = D + 1;
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= n_elem) return;
-  double K[B * B];
-  p1_local<T, D, FORM>(conn, pts, coeff, c, e, K);
-  T* out = slot + e * (B * B);
+  constexpr int BB = B * B;
+  __shared__ T tile[kBlock * BB];
+  const int64_t e0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t e = e0 + threadIdx.x;
+  if (e < n_elem) {
+    double K[BB];
+    p1_local<T, D, FORM>(conn, pts, coeff, c, e, K);
 #pragma unroll
-  for (int k = 0; k < B * B; ++k) out[k] = (T)K[k];
-}
-
-template <typename T, int HH>
-__global__ void __launch_bounds__(kBlock) segmented_reduce_kernel(const int32_t* __restrict__ seg_ptr,
-                                                                  const int32_t* __restrict__ seg_slot,
-                                                                  const T* __restrict__ slot,
-                                                                  int64_t nnz, T* __restrict__ out) {
-  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (k >= nnz) return;
-  const int32_t b = seg_ptr[k], e = seg_ptr[k + 1];
-  T acc[HH];
-#pragma unroll
-  for (int h = 0; h < HH; ++h) acc[h] = T(0);
-  for (int32_t t = b; t < e; ++t) {
-    const int64_t s = seg_slot[t];
-#pragma unroll
-    for (int h = 0; h < HH; ++h) acc[h] += slot[s * HH + h];
+    for (int k = 0; k < BB; ++k) tile[threadIdx.x * BB + k] = (T)K[k];
   }
-#pragma unroll
-  for (int h = 0; h < HH; ++h) out[k * HH + h] = acc[h];
+  __syncthreads();
+  const int64_t nvalid = (n_elem - e0) < kBlock ? (n_elem - e0) : kBlock;
+  const int64_t nbytes = nvalid * BB * (int64_t)sizeof(T);
+  const int64_t nvec = nbytes / 16;
+  const uint4* src = reinterpret_cast<const uint4*>(tile);
+  uint4* dst = reinterpret_cast<uint4*>(slot + e0 * BB);
+  for (int64_t t = threadIdx.x; t < nvec; t += kBlock) dst[t] = src[t];
+  const int64_t tail = nbytes - nvec * 16;  // multiple of sizeof(T) < 16
+  if (threadIdx.x < tail / (int64_t)sizeof(T)) {
+    const int64_t off = nvec * 16 / (int64_t)sizeof(T) + threadIdx.x;
+    slot[e0 * BB + off] = tile[off];
+  }
 }
 
 int fill_consts(const tm_form_desc* f, int B, int D, P1Consts& c) {
@@ -233,27 +229,6 @@ int launch_p1(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipSt
   return TM_OK;
 }
 
-template <typename T>
-int launch_reduce(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot, const void* slot,
-                  void* out, hipStream_t stream) {
-  const int64_t grid = (nnz + kBlock - 1) / kBlock;
-  if (grid == 0) return TM_OK;
-  const T* s = static_cast<const T*>(slot);
-  T* o = static_cast<T*>(out);
-  switch (hh) {
-    case 1: segmented_reduce_kernel<T, 1><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o); break;
-    case 4: segmented_reduce_kernel<T, 4><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o); break;
-    case 9: segmented_reduce_kernel<T, 9><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o); break;
-    default: return tmh::fail(TM_ERR_UNSUPPORTED, "block size must be 1, 2x2 or 3x3");
-  }
-  hipError_t err = hipGetLastError();
-  if (err != hipSuccess) {
-    tmh::set_error("segmented reduce launch: %s", hipGetErrorString(err));
-    return TM_ERR_HIP;
-  }
-  return TM_OK;
-}
-
 int check_mesh(const tm_mesh_view* m) {
   if (!m) return tmh::fail(TM_ERR_ARG, "mesh is NULL");
   if (tmh::cell_basis(m->cell) < 0) return tmh::fail(TM_ERR_ARG, "unknown cell type");
@@ -294,19 +269,6 @@ extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc*
   TM_TRY_END
 }
 
-extern "C" int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
-                                   const int32_t* seg_slot, const void* slot, void* out,
-                                   tm_stream_t stream_) {
-  TM_TRY_BEGIN
-  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
-  if (nnz < 0 || nnz >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "nnz out of range");
-  if (nnz > 0 && (!seg_ptr || !seg_slot || !slot || !out)) return tmh::fail(TM_ERR_ARG, "NULL pointer");
-  if (dtype == TM_F64) return launch_reduce<double>(nnz, hh, seg_ptr, seg_slot, slot, out, stream);
-  if (dtype == TM_F32) return launch_reduce<float>(nnz, hh, seg_ptr, seg_slot, slot, out, stream);
-  return tmh::fail(TM_ERR_ARG, "unknown dtype");
-  TM_TRY_END
-}
-
 extern "C" int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc* form,
                                     const tm_pattern_view* pat, void* slot_ws, void* values,
                                     tm_stream_t stream) {
@@ -315,6 +277,7 @@ extern "C" int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc
   int st = tm_element_matrices(mesh, form, slot_ws, stream);
   if (st) return st;
   const int hh = form->form == TM_FORM_LINEAR_ELASTICITY ? tmh::cell_dim(mesh->cell) * tmh::cell_dim(mesh->cell) : 1;
-  return tm_segmented_reduce(mesh->dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, slot_ws, values, stream);
+  return tm_segmented_reduce(mesh->dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, pat->n_blocks, pat->blk_ptr,
+                             slot_ws, values, stream);
   TM_TRY_END
 }

@@ -187,3 +187,23 @@ extern "C" int tm_incidence_build(const int32_t* conn, int64_t n_elem, int32_t n
   return TM_OK;
   TM_TRY_END
 }
+
+extern "C" int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t cap, int32_t* blk_ptr_out,
+                              int64_t* n_blocks) {
+  TM_TRY_BEGIN
+  if (!seg_ptr || !blk_ptr_out || !n_blocks) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  if (n_entries < 0 || n_entries >= tmh::kMaxIndex || cap <= 0) return tmh::fail(TM_ERR_ARG, "bad size");
+  int64_t nb = 0;
+  int64_t k = 0;
+  blk_ptr_out[0] = 0;
+  while (k < n_entries) {
+    const int64_t base = seg_ptr[k];
+    int64_t e = k + 1;  // an entry longer than cap still gets a block of its own
+    while (e < n_entries && (int64_t)seg_ptr[e + 1] - base <= cap) ++e;
+    blk_ptr_out[++nb] = (int32_t)e;
+    k = e;
+  }
+  *n_blocks = nb;
+  return TM_OK;
+  TM_TRY_END
+}

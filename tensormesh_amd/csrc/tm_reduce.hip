@@ -1,0 +1,163 @@
+// Stage S: deterministic segmented reduction of slot values into entries.
+//
+// Replaces ReduceProjector.__call__ (tensormesh/assemble/projector.py:95-123),
+// whose index_add_ uses float atomics on a GPU.  Entry k is the sum of
+// slot[seg_slot[t]] for t in [seg_ptr[k], seg_ptr[k+1]), added from zero left to
+// right -- the ascending-slot order in which the reference's sequential CPU
+// index_add_ accumulates -- so results are bitwise reproducible.
+//
+// With a reduction plan (tm_reduce_plan) each workgroup owns a run of whole
+// entries whose contributions fit in LDS ("CSR-stream" SpMV with unit values):
+//   1. all 256 threads gather the run's contributions into LDS, reading
+//      seg_slot contiguously and keeping many independent gathers in flight;
+//   2. one thread per entry sums its contiguous LDS range in order and writes
+//      the result (coalesced across the workgroup).
+// Runs that exceed the LDS capacity (a single very long entry) fall back to a
+// thread-per-entry walk over global memory.
+//
+// Workgroups are remapped so that each XCD gets a contiguous range of runs:
+// neighbouring runs read the same element slot rows, which then stay in one
+// XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include "tm_internal.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLdsBytes = 32768;
+
+// Bijection from the hardware dispatch order (block b runs on XCD b % 8) to a
+// logical order in which every XCD owns one contiguous range.
+__device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, r = nb % 8;
+  const int64_t x = b % 8, y = b / 8;
+  return x < r ? x * (q + 1) + y : r * (q + 1) + (x - r) * q + y;
+}
+
+template <typename T, int HH>
+__global__ void __launch_bounds__(kBlock) reduce_walk_kernel(const int32_t* __restrict__ seg_ptr,
+                                                             const int32_t* __restrict__ seg_slot,
+                                                             const T* __restrict__ slot, int64_t nnz,
+                                                             T* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nnz) return;
+  T acc[HH];
+#pragma unroll
+  for (int h = 0; h < HH; ++h) acc[h] = T(0);
+  for (int32_t t = seg_ptr[k]; t < seg_ptr[k + 1]; ++t) {
+    const int64_t s = seg_slot[t];
+#pragma unroll
+    for (int h = 0; h < HH; ++h) acc[h] += slot[s * HH + h];
+  }
+#pragma unroll
+  for (int h = 0; h < HH; ++h) out[k * HH + h] = acc[h];
+}
+
+template <typename T, int HH>
+__global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __restrict__ blk_ptr, int64_t n_blocks,
+                                                               const int32_t* __restrict__ seg_ptr,
+                                                               const int32_t* __restrict__ seg_slot,
+                                                               const T* __restrict__ slot, T* __restrict__ out) {
+  constexpr int kCap = kLdsBytes / (int)(sizeof(T) * HH);
+  __shared__ T lds[kCap * HH];
+  const int64_t b = xcd_logical(blockIdx.x, n_blocks);
+  const int32_t kb = blk_ptr[b], ke = blk_ptr[b + 1];
+  const int32_t cb = seg_ptr[kb], ce = seg_ptr[ke];
+  const int32_t n = ce - cb;
+  if (n > kCap) {
+    // one oversized entry run: walk global memory
+    for (int32_t k = kb + threadIdx.x; k < ke; k += kBlock) {
+      T acc[HH];
+#pragma unroll
+      for (int h = 0; h < HH; ++h) acc[h] = T(0);
+      for (int32_t t = seg_ptr[k]; t < seg_ptr[k + 1]; ++t) {
+        const int64_t s = seg_slot[t];
+#pragma unroll
+        for (int h = 0; h < HH; ++h) acc[h] += slot[s * HH + h];
+      }
+#pragma unroll
+      for (int h = 0; h < HH; ++h) out[(int64_t)k * HH + h] = acc[h];
+    }
+    return;
+  }
+  if constexpr (HH == 1) {
+    int32_t t = threadIdx.x;
+    // four independent gathers per iteration keep the memory pipeline full
+    for (; t + 3 * kBlock < n; t += 4 * kBlock) {
+      const int32_t s0 = seg_slot[cb + t], s1 = seg_slot[cb + t + kBlock];
+      const int32_t s2 = seg_slot[cb + t + 2 * kBlock], s3 = seg_slot[cb + t + 3 * kBlock];
+      const T v0 = slot[s0], v1 = slot[s1], v2 = slot[s2], v3 = slot[s3];
+      lds[t] = v0;
+      lds[t + kBlock] = v1;
+      lds[t + 2 * kBlock] = v2;
+      lds[t + 3 * kBlock] = v3;
+    }
+    for (; t < n; t += kBlock) lds[t] = slot[seg_slot[cb + t]];
+  } else {
+    for (int32_t u = threadIdx.x; u < n * HH; u += kBlock) {
+      const int32_t t = u / HH, h = u - t * HH;
+      lds[u] = slot[(int64_t)seg_slot[cb + t] * HH + h];
+    }
+  }
+  __syncthreads();
+  for (int32_t k = kb + threadIdx.x; k < ke; k += kBlock) {
+    const int32_t a = seg_ptr[k] - cb, z = seg_ptr[k + 1] - cb;
+    T acc[HH];
+#pragma unroll
+    for (int h = 0; h < HH; ++h) acc[h] = T(0);
+    for (int32_t u = a; u < z; ++u) {
+#pragma unroll
+      for (int h = 0; h < HH; ++h) acc[h] += lds[u * HH + h];
+    }
+#pragma unroll
+    for (int h = 0; h < HH; ++h) out[(int64_t)k * HH + h] = acc[h];
+  }
+}
+
+template <typename T, int HH>
+int launch_hh(int64_t nnz, const int32_t* seg_ptr, const int32_t* seg_slot, int64_t n_blocks,
+              const int32_t* blk_ptr, const void* slot, void* out, hipStream_t stream) {
+  const T* s = static_cast<const T*>(slot);
+  T* o = static_cast<T*>(out);
+  if (n_blocks > 0 && blk_ptr) {
+    reduce_stream_kernel<T, HH><<<n_blocks, kBlock, 0, stream>>>(blk_ptr, n_blocks, seg_ptr, seg_slot, s, o);
+  } else {
+    const int64_t grid = (nnz + kBlock - 1) / kBlock;
+    reduce_walk_kernel<T, HH><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o);
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("segmented reduce launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+}
+
+template <typename T>
+int launch(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot, int64_t n_blocks,
+           const int32_t* blk_ptr, const void* slot, void* out, hipStream_t stream) {
+  switch (hh) {
+    case 1: return launch_hh<T, 1>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+    case 4: return launch_hh<T, 4>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+    case 9: return launch_hh<T, 9>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+    default: return tmh::fail(TM_ERR_UNSUPPORTED, "block size must be 1, 2x2 or 3x3");
+  }
+}
+
+}  // namespace
+
+extern "C" int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
+                                   const int32_t* seg_slot, int64_t n_blocks, const int32_t* blk_ptr,
+                                   const void* slot, void* out, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (nnz < 0 || nnz >= tmh::kMaxIndex || n_blocks < 0 || n_blocks > nnz)
+    return tmh::fail(TM_ERR_SIZE, "nnz / plan size out of range");
+  if (nnz == 0) return TM_OK;
+  if (!seg_ptr || !seg_slot || !slot || !out) return tmh::fail(TM_ERR_ARG, "NULL pointer");
+  if (dtype == TM_F64) return launch<double>(nnz, hh, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+  if (dtype == TM_F32) return launch<float>(nnz, hh, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+  return tmh::fail(TM_ERR_ARG, "unknown dtype");
+  TM_TRY_END
+}

@@ -46,6 +46,18 @@ class Pattern:
     inc_ptr: Optional[torch.Tensor] = None   # int32 [N+1]
     inc: Optional[torch.Tensor] = None       # int32 [E*B]
 
+    def __post_init__(self):
+        self._plans = {}
+
+    def plan(self, cap: int, node: bool = False) -> torch.Tensor:
+        """Reduction plan (blk_ptr, int32, on the pattern's device) for entries
+        (node=False) or for the node incidence lists (node=True)."""
+        key = (cap, node)
+        if key not in self._plans:
+            seg = (self.inc_ptr if node else self.seg_ptr).cpu().numpy()
+            self._plans[key] = torch.from_numpy(nat.reduce_plan(seg, cap)).to(self.seg_ptr.device)
+        return self._plans[key]
+
     @staticmethod
     def build(conn: torch.Tensor, n_points: int, device=None, incidence: bool = True) -> "Pattern":
         L = nat.lib()
@@ -86,6 +98,7 @@ class Pattern:
                        mv(self.seg_ptr), mv(self.seg_slot), mv(self.slot2nnz), mv(self.row), mv(self.col),
                        self.hash, mv(self.inc_ptr), mv(self.inc))
 
-    def view(self) -> nat.PatternView:
+    def view(self, dtype: torch.dtype = torch.float64, hh: int = 1) -> nat.PatternView:
+        bp = self.plan(nat.reduce_cap(dtype, hh))
         return nat.PatternView(self.nnz, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
-                               self.seg_ptr.data_ptr(), self.seg_slot.data_ptr())
+                               self.seg_ptr.data_ptr(), self.seg_slot.data_ptr(), bp.numel() - 1, bp.data_ptr())

@@ -107,6 +107,46 @@ typedef struct tm_incidence_view {
   const int32_t* blk_ptr;    /* [n_blocks + 1] */
 } tm_incidence_view;
 
+/* Row-side view of a P1 pattern for the fused path: CSR rows, node
+ * incidence, and per incidence the packed positions (byte j = position of
+ * conn[e,j] inside row conn[e,i]). */
+typedef struct tm_rows_view {
+  int64_t nnz;
+  const int32_t* row_ptr;    /* [n_points + 1] */
+  const int32_t* inc_ptr;    /* [n_points + 1] */
+  const int32_t* inc;        /* [n_elem*B] packed e*B + i */
+  const uint32_t* colpos;    /* [n_elem*B] */
+  int64_t n_blocks;          /* reduction plan over inc_ptr (rows) */
+  const int32_t* blk_ptr;    /* [n_blocks + 1] */
+} tm_rows_view;
+
+/* Block-diagonal stack of SPD CSR systems for the batched Jacobi-PCG.
+ * Workgroups are system-aligned: workgroup w covers rows
+ * [blk_row0[w], blk_row1[w]) of system blk_sys[w] (at most 256 rows);
+ * sys_blk[s]..sys_blk[s+1] are system s's workgroups. */
+typedef struct tm_pcg_view {
+  int32_t dtype;             /* enum tm_dtype of values and vectors */
+  int32_t reserved;
+  int64_t n_rows;            /* total rows of all systems */
+  int64_t n_sys;
+  int64_t n_blocks;
+  const int32_t* row_ptr;    /* [n_rows + 1] global CSR (block diagonal) */
+  const int32_t* col_idx;    /* [nnz] global column ids */
+  const void* values;        /* [nnz] */
+  const void* dinv;          /* [n_rows] inverse diagonal (tm_jacobi) */
+  void* r;                   /* [n_rows] workspaces */
+  void* z;
+  void* p;
+  void* q;
+  double* partials;          /* [n_blocks * 3] */
+  double* scalars;           /* [n_sys * 8]: rz, pq, alpha, beta, |r|^2, |b|^2, active, iterations */
+  const int32_t* blk_sys;    /* [n_blocks] */
+  const int32_t* blk_row0;   /* [n_blocks] */
+  const int32_t* blk_row1;   /* [n_blocks] */
+  const int32_t* sys_blk;    /* [n_sys + 1] */
+  double rtol;               /* stop system s once |r_s| <= rtol |b_s| */
+} tm_pcg_view;
+
 int tm_abi_version(void);
 const char* tm_last_error(void);
 
@@ -133,6 +173,12 @@ int tm_pattern_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64
 int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t cap, int32_t* blk_ptr_out,
                    int64_t* n_blocks);
 
+/* Packed column positions of every incidence (host), for tm_rows_view.
+ * B <= 4 and rows of at most 255 entries. */
+int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                    const int32_t* row_ptr, const int32_t* col_idx, const int32_t* inc_ptr, const int32_t* inc,
+                    uint32_t* colpos);
+
 /* Node incidence (host), the scatter map of NodeAssembler's ReduceProjector
  * (tensormesh/assemble/node_assembler.py:665-670) in gather form. */
 int tm_incidence_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
@@ -154,6 +200,39 @@ int tm_incidence_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int
 int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc* form,
                          const tm_pattern_view* pat, void* slot_ws, void* values,
                          tm_stream_t stream);
+
+/* Fused row-centric P1 assembly (scalar forms on TRI3 / TET4): same values,
+ * same entry order and the same per-entry summation order as
+ * tm_assemble_bilinear, without the slot buffer -- every CSR row walks its
+ * incident elements in ascending order and recomputes their local rows.
+ * Replaces the same reference chain as tm_assemble_bilinear. */
+int tm_assemble_p1_rows(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_rows_view* rows,
+                        void* values, tm_stream_t stream);
+
+/* Adjoint of the P1 scalar assembly (TRI3 / TET4; laplace, diffusion, mass).
+ * Given grad_values = dL/dvalues [nnz] and slot2nnz [n_elem*B*B] (the slot ->
+ * entry map of tm_pattern_build), writes
+ *   grad_coeff  [n_elem] (DIFFUSION_ELEM) or [n_points] (DIFFUSION_NODE), may be NULL
+ *   grad_points [n_points*D], may be NULL
+ * Workspaces: ws_points [n_elem*B*D] when grad_points is wanted, ws_coeff
+ * [n_elem*B] for a nodal coefficient; inc (node incidence) for both node
+ * reductions.  Replaces autograd through the reference's assembly
+ * (element_assembler.py:416-441, element.py:956-968, transformation.py:1265-1272)
+ * driven by the sparse solve's dL/dA_ij = -lambda_i u_j (torch_solve.py:262-272). */
+int tm_assemble_p1_adjoint(const tm_mesh_view* mesh, const tm_form_desc* form, const int32_t* slot2nnz,
+                           const void* grad_values, const tm_incidence_view* inc, void* ws_points,
+                           void* ws_coeff, void* grad_points, void* grad_coeff, tm_stream_t stream);
+
+/* Neo-Hookean residual and consistent tangent at displacement disp [n_points, D].
+ * psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2, mu and lam from form->young /
+ * form->poisson (builtin.py:280-314); the reference obtains R and K by autograd of
+ * NeoHookeanModel.energy (element_assembler.py:447-659).
+ *   tangent  [nnz*D*D]     block data in node-pair CSR order (as LINEAR_ELASTICITY),
+ *                          needs pat and tslot_ws [n_elem*B*B*D*D]; may be NULL
+ *   residual [n_points*D]  dE/du, needs inc and rslot_ws [n_elem*B*D]; may be NULL */
+int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
+                           const tm_pattern_view* pat, const tm_incidence_view* inc, void* tslot_ws,
+                           void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
 
 /* Stage G alone (element matrices into slot_ws).  Same citation as above. */
 int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
@@ -192,6 +271,24 @@ int tm_condense(int32_t dtype, int64_t n_inner, const void* values, const int32_
                 const int32_t* inner_idx, const int32_t* ou_rp, const int32_t* ou_idx, const int32_t* ou_col,
                 const int32_t* inner_dof, const void* rhs, const void* u_o, void* kii, void* f_i,
                 tm_stream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Sparse solve.  The reference inherits solve / @ from torch_sla (not
+ * vendored); its in-tree fallback is the Jacobi-CG of
+ * tensormesh/sparse/solve/torch_solve.py:54-128.
+ *   tm_spmv:        y = A x (CSR, one thread per row)
+ *   tm_jacobi:      dinv[i] = 1 / A_ii (1 where A_ii == 0)
+ *   tm_pcg_init:    r = b - A x, z = dinv r, p = z, per-system scalars
+ *   tm_pcg_iterate: n_iter PCG steps for every still-active system; no host
+ *                   synchronisation, so the caller may capture it in a graph
+ *                   and poll view->scalars between chunks.
+ * ------------------------------------------------------------------- */
+int tm_spmv(int32_t dtype, int64_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+            const void* x, void* y, tm_stream_t stream);
+int tm_jacobi(int32_t dtype, int64_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
+              void* dinv, tm_stream_t stream);
+int tm_pcg_init(const tm_pcg_view* view, const void* b, const void* x, tm_stream_t stream);
+int tm_pcg_iterate(const tm_pcg_view* view, void* x, int32_t n_iter, tm_stream_t stream);
 
 #ifdef __cplusplus
 }

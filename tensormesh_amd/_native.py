@@ -46,6 +46,21 @@ class PatternView(ctypes.Structure):
                 ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
 
 
+class RowsView(ctypes.Structure):
+    _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p),
+                ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("n_blocks", ctypes.c_int64),
+                ("blk_ptr", ctypes.c_void_p)]
+
+
+class PcgView(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int32), ("reserved", ctypes.c_int32), ("n_rows", ctypes.c_int64),
+                ("n_sys", ctypes.c_int64), ("n_blocks", ctypes.c_int64)]
+    _fields_ += [(n, ctypes.c_void_p) for n in ("row_ptr", "col_idx", "values", "dinv", "r", "z", "p", "q",
+                                                 "partials", "scalars", "blk_sys", "blk_row0", "blk_row1",
+                                                 "sys_blk")]
+    _fields_ += [("rtol", ctypes.c_double)]
+
+
 class IncidenceView(ctypes.Structure):
     _fields_ = [("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p),
                 ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
@@ -73,6 +88,21 @@ SIGNATURES = {
     "tm_assemble_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                           ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_colpos_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
+                        + [ctypes.c_void_p] * 5),
+    "tm_assemble_p1_rows": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                           ctypes.POINTER(RowsView), ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_assemble_p1_adjoint": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_spmv": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 6),
+    "tm_jacobi": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 5),
+    "tm_pcg_init": (ctypes.c_int, [ctypes.POINTER(PcgView), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_pcg_iterate": (ctypes.c_int, [ctypes.POINTER(PcgView), ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "tm_assemble_neohookean": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,
+                                              ctypes.POINTER(PatternView), ctypes.POINTER(IncidenceView),
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                              ctypes.c_void_p]),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 

@@ -139,6 +139,14 @@ class _AssemblerBase(nn.Module):
 class ElementAssembler(_AssemblerBase):
     """Bilinear-form assembler; see the module docstring."""
 
+    #: P1 scalar forms: "rows" = fused row-centric kernel (tm_assemble_p1_rows),
+    #: "two_stage" = element kernel + segmented reduction (tm_assemble_bilinear).
+    p1_path: str = "rows"
+    #: True runs every integrand through the restated reference vmap/autograd
+    #: path (torch ops on the same device) -- the "plain" side of a
+    #: measurement, never chosen implicitly.
+    reference_path: bool = False
+
     def __init__(self, elements: Dict[str, torch.Tensor], points: torch.Tensor, pattern: Pattern,
                  quadrature_order: int = 2, *args, **kwargs):
         super().__init__()
@@ -211,7 +219,7 @@ class ElementAssembler(_AssemblerBase):
             assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
 
         fn = self.forward if func is None else func
-        match = self._match(fn, func is None, element_data, point_data, scalar_data)
+        match = None if self.reference_path else self._match(fn, func is None, element_data, point_data, scalar_data)
         if match is not None:
             coeff = None
             if match.coeff_key is not None:
@@ -262,17 +270,58 @@ class ElementAssembler(_AssemblerBase):
             nat.require_device(coeff, match.coeff_key)
             coeff = coeff.detach().to(dtype).contiguous()
         tab, tab_dev = self._tables(dev)
-        slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
-        values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
         mesh = self._mesh_view(points)
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
                             tab.ctypes.data, tab_dev.data_ptr(),
                             None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
+        if H == 1 and self.p1_path == "rows" and self.etype in ("triangle", "tetra"):
+            rv = pat.rows_view(conn)
+            if rv is not None:
+                values = torch.empty(pat.nnz, dtype=dtype, device=dev)
+                nat.check(L.tm_assemble_p1_rows(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
+                                                values.data_ptr(), nat.stream_of(dev)), "tm_assemble_p1_rows")
+                return values
+        slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
+        values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
         pv = pat.view(dtype, H * H)
         nat.check(L.tm_assemble_bilinear(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(pv),
                                          slot.data_ptr(), values.data_ptr(), nat.stream_of(dev)),
                   "tm_assemble_bilinear")
         return values if H == 1 else values.reshape(pat.nnz, H, H)
+
+    def _native_hyperelastic(self, u: torch.Tensor, points: Optional[torch.Tensor], young: float, poisson: float,
+                             want_residual: bool = True, want_tangent: bool = True):
+        """Neo-Hookean residual [N*D] and tangent (block SparseMatrix) at
+        displacement u [N, D] on the HIP kernels (tm_assemble_neohookean)."""
+        points = self._points if points is None else points
+        nat.require_device(points, "points")
+        nat.require_device(u, "displacement")
+        L = nat.lib()
+        dev = points.device
+        dtype = points.dtype
+        points = points.detach().contiguous()
+        u = u.detach().to(dtype).contiguous()
+        assert u.shape == points.shape, f"displacement must be [{points.shape[0]}, {points.shape[1]}], got {tuple(u.shape)}"
+        pat = self._pattern_on(dev)
+        conn = self._conn32(dev)
+        E, B = conn.shape
+        D = self.dimension
+        tab, tab_dev = self._tables(dev)
+        mesh = self._mesh_view(points)
+        form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
+                            None, float(young), float(poisson))
+        pv = pat.view(dtype, D * D)
+        iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
+        tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None
+        tan = torch.empty(pat.nnz * D * D, dtype=dtype, device=dev) if want_tangent else None
+        rslot = torch.empty(E * B * D, dtype=dtype, device=dev) if want_residual else None
+        res = torch.empty(self.n_points * D, dtype=dtype, device=dev) if want_residual else None
+        p = lambda t: None if t is None else t.data_ptr()
+        nat.check(L.tm_assemble_neohookean(ctypes.byref(mesh), ctypes.byref(form), u.data_ptr(), ctypes.byref(pv),
+                                           ctypes.byref(iv), p(tslot), p(tan), p(rslot), p(res),
+                                           nat.stream_of(dev)), "tm_assemble_neohookean")
+        K = self._wrap_output(tan.reshape(pat.nnz, D, D), dev) if want_tangent else None
+        return res, K
 
     def _native_bilinear(self, match, points, coeff) -> SparseMatrix:
         return self._wrap_output(self._native_values(match, points, coeff), points.device)

@@ -81,6 +81,19 @@ class NeoHookeanModel(ElementAssembler):
     def energy(self, u):
         return super().energy(point_data={"displacement": u})
 
+    # The reference obtains these by autograd of ``energy`` (element_assembler.py:447-659);
+    # here they are assembled directly on the HIP kernels.
+    def residual(self, u, points=None):
+        """dE/du, flattened [N*D] (node-major, as NodeAssembler vector output)."""
+        return self._native_hyperelastic(u, points, self.E, self.nu, True, False)[0]
+
+    def tangent(self, u, points=None):
+        """d2E/du2 as a block SparseMatrix [N*D, N*D] in the reference's block layout."""
+        return self._native_hyperelastic(u, points, self.E, self.nu, False, True)[1]
+
+    def residual_and_tangent(self, u, points=None):
+        return self._native_hyperelastic(u, points, self.E, self.nu, True, True)
+
 
 def const_node_assembler(c=1):
     """NodeAssembler subclass for f_i = int c N_i (builtin.py:730-770)."""

@@ -152,6 +152,234 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   }
 }
 
+// Neo-Hookean residual and consistent tangent, one wavefront per element.
+// Replaces autograd of NeoHookeanModel.energy (builtin.py:301-333,
+// element_assembler.py:631-657): psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2
+// with F = I + grad u, grad u[a][d] = sum_b u[n_b][a] dphi_b/dx_d.
+//   P   = mu F + (lam ln J - mu) F^-T
+//   R_i[a]       = sum_q jxw_q sum_A P_q[a][A] g_q,i[A]
+//   K_ij[a][b]   = sum_q jxw_q ( mu delta_ab g_i.g_j + lam h_i[a] h_j[b] - c2 h_i[b] h_j[a] )
+// with h_i = F^-T g_i  (h_i[a] = sum_A g_i[A] Finv[A][a]) and c2 = lam ln J - mu:
+// the 9x9 tangent dP/dF = mu I + lam F^-T (x) F^-T - c2 (F^-1 swapped) contracted
+// with the shape gradients on both sides.  At u = 0 it equals the linear
+// elasticity block.  Tangent layout as the block forms: ((e*B+i)*B+j)*D*D + a*D + b;
+// residual slots [e][i][a] summed into [N][D] by the node incidence reduction.
+template <typename T, int D, int B>
+__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+    hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
+                      const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem,
+                      T* __restrict__ tslot, T* __restrict__ rslot) {
+  __shared__ double s_tab[kMaxTab];
+  __shared__ double s_x[kWavesPerBlock][B * D];
+  __shared__ double s_u[kWavesPerBlock][B * D];
+  __shared__ double s_inv[kWavesPerBlock][kMaxQ][D * D];
+  __shared__ double s_jxw[kWavesPerBlock][kMaxQ];
+  __shared__ double s_g[kWavesPerBlock][kMaxQ * B * D];
+  __shared__ double s_h[kWavesPerBlock][kMaxQ * B * D];
+  __shared__ double s_P[kWavesPerBlock][kMaxQ][D * D];
+  __shared__ double s_c2[kWavesPerBlock][kMaxQ];
+  __shared__ double s_Fi[kWavesPerBlock][kMaxQ][D * D];
+  __shared__ int s_v[kWavesPerBlock][B];
+
+  const int lane = threadIdx.x % kWave;
+  const int wv = threadIdx.x / kWave;
+  const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  const bool active = e < n_elem;
+  const int tab_len = Q + Q * B + Q * D * B;
+  for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
+  const double* w = s_tab;
+  const double* dphi = s_tab + Q + Q * B;
+  if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
+  __syncthreads();
+  if (active && lane < B * D) {
+    const int b = lane / D, d = lane % D;
+    s_x[wv][lane] = (double)pts[(int64_t)s_v[wv][b] * D + d];
+    s_u[wv][lane] = (double)disp[(int64_t)s_v[wv][b] * D + d];
+  }
+  __syncthreads();
+  if (active && lane < Q) {
+    const int q = lane;
+    double J[D][D];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        double s = 0.0;
+        for (int b = 0; b < B; ++b) s += s_x[wv][b * D + j] * dphi[(q * D + i) * B + b];
+        J[i][j] = s;
+      }
+    double inv[D][D];
+    double det;
+    if constexpr (D == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      inv[0][0] = J[1][1] / det;
+      inv[0][1] = -J[0][1] / det;
+      inv[1][0] = -J[1][0] / det;
+      inv[1][1] = J[0][0] / det;
+    } else {
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      const double r = 1.0 / det;
+      inv[0][0] = c00 * r;
+      inv[1][0] = c01 * r;
+      inv[2][0] = c02 * r;
+      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+      for (int j = 0; j < D; ++j) s_inv[wv][q][i * D + j] = inv[i][j];
+    s_jxw[wv][q] = w[q] * fabs(det);
+  }
+  __syncthreads();
+  if (active) {
+    for (int t = lane; t < Q * B * D; t += kWave) {
+      const int q = t / (B * D), b = (t / D) % B, d = t % D;
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) s += dphi[(q * D + i) * B + b] * s_inv[wv][q][d * D + i];
+      s_g[wv][t] = s;
+    }
+  }
+  __syncthreads();
+  if (active && lane < Q) {
+    const int q = lane;
+    const double* g = &s_g[wv][q * B * D];
+    double F[D][D];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double s = (a == d) ? 1.0 : 0.0;
+        for (int b = 0; b < B; ++b) s += s_u[wv][b * D + a] * g[b * D + d];
+        F[a][d] = s;
+      }
+    double Fi[D][D];
+    double Jd;
+    if constexpr (D == 2) {
+      Jd = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+      Fi[0][0] = F[1][1] / Jd;
+      Fi[0][1] = -F[0][1] / Jd;
+      Fi[1][0] = -F[1][0] / Jd;
+      Fi[1][1] = F[0][0] / Jd;
+    } else {
+      const double c00 = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+      const double c01 = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+      const double c02 = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+      Jd = F[0][0] * c00 + F[0][1] * c01 + F[0][2] * c02;
+      const double r = 1.0 / Jd;
+      Fi[0][0] = c00 * r;
+      Fi[1][0] = c01 * r;
+      Fi[2][0] = c02 * r;
+      Fi[0][1] = (F[0][2] * F[2][1] - F[0][1] * F[2][2]) * r;
+      Fi[1][1] = (F[0][0] * F[2][2] - F[0][2] * F[2][0]) * r;
+      Fi[2][1] = (F[0][1] * F[2][0] - F[0][0] * F[2][1]) * r;
+      Fi[0][2] = (F[0][1] * F[1][2] - F[0][2] * F[1][1]) * r;
+      Fi[1][2] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * r;
+      Fi[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * r;
+    }
+    const double c2 = lam * log(Jd) - mu;
+    s_c2[wv][q] = c2;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int A = 0; A < D; ++A) {
+        s_P[wv][q][a * D + A] = mu * F[a][A] + c2 * Fi[A][a];
+        s_Fi[wv][q][a * D + A] = Fi[a][A];
+      }
+  }
+  __syncthreads();
+  if (active) {
+    for (int t = lane; t < Q * B * D; t += kWave) {
+      const int q = t / (B * D), b = (t / D) % B, a = t % D;
+      const double* g = &s_g[wv][(q * B + b) * D];
+      double s = 0.0;
+#pragma unroll
+      for (int A = 0; A < D; ++A) s += g[A] * s_Fi[wv][q][A * D + a];
+      s_h[wv][t] = s;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  if (rslot) {
+    for (int t = lane; t < B * D; t += kWave) {
+      const int i = t / D, a = t % D;
+      double acc = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const double* g = &s_g[wv][(q * B + i) * D];
+        double s = 0.0;
+#pragma unroll
+        for (int A = 0; A < D; ++A) s += s_P[wv][q][a * D + A] * g[A];
+        acc += s * s_jxw[wv][q];
+      }
+      rslot[e * (B * D) + t] = (T)acc;
+    }
+  }
+  if (tslot) {
+    constexpr int DD = D * D;
+    T* out = tslot + e * (int64_t)(B * B * DD);
+    for (int t = lane; t < B * B * DD; t += kWave) {
+      const int ij = t / DD, ab = t % DD;
+      const int i = ij / B, j = ij % B, a = ab / D, b = ab % D;
+      double acc = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const double* gi = &s_g[wv][(q * B + i) * D];
+        const double* gj = &s_g[wv][(q * B + j) * D];
+        const double* hi = &s_h[wv][(q * B + i) * D];
+        const double* hj = &s_h[wv][(q * B + j) * D];
+        double val = lam * hi[a] * hj[b] - s_c2[wv][q] * hi[b] * hj[a];
+        if (a == b) {
+          double dot = 0.0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) dot += gi[d] * gj[d];
+          val += mu * dot;
+        }
+        acc += val * s_jxw[wv][q];
+      }
+      out[t] = (T)acc;
+    }
+  }
+}
+
+template <typename T, int D, int B>
+int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
+                 hipStream_t stream) {
+  const int Q = f->n_quad;
+  if (Q <= 0 || Q > kMaxQ) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
+  if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
+  const int64_t grid = (m->n_elem + kWavesPerBlock - 1) / kWavesPerBlock;
+  if (grid == 0) return TM_OK;
+  const double mu = f->young / (2.0 * (1.0 + f->poisson));
+  const double lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
+  hyper_wave_kernel<T, D, B><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
+      m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("neo-hookean kernel launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+}
+
+template <typename T>
+int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
+                   hipStream_t s) {
+  switch (m->cell) {
+    case TM_TRI3: return launch_hyper<T, 2, 3>(m, f, disp, tslot, rslot, s);
+    case TM_TET4: return launch_hyper<T, 3, 4>(m, f, disp, tslot, rslot, s);
+    case TM_TRI6: return launch_hyper<T, 2, 6>(m, f, disp, tslot, rslot, s);
+    case TM_TET10: return launch_hyper<T, 3, 10>(m, f, disp, tslot, rslot, s);
+    default: return tmh::fail(TM_ERR_ARG, "unknown cell");
+  }
+}
+
 template <typename T, int D, int B>
 int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream,
                            int* hh_out) {
@@ -215,4 +443,34 @@ int tm_p2_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, v
                            hipStream_t stream, int* hh_out) {
   if (mesh->dtype == TM_F64) return dispatch_cell<double>(mesh, form, slot_ws, stream, hh_out);
   return dispatch_cell<float>(mesh, form, slot_ws, stream, hh_out);
+}
+
+extern "C" int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
+                                      const tm_pattern_view* pat, const tm_incidence_view* inc, void* tslot_ws,
+                                      void* tangent, void* rslot_ws, void* residual, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (!mesh || !form || !disp) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  if (tmh::cell_basis(mesh->cell) < 0) return tmh::fail(TM_ERR_ARG, "unknown cell");
+  const int64_t B = tmh::cell_basis(mesh->cell), D = tmh::cell_dim(mesh->cell);
+  if (mesh->n_elem * B * B >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "mesh too large");
+  if (tangent && (!pat || !tslot_ws)) return tmh::fail(TM_ERR_ARG, "tangent needs a pattern and a slot workspace");
+  if (residual && (!inc || !rslot_ws)) return tmh::fail(TM_ERR_ARG, "residual needs incidence and a workspace");
+  void* ts = tangent ? tslot_ws : nullptr;
+  void* rs = residual ? rslot_ws : nullptr;
+  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, stream)
+                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, stream);
+  if (st) return st;
+  if (tangent) {
+    st = tm_segmented_reduce(mesh->dtype, pat->nnz, (int32_t)(D * D), pat->seg_ptr, pat->seg_slot, pat->n_blocks,
+                             pat->blk_ptr, tslot_ws, tangent, stream_);
+    if (st) return st;
+  }
+  if (residual) {
+    st = tm_segmented_reduce(mesh->dtype, mesh->n_points, (int32_t)D, inc->inc_ptr, inc->inc, 0, nullptr, rslot_ws,
+                             residual, stream_);
+    if (st) return st;
+  }
+  return TM_OK;
+  TM_TRY_END
 }

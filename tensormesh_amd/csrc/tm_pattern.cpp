@@ -207,3 +207,35 @@ extern "C" int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t
   return TM_OK;
   TM_TRY_END
 }
+
+extern "C" int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                               const int32_t* row_ptr, const int32_t* col_idx, const int32_t* inc_ptr,
+                               const int32_t* inc, uint32_t* colpos) {
+  TM_TRY_BEGIN
+  if (!conn || !row_ptr || !col_idx || !inc_ptr || !inc || !colpos) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  if (n_basis > 4) return tmh::fail(TM_ERR_UNSUPPORTED, "colpos packs at most 4 positions");
+  for (int64_t r = 0; r < n_points; ++r)
+    if (row_ptr[r + 1] - row_ptr[r] > 255) return tmh::fail(TM_ERR_SIZE, "a row has more than 255 entries");
+  int bad = 0;
+  parallel_rows(n_points, [&](int64_t a, int64_t b) {
+    for (int64_t r = a; r < b; ++r) {
+      const int32_t* cb = col_idx + row_ptr[r];
+      const int32_t* ce = col_idx + row_ptr[r + 1];
+      for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+        const int64_t e = inc[t] / n_basis;
+        uint32_t word = 0;
+        for (int32_t j = 0; j < n_basis; ++j) {
+          const int32_t c = conn[e * n_basis + j];
+          const int32_t* it = std::lower_bound(cb, ce, c);
+          if (it == ce || *it != c) { bad = 1; continue; }
+          word |= (uint32_t)(it - cb) << (8 * j);
+        }
+        colpos[t] = word;
+      }
+    }
+  });
+  if (bad) return tmh::fail(TM_ERR_ARG, "pattern does not contain an element pair");
+  (void)n_elem;
+  return TM_OK;
+  TM_TRY_END
+}

@@ -139,9 +139,11 @@ int launch(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot,
            const int32_t* blk_ptr, const void* slot, void* out, hipStream_t stream) {
   switch (hh) {
     case 1: return launch_hh<T, 1>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+    case 2: return launch_hh<T, 2>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+    case 3: return launch_hh<T, 3>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
     case 4: return launch_hh<T, 4>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
     case 9: return launch_hh<T, 9>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-    default: return tmh::fail(TM_ERR_UNSUPPORTED, "block size must be 1, 2x2 or 3x3");
+    default: return tmh::fail(TM_ERR_UNSUPPORTED, "values per slot must be 1, 2, 3, 4 or 9");
   }
 }
 

@@ -80,14 +80,15 @@ def test_cfg1_laplace_load_condense_against_golden():
     assert rel < 1e-2
 
 
-@pytest.mark.parametrize("q", [1, 2])
-def test_tet_diffusion_and_mass_against_golden(q):
-    say(f"case tet3 diffusion/mass qorder {q}")
+@pytest.mark.parametrize("q,path", [(1, "rows"), (2, "rows"), (1, "two_stage"), (2, "two_stage")])
+def test_tet_diffusion_and_mass_against_golden(q, path):
+    say(f"case tet3 diffusion/mass qorder {q} {path}")
     d = np.load(GOLD / "tet3_diffusion.npz")
     mesh = Mesh(torch.from_numpy(d["points"]), {"tetra": torch.from_numpy(d["cells"])}).to(DEV)
     ke = torch.from_numpy(d["kappa_e"]).to(DEV)
     kn = torch.from_numpy(d["kappa_n"]).to(DEV)
     A = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=q)
+    A.p1_path = path
     Ke = A(element_data={"kappa": ke})
     assert np.array_equal(Ke.row.cpu().numpy(), d["row"]) and np.array_equal(Ke.col.cpu().numpy(), d["col"])
     ok, err = close64(Ke.values, torch.from_numpy(d[f"diff_e_q{q}"]))
@@ -95,7 +96,9 @@ def test_tet_diffusion_and_mass_against_golden(q):
     Kn = A(point_data={"kappa": kn})
     ok, err = close64(Kn.values, torch.from_numpy(d[f"diff_n_q{q}"]))
     assert ok, err
-    M = tm.MassElementAssembler.from_mesh(mesh, quadrature_order=q)()
+    MA = tm.MassElementAssembler.from_mesh(mesh, quadrature_order=q)
+    MA.p1_path = path
+    M = MA()
     ok, err = close64(M.values, torch.from_numpy(d[f"mass_q{q}"]))
     assert ok, err
 
@@ -144,13 +147,15 @@ def _cfg2(dtype=torch.float64):
     return p, c, kap
 
 
-def test_cfg2_full_size_fp64_against_oracle():
-    say("case cfg2 64^3 Kuhn tets fp64 vs oracle")
+@pytest.mark.parametrize("path", ["rows", "two_stage"])
+def test_cfg2_full_size_fp64_against_oracle(path):
+    say(f"case cfg2 64^3 Kuhn tets fp64 vs oracle, {path} path")
     p, c, kap = _cfg2()
     P, C, Kp = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(kap)
     ref_v, ref_r, ref_c, _ = fem.assemble_bilinear(P, C, "tetra", 1, "diffusion_elem", kappa=Kp)
     mesh = Mesh(P, {"tetra": C}).to(DEV)
     A = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=1)
+    A.p1_path = path
     K = A(element_data={"kappa": Kp.to(DEV)})
     assert K.nnz == 4018753
     assert torch.equal(K.row.cpu(), ref_r) and torch.equal(K.col.cpu(), ref_c)
@@ -164,15 +169,18 @@ def test_cfg2_full_size_fp64_against_oracle():
     assert rs.abs().max().item() < 1e-12 * K.values.abs().max().item()
 
 
-def test_cfg2_fp32_error_bound_against_fp64_oracle():
-    say("case cfg2 fp32 vs fp64 oracle")
+@pytest.mark.parametrize("path", ["rows", "two_stage"])
+def test_cfg2_fp32_error_bound_against_fp64_oracle(path):
+    say(f"case cfg2 fp32 vs fp64 oracle, {path} path")
     p, c, kap = _cfg2()
     # the fp32 path sees fp32 inputs; the fp64 oracle runs on the same (fp32-representable) inputs
     p32, k32 = p.astype(np.float32), kap.astype(np.float32)
     P, C, Kp = torch.from_numpy(p32.astype(np.float64)), torch.from_numpy(c), torch.from_numpy(k32.astype(np.float64))
     ref_v, _, _, _ = fem.assemble_bilinear(P, C, "tetra", 1, "diffusion_elem", kappa=Kp)
     mesh = Mesh(torch.from_numpy(p32), {"tetra": C}).to(DEV)
-    K = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=1)(element_data={"kappa": torch.from_numpy(k32).to(DEV)})
+    A = tm.DiffusionElementAssembler.from_mesh(mesh, quadrature_order=1)
+    A.p1_path = path
+    K = A(element_data={"kappa": torch.from_numpy(k32).to(DEV)})
     assert K.dtype == torch.float32
     d = (K.values.double().cpu() - ref_v)
     normwise = (d.norm() / ref_v.norm()).item()
@@ -181,3 +189,37 @@ def test_cfg2_fp32_error_bound_against_fp64_oracle():
     entry = (d[big].abs() / ref_v[big].abs()).max().item()
     say(f"cfg2 fp32: normwise {normwise:.2e}  max|d|/max|K| {maxrel:.2e}  entrywise(|K|>1e-3max) {entry:.2e}")
     assert normwise < 1e-6 and maxrel < 1e-6 and entry < 1e-5
+
+
+@pytest.mark.parametrize("etype", ["tetra", "tetra10", "triangle6"])
+def test_neohookean_residual_tangent_against_oracle_autograd(etype):
+    say(f"case neo-hookean {etype}")
+    from tensormesh_amd.mesh import unit_square_tri
+    order = 2 if etype in ("tetra10", "triangle6") else 1
+    m = unit_square_tri(3, 0.1, 2, order=order) if etype.startswith("tri") else unit_cube_tet(2, 0.1, 2, order=order)
+    g = torch.Generator().manual_seed(5)
+    # admissible deformation (det F > 0 at every quadrature point): a smooth
+    # field plus a small random part
+    P = m.points
+    u = 0.05 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, -1], 0.5 * P[:, 0] ** 2][:P.shape[1]], 1)
+    u = u + 0.002 * torch.randn(m.points.shape, generator=g, dtype=torch.float64)
+    res_e, tan_e = fem.neohookean_element_tangent(m.points, m.elements(), etype, 2, u, E=1.0, nu=0.3)
+    assert torch.isfinite(res_e).all() and torch.isfinite(tan_e).all()
+    row, col, eids = fem.pattern(m.elements().numpy(), m.n_points)
+    D = m.points.shape[1]
+    eids = torch.from_numpy(eids)
+    tan = fem._index_add(tan_e, eids, row.size)                            # [nnz, D, D]
+    res = torch.zeros(m.n_points, D, dtype=torch.float64).index_add(0, m.elements().reshape(-1),
+                                                                     res_e.reshape(-1, D)).reshape(-1)
+    model = tm.NeoHookeanModel.from_mesh(m.to(DEV), quadrature_order=2, E=1.0, nu=0.3)
+    R, K = model.residual_and_tangent(u.to(DEV))
+    ok, err = close64(R, res)
+    assert ok, err
+    v, r, c = fem.block_coo(tan, torch.from_numpy(row), torch.from_numpy(col))
+    assert torch.equal(K.row.cpu(), r) and torch.equal(K.col.cpu(), c)
+    ok, err = close64(K.values, v, rtol=1e-11)
+    assert ok, err
+    # residual is the gradient of the (torch-path) energy
+    ur = u.clone().requires_grad_(True)
+    tm.NeoHookeanModel.from_mesh(m, quadrature_order=2, E=1.0, nu=0.3).energy(ur).backward()
+    assert torch.allclose(R.cpu(), ur.grad.reshape(-1), rtol=1e-10, atol=1e-13)

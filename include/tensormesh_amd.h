@@ -116,6 +116,7 @@ typedef struct tm_rows_view {
   const int32_t* inc_ptr;    /* [n_points + 1] */
   const int32_t* inc;        /* [n_elem*B] packed e*B + i */
   const uint32_t* colpos;    /* [n_elem*B] */
+  const uint32_t* entry_mask; /* [nnz] bit t: the row's t-th incidence contributes (rows with <= 32 incidences), or NULL */
   int64_t n_blocks;          /* reduction plan over inc_ptr (rows) */
   const int32_t* blk_ptr;    /* [n_blocks + 1] */
 } tm_rows_view;
@@ -173,11 +174,12 @@ int tm_pattern_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64
 int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t cap, int32_t* blk_ptr_out,
                    int64_t* n_blocks);
 
-/* Packed column positions of every incidence (host), for tm_rows_view.
+/* Packed column positions of every incidence (host), for tm_rows_view, and
+ * optionally (entry_mask != NULL) the per-entry incidence masks.
  * B <= 4 and rows of at most 255 entries. */
 int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
                     const int32_t* row_ptr, const int32_t* col_idx, const int32_t* inc_ptr, const int32_t* inc,
-                    uint32_t* colpos);
+                    uint32_t* colpos, uint32_t* entry_mask);
 
 /* Node incidence (host), the scatter map of NodeAssembler's ReduceProjector
  * (tensormesh/assemble/node_assembler.py:665-670) in gather form. */
@@ -233,6 +235,13 @@ int tm_assemble_p1_adjoint(const tm_mesh_view* mesh, const tm_form_desc* form, c
 int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
                            const tm_pattern_view* pat, const tm_incidence_view* inc, void* tslot_ws,
                            void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
+
+/* Row-centric stage S for P1 (TRI3 / TET4): the entries of tm_assemble_bilinear
+ * from a stage-G slot buffer, each CSR row reading its incidences' local rows
+ * (B contiguous values each) and routing them by colpos; same per-entry order
+ * (ascending element).  Replaces ReduceProjector.__call__ (projector.py:95-123). */
+int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot, void* values,
+                      tm_stream_t stream);
 
 /* Stage G alone (element matrices into slot_ws).  Same citation as above. */
 int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,

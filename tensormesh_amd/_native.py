@@ -48,7 +48,8 @@ class PatternView(ctypes.Structure):
 
 class RowsView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p),
-                ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("n_blocks", ctypes.c_int64),
+                ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("entry_mask", ctypes.c_void_p),
+                ("n_blocks", ctypes.c_int64),
                 ("blk_ptr", ctypes.c_void_p)]
 
 
@@ -89,7 +90,7 @@ SIGNATURES = {
                                           ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),
     "tm_colpos_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
-                        + [ctypes.c_void_p] * 5),
+                        + [ctypes.c_void_p] * 6),
     "tm_assemble_p1_rows": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                            ctypes.POINTER(RowsView), ctypes.c_void_p, ctypes.c_void_p]),
     "tm_assemble_p1_adjoint": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,
@@ -103,6 +104,8 @@ SIGNATURES = {
                                               ctypes.POINTER(PatternView), ctypes.POINTER(IncidenceView),
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]),
+    "tm_reduce_p1_rows": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RowsView), ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 

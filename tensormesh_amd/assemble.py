@@ -140,7 +140,8 @@ class ElementAssembler(_AssemblerBase):
     """Bilinear-form assembler; see the module docstring."""
 
     #: P1 scalar forms: "rows" = fused row-centric kernel (tm_assemble_p1_rows),
-    #: "two_stage" = element kernel + segmented reduction (tm_assemble_bilinear).
+    #: "two_stage" = element kernel + segmented reduction (tm_assemble_bilinear),
+    #: "two_stage_rows" = element kernel + row-centric reduction (tm_reduce_p1_rows).
     p1_path: str = "rows"
     #: True runs every integrand through the restated reference vmap/autograd
     #: path (torch ops on the same device) -- the "plain" side of a
@@ -274,12 +275,20 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
                             tab.ctypes.data, tab_dev.data_ptr(),
                             None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
-        if H == 1 and self.p1_path == "rows" and self.etype in ("triangle", "tetra"):
+        if H == 1 and self.p1_path in ("rows", "two_stage_rows") and self.etype in ("triangle", "tetra"):
             rv = pat.rows_view(conn)
             if rv is not None:
                 values = torch.empty(pat.nnz, dtype=dtype, device=dev)
-                nat.check(L.tm_assemble_p1_rows(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
-                                                values.data_ptr(), nat.stream_of(dev)), "tm_assemble_p1_rows")
+                if self.p1_path == "rows":
+                    nat.check(L.tm_assemble_p1_rows(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
+                                                    values.data_ptr(), nat.stream_of(dev)), "tm_assemble_p1_rows")
+                else:
+                    slot = torch.empty(E * B * B, dtype=dtype, device=dev)
+                    st = nat.stream_of(dev)
+                    nat.check(L.tm_element_matrices(ctypes.byref(mesh), ctypes.byref(form), slot.data_ptr(), st),
+                              "tm_element_matrices")
+                    nat.check(L.tm_reduce_p1_rows(nat.dtype_code(dtype), nat.CELL[self.etype], ctypes.byref(rv),
+                                                  slot.data_ptr(), values.data_ptr(), st), "tm_reduce_p1_rows")
                 return values
         slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
         values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)

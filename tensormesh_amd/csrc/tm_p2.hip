@@ -126,20 +126,45 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   if (!active) return;
   constexpr int HH = H * H;
   T* out = slot + e * (int64_t)(B * B * HH);
+  if constexpr (FORM == TM_FORM_LINEAR_ELASTICITY) {
+    // one lane per node pair: the H x H block from one read of g_i, g_j
+    for (int ij = lane; ij < B * B; ij += kWave) {
+      const int i = ij / B, j = ij % B;
+      double acc[HH];
+#pragma unroll
+      for (int k = 0; k < HH; ++k) acc[k] = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const double* g = &s_g[wv][q * B * D];
+        double gi[D], gj[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          gi[d] = g[i * D + d];
+          gj[d] = g[j * D + d];
+        }
+        double dot = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) dot += gi[d] * gj[d];
+        const double w = s_jxw[wv][q];
+#pragma unroll
+        for (int a = 0; a < H; ++a)
+#pragma unroll
+          for (int c = 0; c < H; ++c)
+            acc[a * H + c] += (lam * gi[a] * gj[c] + mu * gi[c] * gj[a] + (a == c ? mu * dot : 0.0)) * w;
+      }
+#pragma unroll
+      for (int k = 0; k < HH; ++k) out[ij * HH + k] = (T)acc[k];
+    }
+    return;
+  }
   for (int t = lane; t < B * B * HH; t += kWave) {
-    const int ij = t / HH, ac = t % HH;
-    const int i = ij / B, j = ij % B, a = ac / H, c = ac % H;
+    const int ij = t / HH;
+    const int i = ij / B, j = ij % B;
     double acc = 0.0;
     for (int q = 0; q < Q; ++q) {
       const double* g = &s_g[wv][q * B * D];
       double val;
       if constexpr (FORM == TM_FORM_MASS) {
         val = phi[q * B + i] * phi[q * B + j];
-      } else if constexpr (FORM == TM_FORM_LINEAR_ELASTICITY) {
-        double dot = 0.0;
-#pragma unroll
-        for (int d = 0; d < D; ++d) dot += g[i * D + d] * g[j * D + d];
-        val = lam * g[i * D + a] * g[j * D + c] + mu * g[i * D + c] * g[j * D + a] + (a == c ? mu * dot : 0.0);
       } else {
         double dot = 0.0;
 #pragma unroll
@@ -323,27 +348,42 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     }
   }
   if (tslot) {
+    // one lane per node pair (i, j): the whole D x D block from one read of
+    // g_i, g_j, h_i, h_j per quadrature point
     constexpr int DD = D * D;
     T* out = tslot + e * (int64_t)(B * B * DD);
-    for (int t = lane; t < B * B * DD; t += kWave) {
-      const int ij = t / DD, ab = t % DD;
-      const int i = ij / B, j = ij % B, a = ab / D, b = ab % D;
-      double acc = 0.0;
+    for (int ij = lane; ij < B * B; ij += kWave) {
+      const int i = ij / B, j = ij % B;
+      double acc[DD];
+#pragma unroll
+      for (int k = 0; k < DD; ++k) acc[k] = 0.0;
       for (int q = 0; q < Q; ++q) {
         const double* gi = &s_g[wv][(q * B + i) * D];
         const double* gj = &s_g[wv][(q * B + j) * D];
         const double* hi = &s_h[wv][(q * B + i) * D];
         const double* hj = &s_h[wv][(q * B + j) * D];
-        double val = lam * hi[a] * hj[b] - s_c2[wv][q] * hi[b] * hj[a];
-        if (a == b) {
-          double dot = 0.0;
+        double vgi[D], vgj[D], vhi[D], vhj[D];
 #pragma unroll
-          for (int d = 0; d < D; ++d) dot += gi[d] * gj[d];
-          val += mu * dot;
+        for (int d = 0; d < D; ++d) {
+          vgi[d] = gi[d];
+          vgj[d] = gj[d];
+          vhi[d] = hi[d];
+          vhj[d] = hj[d];
         }
-        acc += val * s_jxw[wv][q];
+        const double w = s_jxw[wv][q];
+        const double lw = lam * w, cw = s_c2[wv][q] * w;
+        double dot = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) dot += vgi[d] * vgj[d];
+        const double mw = mu * dot * w;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int b = 0; b < D; ++b)
+            acc[a * D + b] += lw * vhi[a] * vhj[b] - cw * vhi[b] * vhj[a] + (a == b ? mw : 0.0);
       }
-      out[t] = (T)acc;
+#pragma unroll
+      for (int k = 0; k < DD; ++k) out[ij * DD + k] = (T)acc[k];
     }
   }
 }

@@ -210,7 +210,7 @@ extern "C" int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t
 
 extern "C" int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
                                const int32_t* row_ptr, const int32_t* col_idx, const int32_t* inc_ptr,
-                               const int32_t* inc, uint32_t* colpos) {
+                               const int32_t* inc, uint32_t* colpos, uint32_t* entry_mask) {
   TM_TRY_BEGIN
   if (!conn || !row_ptr || !col_idx || !inc_ptr || !inc || !colpos) return tmh::fail(TM_ERR_ARG, "NULL argument");
   if (n_basis > 4) return tmh::fail(TM_ERR_UNSUPPORTED, "colpos packs at most 4 positions");
@@ -231,6 +231,17 @@ extern "C" int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_ba
           word |= (uint32_t)(it - cb) << (8 * j);
         }
         colpos[t] = word;
+      }
+      if (entry_mask) {
+        // bit (t - inc_ptr[r]) of entry (r, p): incidence t contributes to it
+        const int32_t k0 = row_ptr[r], nk = row_ptr[r + 1] - row_ptr[r];
+        for (int32_t p = 0; p < nk; ++p) entry_mask[k0 + p] = 0u;
+        const int32_t n_inc = inc_ptr[r + 1] - inc_ptr[r];
+        if (n_inc <= 32) {
+          for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t)
+            for (int32_t j = 0; j < n_basis; ++j)
+              entry_mask[k0 + ((colpos[t] >> (8 * j)) & 0xffu)] |= 1u << (t - inc_ptr[r]);
+        }
       }
     }
   });

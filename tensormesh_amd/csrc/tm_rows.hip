@@ -23,7 +23,9 @@
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kCapInc = 1024;  // incidences staged per workgroup
+constexpr int kCapInc = 512;   // incidences staged per workgroup (matches the host plan cap)
+constexpr int kMaxRows = 256;  // rows staged per workgroup
+constexpr int kMaxEnt = 2048;  // CSR entries staged per workgroup
 
 struct RowConsts {
   double W;        // sum_q w_q
@@ -133,17 +135,21 @@ __global__ void __launch_bounds__(kBlock)
     p1_rows_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
                    RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
                    const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
-                   const int32_t* __restrict__ blk_ptr, int64_t n_blocks, T* __restrict__ values) {
+                   const uint32_t* __restrict__ entry_mask, const int32_t* __restrict__ blk_ptr, int64_t n_blocks,
+                   T* __restrict__ values) {
   constexpr int B = D + 1;
   __shared__ T s_val[kCapInc * B];
   __shared__ uint32_t s_pos[kCapInc];
+  __shared__ int32_t s_rp[kMaxRows + 1];    // row starts, relative to k0
+  __shared__ int32_t s_ip[kMaxRows + 1];    // incidence starts, relative to t0
+  __shared__ uint16_t s_erow[kMaxEnt];      // entry -> local row
   const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
   const int32_t r0 = blk_ptr[blk], r1 = blk_ptr[blk + 1];
   const int32_t t0 = inc_ptr[r0], t1 = inc_ptr[r1];
-  const int32_t n_inc = t1 - t0;
   const int32_t k0 = row_ptr[r0], k1 = row_ptr[r1];
-  const bool staged = n_inc <= kCapInc;
-  if (staged) {
+  const int32_t n_inc = t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
+  if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
+    // phase 1: local rows of every incidence, plus the row / incidence maps
     for (int32_t t = threadIdx.x; t < n_inc; t += kBlock) {
       const int32_t ei = inc[t0 + t];
       double Kr[B];
@@ -152,10 +158,46 @@ __global__ void __launch_bounds__(kBlock)
       for (int j = 0; j < B; ++j) s_val[t * B + j] = (T)Kr[j];
       s_pos[t] = colpos[t0 + t];
     }
+    for (int32_t lr = threadIdx.x; lr <= n_rows; lr += kBlock) {
+      s_rp[lr] = row_ptr[r0 + lr] - k0;
+      s_ip[lr] = inc_ptr[r0 + lr] - t0;
+    }
     __syncthreads();
+    for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock)
+      for (int32_t k = s_rp[lr]; k < s_rp[lr + 1]; ++k) s_erow[k] = (uint16_t)lr;
+    __syncthreads();
+    // phase 2: one thread per entry, incidences in ascending element order
+    for (int32_t kl = threadIdx.x; kl < n_ent; kl += kBlock) {
+      const int32_t lr = s_erow[kl];
+      const uint32_t p = (uint32_t)(kl - s_rp[lr]);
+      const int32_t ta = s_ip[lr], tz = s_ip[lr + 1];
+      T acc = T(0);
+      if (entry_mask && tz - ta <= 32) {
+        // only the incidences that contribute, lowest bit (= lowest element) first
+        uint32_t m = entry_mask[k0 + kl];
+        while (m) {
+          const int32_t t = ta + __builtin_ctz(m);
+          m &= m - 1u;
+          const uint32_t cp = s_pos[t];
+          int jj = 0;
+#pragma unroll
+          for (int j = 1; j < B; ++j) jj = (((cp >> (8 * j)) & 0xffu) == p) ? j : jj;
+          acc += s_val[t * B + jj];
+        }
+      } else {
+        for (int32_t t = ta; t < tz; ++t) {
+          const uint32_t cp = s_pos[t];
+#pragma unroll
+          for (int j = 0; j < B; ++j)
+            if (((cp >> (8 * j)) & 0xffu) == p) acc += s_val[t * B + j];
+        }
+      }
+      values[k0 + kl] = acc;
+    }
+    return;
   }
+  // oversized run: every entry walks global memory, recomputing local rows
   for (int32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
-    // row of entry k: row_ptr[r] <= k < row_ptr[r+1], r in [r0, r1)
     int32_t lo = r0, hi = r1 - 1;
     while (lo < hi) {
       const int32_t mid = (lo + hi + 1) >> 1;
@@ -165,20 +207,112 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t p = (uint32_t)(k - row_ptr[r]);
     T acc = T(0);
     for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
-      const uint32_t cp = staged ? s_pos[t - t0] : colpos[t];
+      const uint32_t cp = colpos[t];
 #pragma unroll
       for (int j = 0; j < B; ++j) {
         if (((cp >> (8 * j)) & 0xffu) == p) {
-          if (staged) {
-            acc += s_val[(t - t0) * B + j];
-          } else {
-            const int32_t ei = inc[t];
-            double Kr[B];
-            local_row<T, D, FORM>(conn, pts, coeff, c, ei / B, ei % B, Kr);
-            acc += (T)Kr[j];
-          }
+          const int32_t ei = inc[t];
+          double Kr[B];
+          local_row<T, D, FORM>(conn, pts, coeff, c, ei / B, ei % B, Kr);
+          acc += (T)Kr[j];
         }
       }
+    }
+    values[k] = acc;
+  }
+}
+
+// Row-centric stage S for P1 slot buffers: the same staging and per-entry walk
+// as p1_rows_kernel, but phase 1 reads the element rows K_e[i][0..B) from the
+// slot buffer written by stage G (one contiguous B-value row per incidence)
+// instead of recomputing them.
+template <typename T, int D>
+__global__ void __launch_bounds__(kBlock)
+    p1_rows_reduce_kernel(const T* __restrict__ slot, const int32_t* __restrict__ row_ptr,
+                          const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                          const uint32_t* __restrict__ colpos, const uint32_t* __restrict__ entry_mask,
+                          const int32_t* __restrict__ blk_ptr, int64_t n_blocks, T* __restrict__ values) {
+  constexpr int B = D + 1;
+  __shared__ T s_val[kCapInc * B];
+  __shared__ uint32_t s_pos[kCapInc];
+  __shared__ int32_t s_rp[kMaxRows + 1];
+  __shared__ int32_t s_ip[kMaxRows + 1];
+  __shared__ uint16_t s_erow[kMaxEnt];
+  const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
+  const int32_t r0 = blk_ptr[blk], r1 = blk_ptr[blk + 1];
+  const int32_t t0 = inc_ptr[r0], t1 = inc_ptr[r1];
+  const int32_t k0 = row_ptr[r0], k1 = row_ptr[r1];
+  const int32_t n_inc = t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
+  if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
+    for (int32_t t = threadIdx.x; t < n_inc; t += kBlock) {
+      const int32_t ei = inc[t0 + t];        // e*B + i: the slot row starts at (e*B + i)*B
+      const T* src = slot + (int64_t)ei * B;
+      if constexpr (B == 4 && sizeof(T) == 8) {
+        const double2 a = reinterpret_cast<const double2*>(src)[0];
+        const double2 b = reinterpret_cast<const double2*>(src)[1];
+        s_val[t * B + 0] = a.x; s_val[t * B + 1] = a.y;
+        s_val[t * B + 2] = b.x; s_val[t * B + 3] = b.y;
+      } else if constexpr (B == 4 && sizeof(T) == 4) {
+        const float4 a = reinterpret_cast<const float4*>(src)[0];
+        s_val[t * B + 0] = a.x; s_val[t * B + 1] = a.y;
+        s_val[t * B + 2] = a.z; s_val[t * B + 3] = a.w;
+      } else {
+#pragma unroll
+        for (int j = 0; j < B; ++j) s_val[t * B + j] = src[j];
+      }
+      s_pos[t] = colpos[t0 + t];
+    }
+    for (int32_t lr = threadIdx.x; lr <= n_rows; lr += kBlock) {
+      s_rp[lr] = row_ptr[r0 + lr] - k0;
+      s_ip[lr] = inc_ptr[r0 + lr] - t0;
+    }
+    __syncthreads();
+    for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock)
+      for (int32_t k = s_rp[lr]; k < s_rp[lr + 1]; ++k) s_erow[k] = (uint16_t)lr;
+    __syncthreads();
+    for (int32_t kl = threadIdx.x; kl < n_ent; kl += kBlock) {
+      const int32_t lr = s_erow[kl];
+      const uint32_t p = (uint32_t)(kl - s_rp[lr]);
+      const int32_t ta = s_ip[lr], tz = s_ip[lr + 1];
+      T acc = T(0);
+      if (entry_mask && tz - ta <= 32) {
+        // only the incidences that contribute, lowest bit (= lowest element) first
+        uint32_t m = entry_mask[k0 + kl];
+        while (m) {
+          const int32_t t = ta + __builtin_ctz(m);
+          m &= m - 1u;
+          const uint32_t cp = s_pos[t];
+          int jj = 0;
+#pragma unroll
+          for (int j = 1; j < B; ++j) jj = (((cp >> (8 * j)) & 0xffu) == p) ? j : jj;
+          acc += s_val[t * B + jj];
+        }
+      } else {
+        for (int32_t t = ta; t < tz; ++t) {
+          const uint32_t cp = s_pos[t];
+#pragma unroll
+          for (int j = 0; j < B; ++j)
+            if (((cp >> (8 * j)) & 0xffu) == p) acc += s_val[t * B + j];
+        }
+      }
+      values[k0 + kl] = acc;
+    }
+    return;
+  }
+  for (int32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
+    int32_t lo = r0, hi = r1 - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (row_ptr[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    const int32_t r = lo;
+    const uint32_t p = (uint32_t)(k - row_ptr[r]);
+    T acc = T(0);
+    for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+      const uint32_t cp = colpos[t];
+#pragma unroll
+      for (int j = 0; j < B; ++j)
+        if (((cp >> (8 * j)) & 0xffu) == p) acc += slot[(int64_t)inc[t] * B + j];
     }
     values[k] = acc;
   }
@@ -205,7 +339,7 @@ int launch(const tm_mesh_view* m, const tm_form_desc* f, const tm_rows_view* rv,
   if (nb == 0) return TM_OK;
 #define TM_ROWS_LAUNCH(FORM)                                                                              \
   p1_rows_kernel<T, D, FORM><<<nb, kBlock, 0, s>>>(m->conn, pts, coeff, c, rv->row_ptr, rv->inc_ptr, rv->inc, \
-                                                   rv->colpos, rv->blk_ptr, nb, out)
+                                                   rv->colpos, rv->entry_mask, rv->blk_ptr, nb, out)
   switch (f->form) {
     case TM_FORM_LAPLACE: TM_ROWS_LAUNCH(TM_FORM_LAPLACE); break;
     case TM_FORM_DIFFUSION_ELEM:
@@ -246,5 +380,36 @@ extern "C" int tm_assemble_p1_rows(const tm_mesh_view* mesh, const tm_form_desc*
                                  : launch<double, 2>(mesh, form, rows, values, stream);
   return mesh->cell == TM_TET4 ? launch<float, 3>(mesh, form, rows, values, stream)
                                : launch<float, 2>(mesh, form, rows, values, stream);
+  TM_TRY_END
+}
+
+extern "C" int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot,
+                                 void* values, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (!rows) return tmh::fail(TM_ERR_ARG, "NULL rows view");
+  if (cell != TM_TRI3 && cell != TM_TET4) return tmh::fail(TM_ERR_UNSUPPORTED, "row reduction is P1 only");
+  const int64_t nb = rows->n_blocks;
+  if (nb == 0) return TM_OK;
+  if (!slot || !values || !rows->row_ptr || !rows->inc_ptr || !rows->inc || !rows->colpos || !rows->blk_ptr)
+    return tmh::fail(TM_ERR_ARG, "NULL buffer");
+#define TM_RR(T, D)                                                                                          \
+  p1_rows_reduce_kernel<T, D><<<nb, kBlock, 0, stream>>>((const T*)slot, rows->row_ptr, rows->inc_ptr, rows->inc, \
+                                                         rows->colpos, rows->entry_mask, rows->blk_ptr, nb,      \
+                                                         (T*)values)
+  if (dtype == TM_F64) {
+    if (cell == TM_TET4) TM_RR(double, 3); else TM_RR(double, 2);
+  } else if (dtype == TM_F32) {
+    if (cell == TM_TET4) TM_RR(float, 3); else TM_RR(float, 2);
+  } else {
+    return tmh::fail(TM_ERR_ARG, "unknown dtype");
+  }
+#undef TM_RR
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("row reduction launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
   TM_TRY_END
 }

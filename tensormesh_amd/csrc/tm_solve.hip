@@ -148,24 +148,44 @@ __global__ void __launch_bounds__(kBlock)
   if (i < blk_row1[blk]) p[i] = z[i] + beta * p[i];
 }
 
-// Per-system reductions of the workgroup partials, in workgroup order.
+// Per-system reductions of the workgroup partials: one workgroup per system,
+// thread t adds partials t, t+256, ... in order, then a fixed LDS tree -- the
+// same order every run.
 //   stage 0 (after init):   rz, bb, rr; active = rr > tol^2 bb
 //   stage 1 (after spmv):   pq; alpha = rz / pq
 //   stage 2 (after update): rz_new, rr; beta = rz_new / rz; rz = rz_new;
 //                           iterations += 1; active = rr > tol^2 bb
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kBlock)
     pcg_scalar_kernel(int32_t n_sys, const int32_t* __restrict__ sys_blk, const double* __restrict__ part,
                       double* __restrict__ sc, double tol2, int stage) {
-  const int32_t s = blockIdx.x * 64 + threadIdx.x;
+  __shared__ double sh[3][kBlock];
+  const int32_t s = blockIdx.x;
   if (s >= n_sys) return;
   double* c = sc + s * kNS;
-  if (stage != 0 && c[kActive] == 0.0) return;
+  if (stage != 0 && c[kActive] == 0.0) return;  // uniform per workgroup
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (int32_t b = sys_blk[s]; b < sys_blk[s + 1]; ++b) {
+  for (int32_t b = sys_blk[s] + threadIdx.x; b < sys_blk[s + 1]; b += kBlock) {
     a0 += part[b * 3 + 0];
     a1 += part[b * 3 + 1];
     a2 += part[b * 3 + 2];
   }
+  sh[0][threadIdx.x] = a0;
+  sh[1][threadIdx.x] = a1;
+  sh[2][threadIdx.x] = a2;
+  __syncthreads();
+#pragma unroll
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+      sh[2][threadIdx.x] += sh[2][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  a0 = sh[0][0];
+  a1 = sh[1][0];
+  a2 = sh[2][0];
   if (stage == 0) {
     c[kRZ] = a0;
     c[kBB] = a1;
@@ -220,23 +240,23 @@ int pcg_init(const tm_pcg_view* v, const void* b, const void* x, hipStream_t s) 
   pcg_init_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->row_ptr, v->col_idx,
                                            (const T*)v->values, (const T*)b, (const T*)x, (const T*)v->dinv,
                                            (T*)v->r, (T*)v->z, (T*)v->p, v->partials);
-  pcg_scalar_kernel<<<(v->n_sys + 63) / 64, 64, 0, s>>>((int32_t)v->n_sys, v->sys_blk, v->partials, v->scalars,
-                                                        v->rtol * v->rtol, 0);
+  pcg_scalar_kernel<<<v->n_sys, kBlock, 0, s>>>((int32_t)v->n_sys, v->sys_blk, v->partials, v->scalars,
+                                                v->rtol * v->rtol, 0);
   return hip_status("pcg init");
 }
 
 template <typename T>
 int pcg_iterate(const tm_pcg_view* v, void* x, int32_t n_iter, hipStream_t s) {
   const int64_t nb = v->n_blocks;
-  const int gs = (int)((v->n_sys + 63) / 64);
+  const int64_t gs = v->n_sys;
   const double tol2 = v->rtol * v->rtol;
   for (int32_t it = 0; it < n_iter; ++it) {
     pcg_spmv_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->row_ptr, v->col_idx,
                                              (const T*)v->values, (const T*)v->p, (T*)v->q, v->scalars, v->partials);
-    pcg_scalar_kernel<<<gs, 64, 0, s>>>((int32_t)v->n_sys, v->sys_blk, v->partials, v->scalars, tol2, 1);
+    pcg_scalar_kernel<<<gs, kBlock, 0, s>>>((int32_t)v->n_sys, v->sys_blk, v->partials, v->scalars, tol2, 1);
     pcg_update_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, (const T*)v->p, (const T*)v->q,
                                                (const T*)v->dinv, (T*)x, (T*)v->r, (T*)v->z, v->scalars, v->partials);
-    pcg_scalar_kernel<<<gs, 64, 0, s>>>((int32_t)v->n_sys, v->sys_blk, v->partials, v->scalars, tol2, 2);
+    pcg_scalar_kernel<<<gs, kBlock, 0, s>>>((int32_t)v->n_sys, v->sys_blk, v->partials, v->scalars, tol2, 2);
     pcg_dir_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, (const T*)v->z, (T*)v->p,
                                             v->scalars);
   }

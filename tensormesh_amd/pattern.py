@@ -50,7 +50,7 @@ class Pattern:
         self._plans = {}
         self._colpos = None
 
-    ROWS_CAP = 1024   # incidences staged per workgroup by the fused P1 kernel (tm_rows.hip)
+    ROWS_CAP = 512    # incidences staged per workgroup by the row-centric P1 kernels (tm_rows.hip)
 
     def colpos(self, conn: torch.Tensor) -> Optional[torch.Tensor]:
         """Packed per-incidence column positions for the fused P1 path, or
@@ -64,10 +64,15 @@ class Pattern:
                 rp, ci = self.row_ptr.cpu().numpy(), self.col_idx.cpu().numpy()
                 ip, ic = self.inc_ptr.cpu().numpy(), self.inc.cpu().numpy()
                 out = np.empty(ic.shape[0], np.uint32)
+                mask = np.empty(self.nnz, np.uint32)
                 st = nat.lib().tm_colpos_build(c.ctypes.data, c.shape[0], c.shape[1], self.n_points,
                                                rp.ctypes.data, ci.ctypes.data, ip.ctypes.data, ic.ctypes.data,
-                                               out.ctypes.data)
-                self._colpos = torch.from_numpy(out.view(np.int32)).to(self.seg_ptr.device) if st == 0 else False
+                                               out.ctypes.data, mask.ctypes.data)
+                if st == 0:
+                    self._colpos = torch.from_numpy(out.view(np.int32)).to(self.seg_ptr.device)
+                    self._entry_mask = torch.from_numpy(mask.view(np.int32)).to(self.seg_ptr.device)
+                else:
+                    self._colpos = False
         return self._colpos if self._colpos is not False else None
 
     def rows_view(self, conn: torch.Tensor) -> Optional[nat.RowsView]:
@@ -76,7 +81,7 @@ class Pattern:
             return None
         bp = self.plan(self.ROWS_CAP, node=True)
         return nat.RowsView(self.nnz, self.row_ptr.data_ptr(), self.inc_ptr.data_ptr(), self.inc.data_ptr(),
-                            cp.data_ptr(), bp.numel() - 1, bp.data_ptr())
+                            cp.data_ptr(), self._entry_mask.data_ptr(), bp.numel() - 1, bp.data_ptr())
 
     def plan(self, cap: int, node: bool = False) -> torch.Tensor:
         """Reduction plan (blk_ptr, int32, on the pattern's device) for entries

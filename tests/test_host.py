@@ -203,6 +203,20 @@ def test_row_centric_summation_reproduces_oracle_on_cpu():
             vals[k] = acc
     ref, r, c, _ = fem.assemble_bilinear(m.points, conn, "tetra", 1, "laplace")
     assert np.array_equal(vals, ref.numpy())   # same summation order -> bit-identical
+    # the per-entry incidence masks select exactly the contributing incidences
+    mask = pat._entry_mask.numpy().view(np.uint32)
+    vals2 = np.zeros(pat.nnz)
+    for r_ in range(m.n_points):
+        for k in range(rp[r_], rp[r_ + 1]):
+            p, acc, mm = k - rp[r_], 0.0, int(mask[k])
+            while mm:
+                t = ip[r_] + (mm & -mm).bit_length() - 1
+                mm &= mm - 1
+                e, i = divmod(int(inc[t]), 4)
+                j = [jj for jj in range(4) if (int(cp[t]) >> (8 * jj)) & 0xFF == p][0]
+                acc += loc[e, i, j]
+            vals2[k] = acc
+    assert np.array_equal(vals2, ref.numpy())
     # the plan covers every row exactly once, in order
     bp = pat.plan(Pattern.ROWS_CAP, node=True).numpy()
     assert bp[0] == 0 and bp[-1] == m.n_points and np.all(np.diff(bp) > 0)

@@ -348,6 +348,25 @@ def load_traffic():
         return json.load(f)
 
 
+def profile_kernels(reps=5):
+    """Launch each assembly kernel of the workload ``reps`` times, plus a plain
+    256 MiB device-to-device copy (known bytes: 256 MiB read + 256 MiB
+    written) that calibrates the HBM counters, for rocprofv3 runs."""
+    mesh, kappa, A, _, _ = build_workload(torch.float64)
+    ed = {"kappa": kappa}
+    for path in ("rows", "two_stage", "two_stage_rows"):
+        A.p1_path = path
+        for _ in range(reps):
+            A(element_data=ed)
+    src = torch.empty(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"profiled": ["p1_rows_kernel", "p1_element_kernel", "reduce_stream_kernel",
+                                   "p1_rows_reduce_kernel", "copy 268435456 B"], "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -357,6 +376,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad-step", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the Neo-Hookean and batched legs")
+    ap.add_argument("--profile-kernels", action="store_true",
+                    help="only launch each assembly kernel a few times (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -366,6 +387,9 @@ def main():
         print(json.dumps({"error": "no ROCm GPU visible"}))
         sys.exit(2)
 
+    if args.profile_kernels:
+        profile_kernels()
+        return
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     mesh, kappa, A, setup_s, host = build_workload(dtype)
     E = A.elements["tetra"].shape[0]

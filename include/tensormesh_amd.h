@@ -86,7 +86,12 @@ typedef struct tm_form_desc {
   const void* coeff;         /* DIFFUSION_ELEM: [n_elem]; DIFFUSION_NODE: [n_points]; else NULL */
   double young;              /* LINEAR_ELASTICITY: E  */
   double poisson;            /* LINEAR_ELASTICITY: nu */
+  int32_t flags;             /* TM_FLAG_* */
+  int32_t reserved;
 } tm_form_desc;
+
+/* tm_form_desc.flags */
+#define TM_FLAG_MFMA 1       /* float32 block tangents: contract on v_mfma_f32_32x32x2f32 */
 
 /* Sparsity pattern plus the slot -> nnz permutation in segment form. */
 typedef struct tm_pattern_view {
@@ -180,6 +185,20 @@ int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t cap, int32
 int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
                     const int32_t* row_ptr, const int32_t* col_idx, const int32_t* inc_ptr, const int32_t* inc,
                     uint32_t* colpos, uint32_t* entry_mask);
+
+/* Device pattern builder (stream-ordered; all pointers device).  Same arrays as
+ * tm_incidence_build + tm_pattern_build + tm_colpos_build.
+ *   tm_pattern_device_count: inc_ptr [N+1], inc [E*B], row_ptr [N+1]; work is
+ *     int32 scratch of N+1 entries; afterwards nnz = row_ptr[N] and work[N] != 0
+ *     flags a row with more than 2048 neighbour keys (then use the host builder).
+ *   tm_pattern_device_fill: col_idx [nnz], seg_ptr [nnz+1], seg_slot and
+ *     slot2nnz [E*B*B], colpos [E*B] and entry_mask [nnz] (B <= 4, may be NULL). */
+int tm_pattern_device_count(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                            int32_t* inc_ptr, int32_t* inc, int32_t* row_ptr, int32_t* work, tm_stream_t stream);
+int tm_pattern_device_fill(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
+                           const int32_t* inc_ptr, const int32_t* inc, const int32_t* row_ptr, int32_t* col_idx,
+                           int32_t* seg_ptr, int32_t* seg_slot, int32_t* slot2nnz, uint32_t* colpos,
+                           uint32_t* entry_mask, tm_stream_t stream);
 
 /* Node incidence (host), the scatter map of NodeAssembler's ReduceProjector
  * (tensormesh/assemble/node_assembler.py:665-670) in gather form. */

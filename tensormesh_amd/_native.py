@@ -37,7 +37,10 @@ class FormDesc(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("n_quad", ctypes.c_int32),
                 ("tab_host", ctypes.c_void_p), ("tab_dev", ctypes.c_void_p),
                 ("coeff", ctypes.c_void_p), ("young", ctypes.c_double),
-                ("poisson", ctypes.c_double)]
+                ("poisson", ctypes.c_double), ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+TM_FLAG_MFMA = 1
 
 
 class PatternView(ctypes.Structure):
@@ -106,6 +109,10 @@ SIGNATURES = {
                                               ctypes.c_void_p]),
     "tm_reduce_p1_rows": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RowsView), ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_pattern_device_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
+                                + [ctypes.c_void_p] * 5),
+    "tm_pattern_device_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
+                               + [ctypes.c_void_p] * 10),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 

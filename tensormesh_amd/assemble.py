@@ -143,6 +143,8 @@ class ElementAssembler(_AssemblerBase):
     #: "two_stage" = element kernel + segmented reduction (tm_assemble_bilinear),
     #: "two_stage_rows" = element kernel + row-centric reduction (tm_reduce_p1_rows).
     p1_path: str = "rows"
+    #: Build the pattern on the GPU (tm_pattern_device_*) when the mesh is on the device.
+    device_pattern: bool = True
     #: True runs every integrand through the restated reference vmap/autograd
     #: path (torch ops on the same device) -- the "plain" side of a
     #: measurement, never chosen implicitly.
@@ -172,7 +174,11 @@ class ElementAssembler(_AssemblerBase):
         if len(elements) != 1:
             raise NotImplementedError("tensormesh_amd assembles one element type per mesh")
         conn = next(iter(elements.values()))
-        pattern = Pattern.build(conn, mesh.points.shape[0], device=mesh.points.device)
+        pattern = None
+        if mesh.points.device.type == "cuda" and cls.device_pattern:
+            pattern = Pattern.build_device(conn.to(mesh.points.device), mesh.points.shape[0])
+        if pattern is None:
+            pattern = Pattern.build(conn, mesh.points.shape[0], device=mesh.points.device)
         return cls(elements, mesh.points, pattern, quadrature_order, *args, **kwargs)
 
     @classmethod
@@ -318,7 +324,8 @@ class ElementAssembler(_AssemblerBase):
         tab, tab_dev = self._tables(dev)
         mesh = self._mesh_view(points)
         form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
-                            None, float(young), float(poisson))
+                            None, float(young), float(poisson),
+                            nat.TM_FLAG_MFMA if (getattr(self, "use_mfma", False) and dtype == torch.float32) else 0)
         pv = pat.view(dtype, D * D)
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
         tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None

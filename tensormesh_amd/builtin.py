@@ -61,7 +61,12 @@ class LinearElasticityElementAssembler(ElementAssembler):
 
 
 class NeoHookeanModel(ElementAssembler):
-    """psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2 (builtin.py:219-333)."""
+    """psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2 (builtin.py:219-333).
+
+    ``use_mfma = True`` contracts the float32 tangent on the matrix cores
+    (v_mfma_f32_32x32x2f32) instead of the closed-form VALU loop."""
+
+    use_mfma: bool = False
 
     def __post_init__(self, E=1.0, nu=0.3):
         self.E = E

@@ -22,6 +22,8 @@
 // which is the product of the Voigt matrices of functional/elasticity.py:132-284.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tm_internal.h"
 
 namespace {
@@ -189,7 +191,9 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 // with the shape gradients on both sides.  At u = 0 it equals the linear
 // elasticity block.  Tangent layout as the block forms: ((e*B+i)*B+j)*D*D + a*D + b;
 // residual slots [e][i][a] summed into [N][D] by the node incidence reduction.
-template <typename T, int D, int B>
+typedef float tm_floatx16 __attribute__((ext_vector_type(16)));
+
+template <typename T, int D, int B, bool MFMA>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem,
@@ -347,6 +351,53 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       rslot[e * (B * D) + t] = (T)acc;
     }
   }
+  if constexpr (MFMA) {
+    // K_e = sum_q L_q^T R_q as one 32 x 32 tile (rows (i, a), columns (j, b),
+    // B*D <= 32) with inner index (q, c, A), two per v_mfma_f32_32x32x2f32:
+    //   L^T[(i,a)][(q,c,A)] = delta_ac g_q,i[A]
+    //   R[(q,c,A)][(j,b)]   = jxw_q (mu delta_cb g_q,j[A] + lam Finv[A][c] h_j[b] - c2 Finv[A][b] h_j[c])
+    // Lane l supplies A[m = l%32][k = l/32] and B[k = l/32][n = l%32]; the
+    // accumulator register r of lane l holds row (r/4)*8 + (l/32)*4 + r%4,
+    // column l%32.
+    if (tslot) {
+      constexpr int DD = D * D;
+      constexpr int M = B * D;
+      static_assert(M <= 32, "tile holds at most 32 rows");
+      const int m = lane % 32, kh = lane / 32;
+      const int i = m / D, a = m % D;
+      const int j = m / D, b = m % D;  // n == m for the B operand column
+      tm_floatx16 acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+      const int steps = (Q * DD) / 2;
+      for (int st = 0; st < steps; ++st) {
+        const int kk = 2 * st + kh;
+        const int q = kk / DD, c = (kk % DD) / D, A = kk % D;
+        float av = 0.0f, bv = 0.0f;
+        if (m < M) {
+          const double* g = &s_g[wv][q * B * D];
+          const double* h = &s_h[wv][q * B * D];
+          const double* Fi = s_Fi[wv][q];
+          if (a == c) av = (float)g[i * D + A];
+          double v = lam * Fi[A * D + c] * h[j * D + b] - s_c2[wv][q] * Fi[A * D + b] * h[j * D + c];
+          if (c == b) v += mu * g[j * D + A];
+          bv = (float)(v * s_jxw[wv][q]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+      }
+      T* out = tslot + e * (int64_t)(B * B * DD);
+      const int n = m;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = (r / 4) * 8 + kh * 4 + (r % 4);
+        if (row < M && n < M) {
+          const int ii = row / D, aa = row % D, jj = n / D, bb = n % D;
+          out[(ii * B + jj) * DD + aa * D + bb] = (T)acc[r];
+        }
+      }
+    }
+    return;
+  }
   if (tslot) {
     // one lane per node pair (i, j): the whole D x D block from one read of
     // g_i, g_j, h_i, h_j per quadrature point
@@ -398,8 +449,21 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   if (grid == 0) return TM_OK;
   const double mu = f->young / (2.0 * (1.0 + f->poisson));
   const double lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
-  hyper_wave_kernel<T, D, B><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
-      m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+  const bool mfma = (f->flags & TM_FLAG_MFMA) != 0;
+  if constexpr (std::is_same<T, float>::value && B * D <= 32) {
+    if (mfma) {
+      if ((Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
+      hyper_wave_kernel<T, D, B, true><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
+          m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+    } else {
+      hyper_wave_kernel<T, D, B, false><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
+          m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+    }
+  } else {
+    if (mfma) return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
+    hyper_wave_kernel<T, D, B, false><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
+        m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+  }
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     tmh::set_error("neo-hookean kernel launch: %s", hipGetErrorString(err));

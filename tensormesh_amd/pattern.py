@@ -125,6 +125,44 @@ class Pattern:
                        t(row), t(col), h,
                        None if inc_ptr is None else inc_ptr.to(dev), None if inc is None else inc.to(dev))
 
+    @staticmethod
+    def build_device(conn: torch.Tensor, n_points: int) -> Optional["Pattern"]:
+        """The same pattern built on the GPU (tm_pattern_device_count / _fill).
+        Returns None when a row has too many neighbour keys for the device
+        builder (the host builder then serves)."""
+        L = nat.lib()
+        dev = conn.device
+        c = conn.to(torch.int32).contiguous()
+        E, B = c.shape
+        st = nat.stream_of(dev)
+        i32 = lambda n: torch.empty(n, dtype=torch.int32, device=dev)
+        inc_ptr, inc, row_ptr, work = i32(n_points + 1), i32(E * B), i32(n_points + 1), i32(n_points + 1)
+        nat.check(L.tm_pattern_device_count(c.data_ptr(), E, B, n_points, inc_ptr.data_ptr(), inc.data_ptr(),
+                                            row_ptr.data_ptr(), work.data_ptr(), st), "tm_pattern_device_count")
+        nnz = int(row_ptr[n_points].item())
+        if int(work[n_points].item()) != 0:
+            return None
+        S = E * B * B
+        col_idx, seg_ptr, seg_slot, slot2nnz = i32(nnz), i32(nnz + 1), i32(S), i32(S)
+        colpos = i32(E * B) if B <= 4 else None
+        mask = i32(nnz) if B <= 4 else None
+        p = lambda t: None if t is None else t.data_ptr()
+        nat.check(L.tm_pattern_device_fill(c.data_ptr(), E, B, n_points, inc_ptr.data_ptr(), inc.data_ptr(),
+                                           row_ptr.data_ptr(), col_idx.data_ptr(), seg_ptr.data_ptr(),
+                                           seg_slot.data_ptr(), slot2nnz.data_ptr(), p(colpos), p(mask), st),
+                  "tm_pattern_device_fill")
+        row = torch.repeat_interleave(torch.arange(n_points, device=dev), (row_ptr[1:] - row_ptr[:-1]).long())
+        col = col_idx.long()
+        h = layout_hash(row.cpu().numpy(), col.cpu().numpy())
+        pat = Pattern(n_points, E, B, nnz, row_ptr, col_idx, seg_ptr, seg_slot, slot2nnz, row, col, h, inc_ptr, inc)
+        if colpos is not None:
+            too_long = bool(((row_ptr[1:] - row_ptr[:-1]) > 255).any().item())
+            if not too_long:
+                pat._colpos, pat._entry_mask = colpos, mask
+            else:
+                pat._colpos = False
+        return pat
+
     def to(self, device) -> "Pattern":
         dev = torch.device(device)
         mv = lambda x: None if x is None else x.to(dev)

@@ -224,3 +224,25 @@ def test_neohookean_residual_tangent_against_oracle_autograd(etype):
     ur = u.clone().requires_grad_(True)
     tm.NeoHookeanModel.from_mesh(m, quadrature_order=2, E=1.0, nu=0.3).energy(ur).backward()
     assert torch.allclose(R.cpu(), ur.grad.reshape(-1), rtol=1e-10, atol=1e-13)
+
+
+@pytest.mark.parametrize("mfma", [False, True])
+def test_neohookean_fp32_tangent_against_fp64(mfma):
+    say(f"case neo-hookean tetra10 fp32 mfma={mfma}")
+    m = unit_cube_tet(3, 0.0, 3, order=2)
+    P = m.points
+    u = 0.05 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, 2], 0.5 * P[:, 0] ** 2], 1)
+    ref = tm.NeoHookeanModel.from_mesh(m.to(DEV), quadrature_order=2, E=1.0, nu=0.3)
+    R64, K64 = ref.residual_and_tangent(u.to(DEV))
+    m32 = tm.Mesh(P.float(), {"tetra10": m.elements()}).to(DEV)
+    model = tm.NeoHookeanModel.from_mesh(m32, quadrature_order=2, E=1.0, nu=0.3)
+    model.use_mfma = mfma
+    R32, K32 = model.residual_and_tangent(u.float().to(DEV))
+    assert K32.dtype == torch.float32 and torch.equal(K32.row, K64.row)
+    d = K32.values.double() - K64.values
+    normwise = (d.norm() / K64.values.norm()).item()
+    maxrel = (d.abs().max() / K64.values.abs().max()).item()
+    say(f"  fp32 tangent vs fp64: normwise {normwise:.2e}, max|d|/max|K| {maxrel:.2e}")
+    assert normwise < 1e-6 and maxrel < 1e-6
+    dr = (R32.double() - R64).norm() / R64.norm()
+    assert dr.item() < 1e-5

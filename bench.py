@@ -89,7 +89,7 @@ def stage_times(A, mesh, kappa, reps=20):
         if r >= 3:
             ev[4 * (r - 3) + 2].record(stream)
         nat.check(L.tm_reduce_p1_rows(nat.dtype_code(pts.dtype), nat.CELL["tetra"], ctypes.byref(rv),
-                                      slot.data_ptr(), vals.data_ptr(), s), "S_rows")
+                                      slot.data_ptr(), 0, vals.data_ptr(), s), "S_rows")
         if r >= 3:
             ev[4 * (r - 3) + 3].record(stream)
     torch.cuda.synchronize()
@@ -126,6 +126,45 @@ def fused_time(A, mesh, kappa, reps=20):
         ev[r + 1].record(stream)
     torch.cuda.synchronize()
     return float(np.mean([ev[r].elapsed_time(ev[r + 1]) for r in range(reps)]) * 1e-3), rv.n_blocks
+
+
+def incidence_stage_times(A, mesh, kappa, reps=20):
+    """Device time of the incidence-ordered path's two kernels (HIP events on
+    the current stream): stage G writing rows in incidence order, stage S
+    reducing contiguous rows."""
+    import ctypes
+    from tensormesh_amd import _native as nat
+    L = nat.lib()
+    dev = mesh.points.device
+    pts = mesh.points.contiguous()
+    conn = A._conn32(dev)
+    E, B = conn.shape
+    pat = A._pattern_on(dev)
+    tab, tab_dev = A._tables(dev)
+    slot = torch.empty(E * B * B, dtype=pts.dtype, device=dev)
+    vals = torch.empty(pat.nnz, dtype=pts.dtype, device=dev)
+    mv = A._mesh_view(pts)
+    rv = pat.rows_view(conn)
+    inv = pat.inc_inv()
+    fd = nat.FormDesc(nat.FORM["diffusion_elem"], 1, tab.ctypes.data, tab_dev.data_ptr(), kappa.data_ptr(), 0.0, 0.0)
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3 * reps)]
+    for r in range(reps + 3):
+        if r >= 3:
+            ev[3 * (r - 3)].record(stream)
+        nat.check(L.tm_element_matrices_inc(ctypes.byref(mv), ctypes.byref(fd), inv.data_ptr(), slot.data_ptr(), s),
+                  "G_inc")
+        if r >= 3:
+            ev[3 * (r - 3) + 1].record(stream)
+        nat.check(L.tm_reduce_p1_rows(nat.dtype_code(pts.dtype), nat.CELL["tetra"], ctypes.byref(rv),
+                                      slot.data_ptr(), 1, vals.data_ptr(), s), "S_inc")
+        if r >= 3:
+            ev[3 * (r - 3) + 2].record(stream)
+    torch.cuda.synchronize()
+    g = float(np.mean([ev[3 * r].elapsed_time(ev[3 * r + 1]) for r in range(reps)]) * 1e-3)
+    sr = float(np.mean([ev[3 * r + 1].elapsed_time(ev[3 * r + 2]) for r in range(reps)]) * 1e-3)
+    return g, sr
 
 
 def fused_bytes(E, N, nnz, B=4, D=3, s=8, c=8):
@@ -255,7 +294,8 @@ def hyperelastic_bench(n=32, reps=5):
     t0 = time.perf_counter()
     model = tm.NeoHookeanModel.from_mesh(m, quadrature_order=2, E=1.0, nu=0.3)
     out["pattern_setup_s"] = time.perf_counter() - t0
-    for dt in (torch.float64, torch.float32):
+    for dt, mfma in ((torch.float64, False), (torch.float32, False), (torch.float32, True)):
+        model.use_mfma = mfma
         pts = m.points.to(dt)
         u = torch.zeros_like(pts)
         u[:, 0] = 0.1 * pts[:, 0]
@@ -275,7 +315,7 @@ def hyperelastic_bench(n=32, reps=5):
         byts = (4 * 10 * E + s * 3 * 2 * m.n_points + s * E * 930) + (4 * 100 * E + 4 * (nnzb + 1) + s * E * 900
                                                                        + s * nnzb * 9) \
             + (4 * 10 * E + 4 * (m.n_points + 1) + s * E * 30 + s * 3 * m.n_points)
-        out["f64" if dt == torch.float64 else "f32"] = {
+        out["f64" if dt == torch.float64 else ("f32_mfma" if mfma else "f32")] = {
             "ms_per_newton_assembly": ms, "elements_per_s": E / (ms * 1e-3), "block_entries": nnzb,
             "scalar_nnz": nnzb * 9, "algorithmic_bytes": byts, "GBps": byts / (ms * 1e-3) / 1e9,
             "hbm_frac": byts / (ms * 1e-3) / PEAK_HBM}
@@ -411,11 +451,13 @@ def main():
 
     dt_two = timed("two_stage")
     dt_two_rows = timed("two_stage_rows")
+    dt_inc = timed("incidence")
     dt_fused = timed("rows")
-    dt = min(dt_two, dt_two_rows, dt_fused)
+    dt = min(dt_two, dt_two_rows, dt_fused, dt_inc)
 
     g_s, s_s, s2_s = stage_times(A, mesh, kappa)
     f_s, f_blocks = fused_time(A, mesh, kappa)
+    inc_times = incidence_stage_times(A, mesh, kappa)
     s = 8 if dtype == torch.float64 else 4
     gb, sb = algorithmic_bytes(E, N, nnz, s=s, c=s)
     s2b = rows_reduce_bytes(E, N, nnz, s=s)
@@ -427,6 +469,10 @@ def main():
                "two_stage_rows": [("p1_element_kernel", gb, g_s), ("p1_rows_reduce_kernel", s2b, s2_s)]}
     best = min((("fused_rows", dt_fused), ("two_stage_segments", dt_two), ("two_stage_rows", dt_two_rows)),
                key=lambda x: x[1])[0]
+    if dt_inc < min(dt_fused, dt_two, dt_two_rows):
+        best = "incidence"
+        kernels["incidence"] = [("p1_element_inc_kernel", gb + 4 * 4 * E, inc_times[0]),
+                                ("p1_rows_reduce_kernel<inc_order>", s2b - 4 * 4 * E, inc_times[1])]
     name, kb, ks = max(kernels[best], key=lambda k: k[2])
     ach = kb / ks
     roof = {"bound": "hbm", "kernel": name, "path": best,
@@ -452,10 +498,13 @@ def main():
                    "elements": E, "nodes": N, "nnz": nnz, "parallelism": "single GPU"},
         "nnz_per_s": nnz / dt,
         "paths_ms_per_step": {"fused_rows": dt_fused * 1e3, "two_stage_segments": dt_two * 1e3,
-                              "two_stage_rows": dt_two_rows * 1e3},
+                              "two_stage_rows": dt_two_rows * 1e3, "incidence": dt_inc * 1e3},
         "two_stage": {"ms_per_step": dt_two * 1e3, "stage_G_us": g_s * 1e6, "stage_S_us": s_s * 1e6,
                       "stage_S_rows_us": s2_s * 1e6, "bytes_S_rows": s2b, "stage_S_rows_frac": s2b / s2_s / PEAK_HBM,
                       "fused_kernel_us": f_s * 1e6, "bytes_fused": fb, "fused_frac": fb / f_s / PEAK_HBM,
+                      "incidence_G_us": inc_times[0] * 1e6, "incidence_S_us": inc_times[1] * 1e6,
+                      "incidence_G_frac": (gb + 16 * E) / inc_times[0] / PEAK_HBM,
+                      "incidence_S_frac": (s2b - 16 * E) / inc_times[1] / PEAK_HBM,
                       "bytes_G": gb, "bytes_S": sb,
                       "stage_G_frac": gb / g_s / PEAK_HBM, "stage_S_frac": sb / s_s / PEAK_HBM,
                       "pipeline_GBps": pipeline_bw / 1e9, "pipeline_frac": pipeline_bw / PEAK_HBM,

@@ -151,6 +151,8 @@ typedef struct tm_pcg_view {
   const int32_t* blk_row1;   /* [n_blocks] */
   const int32_t* sys_blk;    /* [n_sys + 1] */
   double rtol;               /* stop system s once |r_s| <= rtol |b_s| */
+  void* s;                   /* [n_rows] A p; with counters set, two launches per iteration */
+  int32_t* counters;         /* [n_sys] workgroup tickets (zeroed by tm_pcg_init) */
 } tm_pcg_view;
 
 int tm_abi_version(void);
@@ -258,9 +260,21 @@ int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, c
 /* Row-centric stage S for P1 (TRI3 / TET4): the entries of tm_assemble_bilinear
  * from a stage-G slot buffer, each CSR row reading its incidences' local rows
  * (B contiguous values each) and routing them by colpos; same per-entry order
- * (ascending element).  Replaces ReduceProjector.__call__ (projector.py:95-123). */
-int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot, void* values,
-                      tm_stream_t stream);
+ * (ascending element).  inc_order = 0: slot in element order (tm_element_matrices);
+ * 1: slot in incidence order (tm_element_matrices_inc).  Replaces ReduceProjector.__call__ (projector.py:95-123). */
+int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot, int32_t inc_order,
+                      void* values, tm_stream_t stream);
+
+/* Two-stage P1 assembly in incidence order (TRI3 / TET4 scalar forms): stage G
+ * writes the local row K_e[i][.] to slot_ws[inc_inv[e*B+i] * B] (inc_inv: the
+ * inverse of the incidence permutation, int32 [n_elem*B]); stage S reads each
+ * CSR row's contributions contiguously and routes them by colpos / entry masks.
+ * Same values and per-entry order as tm_assemble_bilinear.  slot_ws: [n_elem*B*B]. */
+int tm_assemble_p1_incidence(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_rows_view* rows,
+                             const int32_t* inc_inv, void* slot_ws, void* values, tm_stream_t stream);
+/* Its stage G alone. */
+int tm_element_matrices_inc(const tm_mesh_view* mesh, const tm_form_desc* form, const int32_t* inc_inv,
+                            void* slot_ws, tm_stream_t stream);
 
 /* Stage G alone (element matrices into slot_ws).  Same citation as above. */
 int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
@@ -309,7 +323,10 @@ int tm_condense(int32_t dtype, int64_t n_inner, const void* values, const int32_
  *   tm_pcg_init:    r = b - A x, z = dinv r, p = z, per-system scalars
  *   tm_pcg_iterate: n_iter PCG steps for every still-active system; no host
  *                   synchronisation, so the caller may capture it in a graph
- *                   and poll view->scalars between chunks.
+ *                   and poll view->scalars between chunks.  With view->s and
+ *                   view->counters set, each step is two launches
+ *                   (Chronopoulos-Gear recurrences, per-system sums closed by
+ *                   the last workgroup in workgroup order); otherwise five.
  * ------------------------------------------------------------------- */
 int tm_spmv(int32_t dtype, int64_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
             const void* x, void* y, tm_stream_t stream);

@@ -62,7 +62,7 @@ class PcgView(ctypes.Structure):
     _fields_ += [(n, ctypes.c_void_p) for n in ("row_ptr", "col_idx", "values", "dinv", "r", "z", "p", "q",
                                                  "partials", "scalars", "blk_sys", "blk_row0", "blk_row1",
                                                  "sys_blk")]
-    _fields_ += [("rtol", ctypes.c_double)]
+    _fields_ += [("rtol", ctypes.c_double), ("s", ctypes.c_void_p), ("counters", ctypes.c_void_p)]
 
 
 class IncidenceView(ctypes.Structure):
@@ -108,11 +108,16 @@ SIGNATURES = {
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]),
     "tm_reduce_p1_rows": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RowsView), ctypes.c_void_p,
-                                         ctypes.c_void_p, ctypes.c_void_p]),
+                                         ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_element_matrices_inc": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
     "tm_pattern_device_count": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
                                 + [ctypes.c_void_p] * 5),
     "tm_pattern_device_fill": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
                                + [ctypes.c_void_p] * 10),
+    "tm_assemble_p1_incidence": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                                ctypes.POINTER(RowsView), ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p]),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 

@@ -141,7 +141,9 @@ class ElementAssembler(_AssemblerBase):
 
     #: P1 scalar forms: "rows" = fused row-centric kernel (tm_assemble_p1_rows),
     #: "two_stage" = element kernel + segmented reduction (tm_assemble_bilinear),
-    #: "two_stage_rows" = element kernel + row-centric reduction (tm_reduce_p1_rows).
+    #: "two_stage_rows" = element kernel + row-centric reduction (tm_reduce_p1_rows),
+    #: "incidence" = element kernel writing in incidence order + contiguous row
+    #: reduction (tm_assemble_p1_incidence).
     p1_path: str = "rows"
     #: Build the pattern on the GPU (tm_pattern_device_*) when the mesh is on the device.
     device_pattern: bool = True
@@ -281,20 +283,26 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
                             tab.ctypes.data, tab_dev.data_ptr(),
                             None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
-        if H == 1 and self.p1_path in ("rows", "two_stage_rows") and self.etype in ("triangle", "tetra"):
+        if H == 1 and self.p1_path in ("rows", "two_stage_rows", "incidence") and self.etype in ("triangle", "tetra"):
             rv = pat.rows_view(conn)
             if rv is not None:
                 values = torch.empty(pat.nnz, dtype=dtype, device=dev)
                 if self.p1_path == "rows":
                     nat.check(L.tm_assemble_p1_rows(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
                                                     values.data_ptr(), nat.stream_of(dev)), "tm_assemble_p1_rows")
+                elif self.p1_path == "incidence":
+                    slot = torch.empty(E * B * B, dtype=dtype, device=dev)
+                    nat.check(L.tm_assemble_p1_incidence(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
+                                                         pat.inc_inv().data_ptr(), slot.data_ptr(),
+                                                         values.data_ptr(), nat.stream_of(dev)),
+                              "tm_assemble_p1_incidence")
                 else:
                     slot = torch.empty(E * B * B, dtype=dtype, device=dev)
                     st = nat.stream_of(dev)
                     nat.check(L.tm_element_matrices(ctypes.byref(mesh), ctypes.byref(form), slot.data_ptr(), st),
                               "tm_element_matrices")
                     nat.check(L.tm_reduce_p1_rows(nat.dtype_code(dtype), nat.CELL[self.etype], ctypes.byref(rv),
-                                                  slot.data_ptr(), values.data_ptr(), st), "tm_reduce_p1_rows")
+                                                  slot.data_ptr(), 0, values.data_ptr(), st), "tm_reduce_p1_rows")
                 return values
         slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
         values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)

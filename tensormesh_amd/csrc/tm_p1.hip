@@ -176,6 +176,45 @@ __global__ void __launch_bounds__(kBlock) p1_element_kernel(const int32_t* __res
   }
 }
 
+// Stage G writing in incidence order: the local row K_e[i][0..B) of element e
+// goes to slot_inc[inc_inv[e*B + i] * B], the position of incidence (e, i) in
+// the node-sorted incidence list, so that the row-centric stage S reads every
+// CSR row's contributions as one contiguous run.
+template <typename T, int D, int FORM>
+__global__ void __launch_bounds__(kBlock) p1_element_inc_kernel(const int32_t* __restrict__ conn,
+                                                                const T* __restrict__ pts,
+                                                                const T* __restrict__ coeff, P1Consts c,
+                                                                const int32_t* __restrict__ inc_inv,
+                                                                int64_t n_elem, T* __restrict__ slot_inc) {
+  constexpr int B = D + 1;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n_elem) return;
+  double K[B * B];
+  p1_local<T, D, FORM>(conn, pts, coeff, c, e, K);
+  int32_t dst[B];
+  if constexpr (B == 4) {
+    const int4 q = reinterpret_cast<const int4*>(inc_inv)[e];
+    dst[0] = q.x; dst[1] = q.y; dst[2] = q.z; dst[3] = q.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < B; ++i) dst[i] = inc_inv[e * B + i];
+  }
+#pragma unroll
+  for (int i = 0; i < B; ++i) {
+    T* out = slot_inc + (int64_t)dst[i] * B;
+    if constexpr (B == 4 && sizeof(T) == 8) {
+      reinterpret_cast<double2*>(out)[0] = make_double2(K[i * B + 0], K[i * B + 1]);
+      reinterpret_cast<double2*>(out)[1] = make_double2(K[i * B + 2], K[i * B + 3]);
+    } else if constexpr (B == 4 && sizeof(T) == 4) {
+      reinterpret_cast<float4*>(out)[0] =
+          make_float4((float)K[i * B + 0], (float)K[i * B + 1], (float)K[i * B + 2], (float)K[i * B + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < B; ++j) out[j] = (T)K[i * B + j];
+    }
+  }
+}
+
 int fill_consts(const tm_form_desc* f, int B, int D, P1Consts& c) {
   c = P1Consts{};
   if (!f->tab_host || f->n_quad <= 0) return tmh::fail(TM_ERR_ARG, "form tables missing");
@@ -279,5 +318,66 @@ extern "C" int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc
   const int hh = form->form == TM_FORM_LINEAR_ELASTICITY ? tmh::cell_dim(mesh->cell) * tmh::cell_dim(mesh->cell) : 1;
   return tm_segmented_reduce(mesh->dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, pat->n_blocks, pat->blk_ptr,
                              slot_ws, values, stream);
+  TM_TRY_END
+}
+
+// Defined in tm_rows.hip: row-centric stage S over an incidence-ordered slot buffer.
+int tm_rows_reduce_launch(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot, bool inc_order,
+                          void* values, hipStream_t stream);
+
+extern "C" int tm_element_matrices_inc(const tm_mesh_view* mesh, const tm_form_desc* form, const int32_t* inc_inv,
+                                       void* slot_ws, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  int st = check_mesh(mesh);
+  if (st) return st;
+  if (!form || !inc_inv) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  if (mesh->cell != TM_TRI3 && mesh->cell != TM_TET4) return tmh::fail(TM_ERR_UNSUPPORTED, "incidence path is P1 only");
+  if (mesh->n_elem > 0 && !slot_ws) return tmh::fail(TM_ERR_ARG, "NULL buffer");
+  const int B = tmh::cell_basis(mesh->cell), D = tmh::cell_dim(mesh->cell);
+  P1Consts c;
+  st = fill_consts(form, B, D, c);
+  if (st) return st;
+  const int64_t grid = (mesh->n_elem + kBlock - 1) / kBlock;
+  if (grid > 0) {
+#define TM_INC(T, DD, FORM)                                                                                    \
+  p1_element_inc_kernel<T, DD, FORM><<<grid, kBlock, 0, stream>>>(mesh->conn, (const T*)mesh->points,         \
+                                                                   (const T*)form->coeff, c, inc_inv, mesh->n_elem, \
+                                                                   (T*)slot_ws)
+#define TM_INC_FORMS(T, DD)                                                                  \
+  switch (form->form) {                                                                      \
+    case TM_FORM_LAPLACE: TM_INC(T, DD, TM_FORM_LAPLACE); break;                             \
+    case TM_FORM_DIFFUSION_ELEM: TM_INC(T, DD, TM_FORM_DIFFUSION_ELEM); break;               \
+    case TM_FORM_DIFFUSION_NODE: TM_INC(T, DD, TM_FORM_DIFFUSION_NODE); break;               \
+    case TM_FORM_MASS: TM_INC(T, DD, TM_FORM_MASS); break;                                   \
+    default: return tmh::fail(TM_ERR_UNSUPPORTED, "form not available for the incidence path"); \
+  }
+    if ((form->form == TM_FORM_DIFFUSION_ELEM || form->form == TM_FORM_DIFFUSION_NODE) && !form->coeff)
+      return tmh::fail(TM_ERR_ARG, "diffusion form needs a coefficient");
+    if (mesh->dtype == TM_F64) {
+      if (D == 3) { TM_INC_FORMS(double, 3) } else { TM_INC_FORMS(double, 2) }
+    } else {
+      if (D == 3) { TM_INC_FORMS(float, 3) } else { TM_INC_FORMS(float, 2) }
+    }
+#undef TM_INC_FORMS
+#undef TM_INC
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+      tmh::set_error("incidence element kernel launch: %s", hipGetErrorString(err));
+      return TM_ERR_HIP;
+    }
+  }
+  return TM_OK;
+  TM_TRY_END
+}
+
+extern "C" int tm_assemble_p1_incidence(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_rows_view* rows,
+                                        const int32_t* inc_inv, void* slot_ws, void* values, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  if (!rows || !values) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  int st = tm_element_matrices_inc(mesh, form, inc_inv, slot_ws, stream_);
+  if (st) return st;
+  return tm_rows_reduce_launch(mesh->dtype, mesh->cell, rows, slot_ws, true, values,
+                               reinterpret_cast<hipStream_t>(stream_));
   TM_TRY_END
 }

@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(kBlock)
 // as p1_rows_kernel, but phase 1 reads the element rows K_e[i][0..B) from the
 // slot buffer written by stage G (one contiguous B-value row per incidence)
 // instead of recomputing them.
-template <typename T, int D>
+template <typename T, int D, bool INC_ORDER>
 __global__ void __launch_bounds__(kBlock)
     p1_rows_reduce_kernel(const T* __restrict__ slot, const int32_t* __restrict__ row_ptr,
                           const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
@@ -245,8 +245,8 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t n_inc = t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
   if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
     for (int32_t t = threadIdx.x; t < n_inc; t += kBlock) {
-      const int32_t ei = inc[t0 + t];        // e*B + i: the slot row starts at (e*B + i)*B
-      const T* src = slot + (int64_t)ei * B;
+      // element-ordered slots: row (e, i) starts at (e*B + i)*B; incidence-ordered: at (t0 + t)*B
+      const T* src = INC_ORDER ? slot + (int64_t)(t0 + t) * B : slot + (int64_t)inc[t0 + t] * B;
       if constexpr (B == 4 && sizeof(T) == 8) {
         const double2 a = reinterpret_cast<const double2*>(src)[0];
         const double2 b = reinterpret_cast<const double2*>(src)[1];
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(kBlock)
       const uint32_t cp = colpos[t];
 #pragma unroll
       for (int j = 0; j < B; ++j)
-        if (((cp >> (8 * j)) & 0xffu) == p) acc += slot[(int64_t)inc[t] * B + j];
+        if (((cp >> (8 * j)) & 0xffu) == p) acc += slot[(INC_ORDER ? (int64_t)t : (int64_t)inc[t]) * B + j];
     }
     values[k] = acc;
   }
@@ -383,24 +383,24 @@ extern "C" int tm_assemble_p1_rows(const tm_mesh_view* mesh, const tm_form_desc*
   TM_TRY_END
 }
 
-extern "C" int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot,
-                                 void* values, tm_stream_t stream_) {
-  TM_TRY_BEGIN
-  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+int tm_rows_reduce_launch(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot, bool inc_order,
+                          void* values, hipStream_t stream) {
   if (!rows) return tmh::fail(TM_ERR_ARG, "NULL rows view");
   if (cell != TM_TRI3 && cell != TM_TET4) return tmh::fail(TM_ERR_UNSUPPORTED, "row reduction is P1 only");
   const int64_t nb = rows->n_blocks;
   if (nb == 0) return TM_OK;
   if (!slot || !values || !rows->row_ptr || !rows->inc_ptr || !rows->inc || !rows->colpos || !rows->blk_ptr)
     return tmh::fail(TM_ERR_ARG, "NULL buffer");
-#define TM_RR(T, D)                                                                                          \
-  p1_rows_reduce_kernel<T, D><<<nb, kBlock, 0, stream>>>((const T*)slot, rows->row_ptr, rows->inc_ptr, rows->inc, \
-                                                         rows->colpos, rows->entry_mask, rows->blk_ptr, nb,      \
-                                                         (T*)values)
+#define TM_RR(T, D, IO)                                                                                          \
+  p1_rows_reduce_kernel<T, D, IO><<<nb, kBlock, 0, stream>>>((const T*)slot, rows->row_ptr, rows->inc_ptr, rows->inc, \
+                                                             rows->colpos, rows->entry_mask, rows->blk_ptr, nb,      \
+                                                             (T*)values)
   if (dtype == TM_F64) {
-    if (cell == TM_TET4) TM_RR(double, 3); else TM_RR(double, 2);
+    if (cell == TM_TET4) { if (inc_order) TM_RR(double, 3, true); else TM_RR(double, 3, false); }
+    else { if (inc_order) TM_RR(double, 2, true); else TM_RR(double, 2, false); }
   } else if (dtype == TM_F32) {
-    if (cell == TM_TET4) TM_RR(float, 3); else TM_RR(float, 2);
+    if (cell == TM_TET4) { if (inc_order) TM_RR(float, 3, true); else TM_RR(float, 3, false); }
+    else { if (inc_order) TM_RR(float, 2, true); else TM_RR(float, 2, false); }
   } else {
     return tmh::fail(TM_ERR_ARG, "unknown dtype");
   }
@@ -411,5 +411,12 @@ extern "C" int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view
     return TM_ERR_HIP;
   }
   return TM_OK;
+}
+
+extern "C" int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view* rows, const void* slot,
+                                 int32_t inc_order, void* values, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  return tm_rows_reduce_launch(dtype, cell, rows, slot, inc_order != 0, values,
+                               reinterpret_cast<hipStream_t>(stream_));
   TM_TRY_END
 }

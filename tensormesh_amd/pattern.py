@@ -75,6 +75,15 @@ class Pattern:
                     self._colpos = False
         return self._colpos if self._colpos is not False else None
 
+    def inc_inv(self) -> torch.Tensor:
+        """Position of incidence e*B + i in the node-sorted incidence list (int32)."""
+        if getattr(self, "_inc_inv", None) is None:
+            n = self.inc.numel()
+            inv = torch.empty(n, dtype=torch.int32, device=self.inc.device)
+            inv[self.inc.long()] = torch.arange(n, dtype=torch.int32, device=self.inc.device)
+            self._inc_inv = inv
+        return self._inc_inv
+
     def rows_view(self, conn: torch.Tensor) -> Optional[nat.RowsView]:
         cp = self.colpos(conn)
         if cp is None:

@@ -63,6 +63,9 @@ def _col32(A) -> torch.Tensor:
 class BatchedPCG:
     """Jacobi-PCG over a block-diagonal stack of CSR systems on the device.
 
+    ``two_launch`` (default True) uses the two-launch Chronopoulos-Gear
+    iteration; False the five-launch textbook PCG.
+
     Parameters
     ----------
     row_ptr, col_idx : int32 device tensors of the stacked CSR (global ids)
@@ -70,6 +73,8 @@ class BatchedPCG:
     sys_rows : row offsets of the systems, length n_sys + 1
     rtol : stop a system once |r| <= rtol |b|
     """
+
+    two_launch: bool = True
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor,
                  sys_rows: Sequence[int], rtol: float = 1e-10):
@@ -93,7 +98,8 @@ class BatchedPCG:
         self.blk_sys, self.blk_row0, self.blk_row1, self.sys_blk = t(blk_sys), t(blk0), t(blk1), t(sys_blk)
         self.n_blocks = len(blk_sys)
         e = lambda: torch.empty(self.n, dtype=self.dtype, device=self.dev)
-        self.dinv, self.r, self.z, self.p, self.q, self.x = e(), e(), e(), e(), e(), e()
+        self.dinv, self.r, self.z, self.p, self.q, self.x, self.s = e(), e(), e(), e(), e(), e(), e()
+        self.counters = torch.zeros(self.n_sys, dtype=torch.int32, device=self.dev)
         self.partials = torch.zeros(self.n_blocks * 3, dtype=torch.float64, device=self.dev)
         self.scalars = torch.zeros(self.n_sys * 8, dtype=torch.float64, device=self.dev)
         self.rtol = rtol
@@ -117,6 +123,8 @@ class BatchedPCG:
                      "blk_sys", "blk_row0", "blk_row1", "sys_blk"):
             setattr(v, name, getattr(self, name).data_ptr())
         v.rtol = self.rtol
+        if self.two_launch:
+            v.s, v.counters = self.s.data_ptr(), self.counters.data_ptr()
         return v
 
     def residual(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:

@@ -81,7 +81,8 @@ def test_cfg1_laplace_load_condense_against_golden():
 
 
 @pytest.mark.parametrize("q,path", [(1, "rows"), (2, "rows"), (1, "two_stage"), (2, "two_stage"),
-                                    (1, "two_stage_rows"), (2, "two_stage_rows")])
+                                    (1, "two_stage_rows"), (2, "two_stage_rows"), (1, "incidence"),
+                                    (2, "incidence")])
 def test_tet_diffusion_and_mass_against_golden(q, path):
     say(f"case tet3 diffusion/mass qorder {q} {path}")
     d = np.load(GOLD / "tet3_diffusion.npz")
@@ -148,7 +149,7 @@ def _cfg2(dtype=torch.float64):
     return p, c, kap
 
 
-@pytest.mark.parametrize("path", ["rows", "two_stage", "two_stage_rows"])
+@pytest.mark.parametrize("path", ["rows", "two_stage", "two_stage_rows", "incidence"])
 def test_cfg2_full_size_fp64_against_oracle(path):
     say(f"case cfg2 64^3 Kuhn tets fp64 vs oracle, {path} path")
     p, c, kap = _cfg2()
@@ -170,7 +171,7 @@ def test_cfg2_full_size_fp64_against_oracle(path):
     assert rs.abs().max().item() < 1e-12 * K.values.abs().max().item()
 
 
-@pytest.mark.parametrize("path", ["rows", "two_stage", "two_stage_rows"])
+@pytest.mark.parametrize("path", ["rows", "two_stage", "two_stage_rows", "incidence"])
 def test_cfg2_fp32_error_bound_against_fp64_oracle(path):
     say(f"case cfg2 fp32 vs fp64 oracle, {path} path")
     p, c, kap = _cfg2()

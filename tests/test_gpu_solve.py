@@ -21,8 +21,14 @@ def _system(n, seed):
     return cond(K, b)
 
 
+@pytest.fixture(params=[True, False], ids=["two_launch", "five_launch"])
+def variant(request, monkeypatch):
+    monkeypatch.setattr(BatchedPCG, "two_launch", request.param)
+    return request.param
+
+
 @pytest.mark.parametrize("graph", [True, False])
-def test_single_system_matches_dense(graph):
+def test_single_system_matches_dense(graph, variant):
     Ki, fi = _system(12, 3)
     s = BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-12)
     x, it, rel = s.solve(fi, graph=graph, chunk=16)
@@ -32,7 +38,7 @@ def test_single_system_matches_dense(graph):
     assert 0 < it[0].item() < 2000
 
 
-def test_batched_uneven_systems_match_dense():
+def test_batched_uneven_systems_match_dense(variant):
     systems = [_system(n, n) for n in (5, 9, 14, 7)]
     rows, cols, vals, bs, offs = [], [], [], [], [0]
     rp = [0]
@@ -64,3 +70,29 @@ def test_sparse_matrix_solve_and_spmv_on_device():
     dense = Ki.to_dense()
     x = torch.randn(Ki.shape[0], dtype=torch.float64, device=DEV)
     assert torch.allclose(Ki @ x, dense @ x, rtol=1e-12, atol=1e-13)
+
+
+def test_variants_agree_on_a_many_workgroup_system():
+    """A 14k-row system spans 55 workgroups: the two-launch variant closes
+    its sums in the last workgroup; both variants are the same Krylov method,
+    so iteration counts agree within a few and solutions to the tolerance."""
+    Ki, fi = _system(120, 5)
+    out = {}
+    for two in (True, False):
+        BatchedPCG.two_launch = two
+        try:
+            s = BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-10)
+            out[two] = s.solve(fi, chunk=32)
+        finally:
+            BatchedPCG.two_launch = True
+    (x2, it2, r2), (x5, it5, r5) = out[True], out[False]
+    assert r2[0].item() <= 1e-10 and r5[0].item() <= 1e-10
+    assert abs(it2[0].item() - it5[0].item()) <= max(3, it5[0].item() // 50)
+    res = torch.linalg.norm(Ki @ x2 - fi) / torch.linalg.norm(fi)
+    assert res.item() <= 2e-10
+    assert torch.allclose(x2, x5, rtol=1e-7, atol=1e-12 * x5.abs().max().item())
+    # deterministic: a second solve reproduces the iterate bit for bit
+    BatchedPCG.two_launch = True
+    s = BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-10)
+    x2b, _, _ = s.solve(fi, chunk=32)
+    assert torch.equal(x2, x2b)

@@ -170,14 +170,14 @@ def incidence_stage_times(A, mesh, kappa, reps=20):
 def fused_bytes(E, N, nnz, B=4, D=3, s=8, c=8):
     """Unique bytes of the fused row-centric kernel, int32 indices:
     conn 4BE + coords cDN + kappa sE + incidence 4BE + colpos 4BE + row_ptr and
-    inc_ptr 8(N+1) + values s*nnz."""
-    return 4 * B * E + c * D * N + s * E + 4 * B * E + 4 * B * E + 8 * (N + 1) + s * nnz
+    inc_ptr 8(N+1) + entry masks 4 nnz + values s*nnz."""
+    return 4 * B * E + c * D * N + s * E + 4 * B * E + 4 * B * E + 8 * (N + 1) + 4 * nnz + s * nnz
 
 
 def rows_reduce_bytes(E, N, nnz, B=4, s=8):
     """Unique bytes of the row-centric reduction: incidence 4BE + colpos 4BE +
-    slot values sB^2E + row_ptr / inc_ptr 8(N+1) + values s*nnz."""
-    return 4 * B * E + 4 * B * E + s * B * B * E + 8 * (N + 1) + s * nnz
+    slot values sB^2E + row_ptr / inc_ptr 8(N+1) + entry masks 4 nnz + values s*nnz."""
+    return 4 * B * E + 4 * B * E + s * B * B * E + 8 * (N + 1) + 4 * nnz + s * nnz
 
 
 def cpu_baseline(p, c, kap, budget_s=25.0):
@@ -381,7 +381,7 @@ def batched_bench(n_meshes=256, seed=0):
 
 
 def load_traffic():
-    path = os.path.join(ROOT, "profiles", "traffic_cfg2_fp64.json")
+    path = os.path.join(ROOT, "profiles", "p1_tet_diffusion_1572864_f64_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -394,17 +394,26 @@ def profile_kernels(reps=5):
     written) that calibrates the HBM counters, for rocprofv3 runs."""
     mesh, kappa, A, _, _ = build_workload(torch.float64)
     ed = {"kappa": kappa}
-    for path in ("rows", "two_stage", "two_stage_rows"):
+    for path in ("rows", "two_stage", "two_stage_rows", "incidence"):
         A.p1_path = path
         for _ in range(reps):
-            A(element_data=ed)
+            K = A(element_data=ed)
+    # a few PCG iterations on the condensed system (two-launch kernels)
+    import tensormesh_amd as tm
+    from tensormesh_amd.solve import BatchedPCG
+    pts = mesh.points
+    bnd = ((pts <= 1e-12) | (pts >= 1.0 - 1e-12)).any(dim=1)
+    Ki, fi = tm.Condenser(bnd)(K, torch.ones(mesh.n_points, dtype=torch.float64, device="cuda"))
+    BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-30).solve(fi, max_iter=64,
+                                                                                          graph=False)
     src = torch.empty(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"profiled": ["p1_rows_kernel", "p1_element_kernel", "reduce_stream_kernel",
-                                   "p1_rows_reduce_kernel", "copy 268435456 B"], "reps": reps}))
+                                   "p1_rows_reduce_kernel", "p1_element_inc_kernel", "cg2_vec_kernel",
+                                   "cg2_spmv_kernel", "copy 268435456 B"], "reps": reps}))
 
 
 def main():

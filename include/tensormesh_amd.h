@@ -124,6 +124,8 @@ typedef struct tm_rows_view {
   const uint32_t* entry_mask; /* [nnz] bit t: the row's t-th incidence contributes (rows with <= 32 incidences), or NULL */
   int64_t n_blocks;          /* reduction plan over inc_ptr (rows) */
   const int32_t* blk_ptr;    /* [n_blocks + 1] */
+  const int32_t* blk_info;   /* [n_blocks + 1][4] {blk_ptr[b], inc_ptr[blk_ptr[b]], row_ptr[blk_ptr[b]], 0},
+                                or NULL (then read through blk_ptr) */
 } tm_rows_view;
 
 /* Block-diagonal stack of SPD CSR systems for the batched Jacobi-PCG.
@@ -151,8 +153,9 @@ typedef struct tm_pcg_view {
   const int32_t* blk_row1;   /* [n_blocks] */
   const int32_t* sys_blk;    /* [n_sys + 1] */
   double rtol;               /* stop system s once |r_s| <= rtol |b_s| */
-  void* s;                   /* [n_rows] A p; with counters set, two launches per iteration */
-  int32_t* counters;         /* [n_sys] workgroup tickets (zeroed by tm_pcg_init) */
+  void* s;                   /* [n_rows] A p (two-launch variant only) */
+  int32_t two_launch;        /* 1: two launches per iteration (partials [5 n_blocks], scalars [16 n_sys]) */
+  int32_t parity;            /* two-launch: number of iterations run since tm_pcg_init, mod 2 */
 } tm_pcg_view;
 
 int tm_abi_version(void);
@@ -307,9 +310,11 @@ int tm_assemble_linear(const tm_mesh_view* mesh, const tm_form_desc* form, const
  *   inner_rp[n_inner+1], inner_idx[nnz_ii]   entries of K_ii per inner row
  *   ou_rp[n_inner+1], ou_idx[nnz_io], ou_col[nnz_io]  entries / outer dof of K_io
  *   inner_dof[n_inner]                        global dof of each inner dof
- * Outputs kii[nnz_ii] = values[inner_idx], f_i[n_inner] = rhs[inner_dof] - K_io u_o.
+ * n_kept = inner_rp[n_inner] (the number of K_ii entries, passed so that the
+ * call never reads the device).
+ * Outputs kii[n_kept] = values[inner_idx], f_i[n_inner] = rhs[inner_dof] - K_io u_o.
  * ------------------------------------------------------------------- */
-int tm_condense(int32_t dtype, int64_t n_inner, const void* values, const int32_t* inner_rp,
+int tm_condense(int32_t dtype, int64_t n_inner, int64_t n_kept, const void* values, const int32_t* inner_rp,
                 const int32_t* inner_idx, const int32_t* ou_rp, const int32_t* ou_idx, const int32_t* ou_col,
                 const int32_t* inner_dof, const void* rhs, const void* u_o, void* kii, void* f_i,
                 tm_stream_t stream);
@@ -318,15 +323,20 @@ int tm_condense(int32_t dtype, int64_t n_inner, const void* values, const int32_
  * Sparse solve.  The reference inherits solve / @ from torch_sla (not
  * vendored); its in-tree fallback is the Jacobi-CG of
  * tensormesh/sparse/solve/torch_solve.py:54-128.
- *   tm_spmv:        y = A x (CSR, one thread per row)
+ *   tm_spmv:        y = A x (CSR; every row summed in ascending entry order)
  *   tm_jacobi:      dinv[i] = 1 / A_ii (1 where A_ii == 0)
  *   tm_pcg_init:    r = b - A x, z = dinv r, p = z, per-system scalars
  *   tm_pcg_iterate: n_iter PCG steps for every still-active system; no host
  *                   synchronisation, so the caller may capture it in a graph
- *                   and poll view->scalars between chunks.  With view->s and
- *                   view->counters set, each step is two launches
- *                   (Chronopoulos-Gear recurrences, per-system sums closed by
- *                   the last workgroup in workgroup order); otherwise five.
+ *                   and poll view->scalars between chunks.  With two_launch
+ *                   set, each step is two launches (Chronopoulos-Gear
+ *                   recurrences; every workgroup re-sums its system's
+ *                   partials in workgroup order, so a step costs
+ *                   O(workgroups per system) extra loads per workgroup:
+ *                   meant for systems of up to ~2048 workgroups); otherwise
+ *                   five.  Host-visible scalars per system s (stride 8 for
+ *                   five launches, 16 for two): [4] |r|^2, [5] |b|^2,
+ *                   [6] active, [7] iterations.
  * ------------------------------------------------------------------- */
 int tm_spmv(int32_t dtype, int64_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, const void* values,
             const void* x, void* y, tm_stream_t stream);

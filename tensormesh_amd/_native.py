@@ -53,7 +53,7 @@ class RowsView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p),
                 ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("entry_mask", ctypes.c_void_p),
                 ("n_blocks", ctypes.c_int64),
-                ("blk_ptr", ctypes.c_void_p)]
+                ("blk_ptr", ctypes.c_void_p), ("blk_info", ctypes.c_void_p)]
 
 
 class PcgView(ctypes.Structure):
@@ -62,7 +62,8 @@ class PcgView(ctypes.Structure):
     _fields_ += [(n, ctypes.c_void_p) for n in ("row_ptr", "col_idx", "values", "dinv", "r", "z", "p", "q",
                                                  "partials", "scalars", "blk_sys", "blk_row0", "blk_row1",
                                                  "sys_blk")]
-    _fields_ += [("rtol", ctypes.c_double), ("s", ctypes.c_void_p), ("counters", ctypes.c_void_p)]
+    _fields_ += [("rtol", ctypes.c_double), ("s", ctypes.c_void_p), ("two_launch", ctypes.c_int32),
+                 ("parity", ctypes.c_int32)]
 
 
 class IncidenceView(ctypes.Structure):
@@ -118,7 +119,7 @@ SIGNATURES = {
     "tm_assemble_p1_incidence": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                                 ctypes.POINTER(RowsView), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p, ctypes.c_void_p]),
-    "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 12),
+    "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 
 _LIB: Optional[ctypes.CDLL] = None
@@ -181,8 +182,9 @@ def reduce_cap(dtype: torch.dtype, hh: int) -> int:
     return max(1, min(4096, LDS_BYTES // (size * hh)))
 
 
-def reduce_plan(seg_ptr_host, cap: int):
-    """blk_ptr (int32 numpy) of tm_reduce_plan for a host seg_ptr array."""
+def reduce_plan(seg_ptr_host, cap: int, max_segments: Optional[int] = None):
+    """blk_ptr (int32 numpy) of tm_reduce_plan for a host seg_ptr array;
+    with max_segments, blocks holding more segments are split."""
     import numpy as np
     seg_ptr_host = np.ascontiguousarray(seg_ptr_host, dtype=np.int32)
     n = seg_ptr_host.shape[0] - 1
@@ -190,7 +192,11 @@ def reduce_plan(seg_ptr_host, cap: int):
     nb = ctypes.c_int64(0)
     check(lib().tm_reduce_plan(seg_ptr_host.ctypes.data, n, cap, out.ctypes.data, ctypes.byref(nb)),
           "tm_reduce_plan")
-    return out[: nb.value + 1].copy()
+    bp = out[: nb.value + 1].copy()
+    if max_segments is not None and bp.size > 1 and int(np.diff(bp).max()) > max_segments:
+        pieces = [np.arange(a, b, max_segments, dtype=np.int64) for a, b in zip(bp[:-1], bp[1:])]
+        bp = np.concatenate(pieces + [np.array([n], np.int64)]).astype(np.int32)
+    return bp
 
 
 def dtype_code(dtype: torch.dtype) -> int:

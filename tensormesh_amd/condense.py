@@ -7,7 +7,8 @@ inner/outer edge masks in the same (order-preserving) entry order, and the
 same layout-hash check.  The layout is computed once per pattern; the
 per-call work -- gathering the K_ii values and forming
 f_i - K_io u_o -- runs in one HIP kernel when the matrix lives on the device
-(``tm_condense``), one thread per inner row.
+(``tm_condense``: a coalesced gather of the kept entries and one thread per
+inner row for the right-hand side).
 """
 
 from __future__ import annotations
@@ -141,7 +142,7 @@ class Condenser(nn.Module):
         kii = torch.empty(inner_idx.numel(), dtype=vals.dtype, device=dev)
         f_i = torch.empty(self.n_inner_dof, dtype=vals.dtype, device=dev)
         dval = self.dirichlet_value.to(vals.dtype).contiguous()
-        nat.check(L.tm_condense(nat.dtype_code(vals.dtype), self.n_inner_dof, vals.data_ptr(),
+        nat.check(L.tm_condense(nat.dtype_code(vals.dtype), self.n_inner_dof, inner_idx.numel(), vals.data_ptr(),
                                 inner_rp.data_ptr(), inner_idx.data_ptr(), ou_rp.data_ptr(), ou_idx.data_ptr(),
                                 ou_col.data_ptr(), inner_dofs.data_ptr(), rhs.contiguous().data_ptr(),
                                 dval.data_ptr(), kii.data_ptr(), f_i.data_ptr(), nat.stream_of(dev)),

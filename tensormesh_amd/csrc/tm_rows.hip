@@ -24,7 +24,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kCapInc = 512;   // incidences staged per workgroup (matches the host plan cap)
-constexpr int kMaxRows = 256;  // rows staged per workgroup
+constexpr int kMaxRows = 128;  // rows staged per workgroup (the host plan splits longer runs)
 constexpr int kMaxEnt = 2048;  // CSR entries staged per workgroup
 
 struct RowConsts {
@@ -39,34 +39,117 @@ __device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
   return x < r ? x * (q + 1) + y : r * (q + 1) + (x - r) * q + y;
 }
 
-// Row i of the local matrix of element e, in double.
-template <typename T, int D, int FORM>
-__device__ __forceinline__ void local_row(const int32_t* __restrict__ conn, const T* __restrict__ pts,
-                                          const T* __restrict__ coeff, const RowConsts& c, int64_t e, int i,
-                                          double (&Kr)[D + 1]) {
-  constexpr int B = D + 1;
-  int v[B];
-  if constexpr (D == 3) {
-    const int4 q = reinterpret_cast<const int4*>(conn)[e];
-    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+// Coordinates of node v as D doubles: for 3-D double points one 16-byte and
+// one 8-byte load (8-byte aligned), for float points one 12-byte load.
+template <typename T, int D>
+__device__ __forceinline__ void load_node(const T* __restrict__ pts, int32_t v, double (&x)[D]) {
+  const T* p = pts + (int64_t)v * D;
+  if constexpr (D == 3 && sizeof(T) == 8) {
+    typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+    const d2u a = *reinterpret_cast<const d2u*>(p);
+    x[0] = a.x;
+    x[1] = a.y;
+    x[2] = p[2];
+  } else if constexpr (D == 3) {
+    typedef float f3u __attribute__((ext_vector_type(3), aligned(4)));
+    const f3u a = *reinterpret_cast<const f3u*>(p);
+    x[0] = a.x;
+    x[1] = a.y;
+    x[2] = a.z;
   } else {
 #pragma unroll
-    for (int b = 0; b < B; ++b) v[b] = conn[e * B + b];
+    for (int j = 0; j < D; ++j) x[j] = (double)p[j];
   }
+}
+
+// Element data a local row needs, gathered in three dependent steps
+// (incidence -> connectivity -> coordinates / coefficients) so that a thread
+// can issue the loads of several incidences before it computes any of them.
+template <int D>
+struct ElemData {
+  static constexpr int B = D + 1;
+  int v[B];
   double x[B][D];
+  double k[B];  // element coefficient in k[0], or the nodal coefficients
+};
+
+template <int D>
+__device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int64_t e, ElemData<D>& d) {
+  constexpr int B = D + 1;
+  if constexpr (D == 3) {
+    const int4 q = reinterpret_cast<const int4*>(conn)[e];
+    d.v[0] = q.x; d.v[1] = q.y; d.v[2] = q.z; d.v[3] = q.w;
+  } else {
 #pragma unroll
-  for (int b = 0; b < B; ++b)
+    for (int b = 0; b < B; ++b) d.v[b] = conn[e * B + b];
+  }
+}
+
+template <typename T, int D, int FORM>
+__device__ __forceinline__ void load_geometry(const T* __restrict__ pts, const T* __restrict__ coeff, int64_t e,
+                                              ElemData<D>& d) {
+  constexpr int B = D + 1;
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[b][d] = (double)pts[(int64_t)v[b] * D + d];
+  for (int b = 0; b < B; ++b) load_node<T, D>(pts, d.v[b], d.x[b]);
+  if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
+    d.k[0] = (double)coeff[e];
+  } else if constexpr (FORM == TM_FORM_DIFFUSION_NODE) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) d.k[b] = (double)coeff[d.v[b]];
+  }
+}
+
+// Rotated element view for the Laplace / element-coefficient forms: for the
+// incidence (e, i), vertex p of the view is vertex (i + p) mod B of e, so the
+// row's own node is always view vertex 0 and nothing is indexed by the
+// runtime i in registers (selects on the connectivity word only).  The row
+// node's coordinates come from LDS (staged per row); the other B - 1
+// vertices are gathered.
+template <int D>
+struct RotData {
+  static constexpr int B = D + 1;
+  int32_t c[B];     // rotated connectivity
+  double x[B][D];   // rotated coordinates (x[0] filled from LDS)
+  double k;         // element coefficient
+};
+
+template <typename T, int D, int FORM>
+__device__ __forceinline__ void load_rot(const int32_t* __restrict__ conn, const T* __restrict__ coeff, int64_t e,
+                                         int i, RotData<D>& d) {
+  if constexpr (D == 3) {
+    const int4 q = reinterpret_cast<const int4*>(conn)[e];
+    d.c[0] = i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w;
+    d.c[1] = i == 0 ? q.y : i == 1 ? q.z : i == 2 ? q.w : q.x;
+    d.c[2] = i == 0 ? q.z : i == 1 ? q.w : i == 2 ? q.x : q.y;
+    d.c[3] = i == 0 ? q.w : i == 1 ? q.x : i == 2 ? q.y : q.z;
+  } else {
+    const int32_t a0 = conn[e * 3 + 0], a1 = conn[e * 3 + 1], a2 = conn[e * 3 + 2];
+    d.c[0] = i == 0 ? a0 : i == 1 ? a1 : a2;
+    d.c[1] = i == 0 ? a1 : i == 1 ? a2 : a0;
+    d.c[2] = i == 0 ? a2 : i == 1 ? a0 : a1;
+  }
+  if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) d.k = (double)coeff[e];
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void gather_rot(const T* __restrict__ pts, RotData<D>& d) {
+#pragma unroll
+  for (int p = 1; p < D + 1; ++p) load_node<T, D>(pts, d.c[p], d.x[p]);
+}
+
+// Row 0 of the rotated element's local matrix: K[0][p] = scale g_0 . g_p
+template <int D, int FORM>
+__device__ __forceinline__ void rot_row(const RotData<D>& d, const RowConsts& c, double (&Kr)[D + 1]) {
+  constexpr int B = D + 1;
   double n[D][D];
   double det;
   if constexpr (D == 3) {
     double a[3], bb[3], cc[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      a[d] = x[1][d] - x[0][d];
-      bb[d] = x[2][d] - x[0][d];
-      cc[d] = x[3][d] - x[0][d];
+    for (int j = 0; j < 3; ++j) {
+      a[j] = d.x[1][j] - d.x[0][j];
+      bb[j] = d.x[2][j] - d.x[0][j];
+      cc[j] = d.x[3][j] - d.x[0][j];
     }
     n[0][0] = bb[1] * cc[2] - bb[2] * cc[1];
     n[0][1] = bb[2] * cc[0] - bb[0] * cc[2];
@@ -79,8 +162,63 @@ __device__ __forceinline__ void local_row(const int32_t* __restrict__ conn, cons
     n[2][2] = a[0] * bb[1] - a[1] * bb[0];
     det = a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
   } else {
-    const double a0 = x[1][0] - x[0][0], a1 = x[1][1] - x[0][1];
-    const double b0 = x[2][0] - x[0][0], b1 = x[2][1] - x[0][1];
+    const double a0 = d.x[1][0] - d.x[0][0], a1 = d.x[1][1] - d.x[0][1];
+    const double b0 = d.x[2][0] - d.x[0][0], b1 = d.x[2][1] - d.x[0][1];
+    n[0][0] = b1;  n[0][1] = -b0;
+    n[1][0] = -a1; n[1][1] = a0;
+    det = a0 * b1 - a1 * b0;
+  }
+  double coef = c.W;
+  if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) coef *= d.k;
+  const double scale = coef / fabs(det);
+  double g0[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s += n[k][j];
+    g0[j] = -s;
+  }
+  double s0 = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) s0 += g0[j] * g0[j];
+  Kr[0] = scale * s0;
+#pragma unroll
+  for (int p = 1; p < B; ++p) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += g0[j] * n[p - 1][j];
+    Kr[p] = scale * s;
+  }
+}
+
+// Row i of the local matrix of the element held in d, in double.
+template <int D, int FORM>
+__device__ __forceinline__ void row_from(const ElemData<D>& d, const RowConsts& c, int i, double (&Kr)[D + 1]) {
+  constexpr int B = D + 1;
+  double n[D][D];
+  double det;
+  if constexpr (D == 3) {
+    double a[3], bb[3], cc[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      a[j] = d.x[1][j] - d.x[0][j];
+      bb[j] = d.x[2][j] - d.x[0][j];
+      cc[j] = d.x[3][j] - d.x[0][j];
+    }
+    n[0][0] = bb[1] * cc[2] - bb[2] * cc[1];
+    n[0][1] = bb[2] * cc[0] - bb[0] * cc[2];
+    n[0][2] = bb[0] * cc[1] - bb[1] * cc[0];
+    n[1][0] = cc[1] * a[2] - cc[2] * a[1];
+    n[1][1] = cc[2] * a[0] - cc[0] * a[2];
+    n[1][2] = cc[0] * a[1] - cc[1] * a[0];
+    n[2][0] = a[1] * bb[2] - a[2] * bb[1];
+    n[2][1] = a[2] * bb[0] - a[0] * bb[2];
+    n[2][2] = a[0] * bb[1] - a[1] * bb[0];
+    det = a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
+  } else {
+    const double a0 = d.x[1][0] - d.x[0][0], a1 = d.x[1][1] - d.x[0][1];
+    const double b0 = d.x[2][0] - d.x[0][0], b1 = d.x[2][1] - d.x[0][1];
     n[0][0] = b1;  n[0][1] = -b0;
     n[1][0] = -a1; n[1][1] = a0;
     det = a0 * b1 - a1 * b0;
@@ -93,40 +231,178 @@ __device__ __forceinline__ void local_row(const int32_t* __restrict__ conn, cons
   } else {
     double g[B][D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
+    for (int j = 0; j < D; ++j) {
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        g[k + 1][d] = n[k][d];
-        s += n[k][d];
+        g[k + 1][j] = n[k][j];
+        s += n[k][j];
       }
-      g[0][d] = -s;
+      g[0][j] = -s;
     }
     double coef = c.W;
     if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
-      coef *= (double)coeff[e];
+      coef *= d.k[0];
     } else if constexpr (FORM == TM_FORM_DIFFUSION_NODE) {
       double kq = 0.0;
 #pragma unroll
-      for (int b = 0; b < B; ++b) kq += (double)coeff[v[b]] * c.wphi[b];
+      for (int b = 0; b < B; ++b) kq += d.k[b] * c.wphi[b];
       coef = kq;
     }
     const double scale = coef / adet;
     double gi[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      double s = g[0][d];
+    for (int j = 0; j < D; ++j) {
+      double s = g[0][j];
 #pragma unroll
-      for (int b = 1; b < B; ++b) s = (i == b) ? g[b][d] : s;
-      gi[d] = s;
+      for (int b = 1; b < B; ++b) s = (i == b) ? g[b][j] : s;
+      gi[j] = s;
     }
 #pragma unroll
     for (int j = 0; j < B; ++j) {
       double s = 0.0;
 #pragma unroll
-      for (int d = 0; d < D; ++d) s += gi[d] * g[j][d];
+      for (int k = 0; k < D; ++k) s += gi[k] * g[j][k];
       Kr[j] = scale * s;
     }
+  }
+}
+
+// Row i of the local matrix of element e (all three load steps in sequence).
+template <typename T, int D, int FORM>
+__device__ __forceinline__ void local_row(const int32_t* __restrict__ conn, const T* __restrict__ pts,
+                                          const T* __restrict__ coeff, const RowConsts& c, int64_t e, int i,
+                                          double (&Kr)[D + 1]) {
+  ElemData<D> d;
+  load_conn<D>(conn, e, d);
+  load_geometry<T, D, FORM>(pts, coeff, e, d);
+  row_from<D, FORM>(d, c, i, Kr);
+}
+
+// Block bounds: {first row, first incidence, first CSR entry} of blocks blk and
+// blk + 1, from the precomputed table (two independent loads) or, without it,
+// from blk_ptr -> inc_ptr / row_ptr (two dependent loads).
+struct BlockBounds {
+  int32_t r0, r1, t0, t1, k0, k1;
+};
+
+__device__ __forceinline__ BlockBounds block_bounds(const int32_t* __restrict__ info,
+                                                    const int32_t* __restrict__ blk_ptr,
+                                                    const int32_t* __restrict__ inc_ptr,
+                                                    const int32_t* __restrict__ row_ptr, int64_t blk) {
+  BlockBounds b;
+  if (info) {
+    const int4 a = reinterpret_cast<const int4*>(info)[blk];
+    const int4 z = reinterpret_cast<const int4*>(info)[blk + 1];
+    b.r0 = a.x; b.t0 = a.y; b.k0 = a.z;
+    b.r1 = z.x; b.t1 = z.y; b.k1 = z.z;
+  } else {
+    b.r0 = blk_ptr[blk];
+    b.r1 = blk_ptr[blk + 1];
+    b.t0 = inc_ptr[b.r0];
+    b.t1 = inc_ptr[b.r1];
+    b.k0 = row_ptr[b.r0];
+    b.k1 = row_ptr[b.r1];
+  }
+  return b;
+}
+
+// Phase 2 of the row kernels: one thread per CSR entry of the staged rows,
+// summing the staged local-row values of the row's incidences in ascending
+// element order.  m0 / m1 are the entry masks of the thread's first two
+// entries, loaded ahead of phase 1.
+// ROT: incidence t's values are staged rotated (s_val[t*B + p] belongs to
+// vertex (i + p) mod B, s_pos bytes likewise), so value 0 of every incidence
+// of row r is its diagonal contribution: the diagonal entry is a contiguous
+// sum over the row's incidences, and only off-diagonal entries walk masks.
+template <typename T, int B, bool ROT>
+__device__ __forceinline__ void sum_entries(const T* s_val, const uint32_t* s_pos, const int32_t* s_rp,
+                                            const int32_t* s_ip, const uint16_t* s_erow,
+                                            const uint32_t* __restrict__ entry_mask, int32_t k0, int32_t n_ent,
+                                            uint32_t m0, uint32_t m1, T* __restrict__ values) {
+  int it = 0;
+  for (int32_t kl = threadIdx.x; kl < n_ent; kl += kBlock, ++it) {
+    const int32_t lr = s_erow[kl];
+    const uint32_t p = (uint32_t)(kl - s_rp[lr]);
+    const int32_t ta = s_ip[lr], tz = s_ip[lr + 1];
+    T acc = T(0);
+    if (ROT && (s_pos[ta] & 0xffu) == p) {
+      // diagonal: value 0 of every incidence of the row, in order
+      for (int32_t t = ta; t < tz; t += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = t + u < tz ? s_val[(t + u) * B] : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (t + u < tz) acc += v[u];
+      }
+    } else if (entry_mask && tz - ta <= 32) {
+      // only the incidences that contribute, lowest bit (= lowest element) first
+      uint32_t m = it == 0 ? m0 : it == 1 ? m1 : entry_mask[k0 + kl];
+      // four contributions per round: their LDS reads are issued together,
+      // the adds stay in ascending element order
+      while (m) {
+        int32_t q[4];
+        bool h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          h[u] = m != 0u;
+          q[u] = ta + (h[u] ? __builtin_ctz(m) : 0);
+          m &= m - 1u;
+        }
+        uint32_t cp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cp[u] = s_pos[q[u]];
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          int jj = 0;
+#pragma unroll
+          for (int j = 1; j < B; ++j) jj = (((cp[u] >> (8 * j)) & 0xffu) == p) ? j : jj;
+          v[u] = s_val[q[u] * B + jj];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (h[u]) acc += v[u];
+      }
+    } else {
+      for (int32_t t = ta; t < tz; ++t) {
+        const uint32_t cp = s_pos[t];
+#pragma unroll
+        for (int j = 0; j < B; ++j)
+          if (((cp >> (8 * j)) & 0xffu) == p) acc += s_val[t * B + j];
+      }
+    }
+    values[k0 + kl] = acc;
+  }
+}
+
+// colpos word of incidence (e, i) rotated like the staged values: byte p of
+// the result is byte (i + p) mod B of cp.
+template <int B>
+__device__ __forceinline__ uint32_t rotate_pos(uint32_t cp, int i) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int p = 0; p < B; ++p) {
+    const int src = i + p < B ? i + p : i + p - B;
+    r |= ((cp >> (8 * src)) & 0xffu) << (8 * p);
+  }
+  return r;
+}
+
+// Entry -> local row map of the staged rows (s_rp must be visible).
+__device__ __forceinline__ void build_entry_rows(const int32_t* s_rp, int32_t n_rows, uint16_t* s_erow) {
+  for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock)
+    for (int32_t k = s_rp[lr]; k < s_rp[lr + 1]; ++k) s_erow[k] = (uint16_t)lr;
+}
+
+// Row / incidence starts of the staged rows (relative to the block) into LDS.
+__device__ __forceinline__ void stage_row_maps(const int32_t* __restrict__ row_ptr,
+                                               const int32_t* __restrict__ inc_ptr, const BlockBounds& bb,
+                                               int32_t* s_rp, int32_t* s_ip) {
+  for (int32_t lr = threadIdx.x; lr <= bb.r1 - bb.r0; lr += kBlock) {
+    s_rp[lr] = row_ptr[bb.r0 + lr] - bb.k0;
+    s_ip[lr] = inc_ptr[bb.r0 + lr] - bb.t0;
   }
 }
 
@@ -135,65 +411,95 @@ __global__ void __launch_bounds__(kBlock)
     p1_rows_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
                    RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
                    const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
-                   const uint32_t* __restrict__ entry_mask, const int32_t* __restrict__ blk_ptr, int64_t n_blocks,
-                   T* __restrict__ values) {
+                   const uint32_t* __restrict__ entry_mask, const int32_t* __restrict__ blk_ptr,
+                   const int32_t* __restrict__ blk_info, int64_t n_blocks, T* __restrict__ values) {
   constexpr int B = D + 1;
+  static_assert(kCapInc == 2 * kBlock, "phase 1 stages two incidences per thread");
   __shared__ T s_val[kCapInc * B];
   __shared__ uint32_t s_pos[kCapInc];
   __shared__ int32_t s_rp[kMaxRows + 1];    // row starts, relative to k0
   __shared__ int32_t s_ip[kMaxRows + 1];    // incidence starts, relative to t0
   __shared__ uint16_t s_erow[kMaxEnt];      // entry -> local row
+  __shared__ double s_xr[kMaxRows * D];     // coordinates of the staged rows' nodes
   const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
-  const int32_t r0 = blk_ptr[blk], r1 = blk_ptr[blk + 1];
-  const int32_t t0 = inc_ptr[r0], t1 = inc_ptr[r1];
-  const int32_t k0 = row_ptr[r0], k1 = row_ptr[r1];
-  const int32_t n_inc = t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
+  const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
+  const int32_t r0 = bb.r0, r1 = bb.r1, t0 = bb.t0, k0 = bb.k0, k1 = bb.k1;
+  const int32_t n_inc = bb.t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
   if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
-    // phase 1: local rows of every incidence, plus the row / incidence maps
-    for (int32_t t = threadIdx.x; t < n_inc; t += kBlock) {
-      const int32_t ei = inc[t0 + t];
-      double Kr[B];
-      local_row<T, D, FORM>(conn, pts, coeff, c, ei / B, ei % B, Kr);
+    // loads that depend only on the block bounds go out first
+    stage_row_maps(row_ptr, inc_ptr, bb, s_rp, s_ip);
+    constexpr bool kRot = FORM == TM_FORM_LAPLACE || FORM == TM_FORM_DIFFUSION_ELEM;
+    if (kRot && (int32_t)threadIdx.x < n_rows) {
+      double xr[D];
+      load_node<T, D>(pts, r0 + (int32_t)threadIdx.x, xr);
 #pragma unroll
-      for (int j = 0; j < B; ++j) s_val[t * B + j] = (T)Kr[j];
-      s_pos[t] = colpos[t0 + t];
+      for (int j = 0; j < D; ++j) s_xr[threadIdx.x * D + j] = xr[j];
     }
-    for (int32_t lr = threadIdx.x; lr <= n_rows; lr += kBlock) {
-      s_rp[lr] = row_ptr[r0 + lr] - k0;
-      s_ip[lr] = inc_ptr[r0 + lr] - t0;
+    const int32_t ta = threadIdx.x, tb = threadIdx.x + kBlock;
+    const bool ha = ta < n_inc, hb = tb < n_inc;
+    uint32_t m0 = 0, m1 = 0;
+    if (entry_mask) {
+      if (ta < n_ent) m0 = entry_mask[k0 + ta];
+      if (tb < n_ent) m1 = entry_mask[k0 + tb];
     }
-    __syncthreads();
-    for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock)
-      for (int32_t k = s_rp[lr]; k < s_rp[lr + 1]; ++k) s_erow[k] = (uint16_t)lr;
+    // phase 1: the (at most two) incidences of this thread, loads interleaved
+    const int32_t ea = ha ? inc[t0 + ta] : 0;
+    const int32_t eb = hb ? inc[t0 + tb] : 0;
+    const uint32_t pa = ha ? colpos[t0 + ta] : 0u;
+    const uint32_t pb = hb ? colpos[t0 + tb] : 0u;
+    const int ia = ea % B, ib = eb % B;
+    if constexpr (kRot) {
+      RotData<D> da, db;
+      if (ha) load_rot<T, D, FORM>(conn, coeff, ea / B, ia, da);
+      if (hb) load_rot<T, D, FORM>(conn, coeff, eb / B, ib, db);
+      if (ha) gather_rot<T, D>(pts, da);
+      if (hb) gather_rot<T, D>(pts, db);
+      __syncthreads();  // s_xr, s_rp, s_ip
+      build_entry_rows(s_rp, n_rows, s_erow);
+      if (ha) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) da.x[0][j] = s_xr[(da.c[0] - r0) * D + j];
+        double Kr[B];
+        rot_row<D, FORM>(da, c, Kr);
+#pragma unroll
+        for (int p = 0; p < B; ++p) s_val[ta * B + p] = (T)Kr[p];
+        s_pos[ta] = rotate_pos<B>(pa, ia);
+      }
+      if (hb) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) db.x[0][j] = s_xr[(db.c[0] - r0) * D + j];
+        double Kr[B];
+        rot_row<D, FORM>(db, c, Kr);
+#pragma unroll
+        for (int p = 0; p < B; ++p) s_val[tb * B + p] = (T)Kr[p];
+        s_pos[tb] = rotate_pos<B>(pb, ib);
+      }
+    } else {
+      ElemData<D> da, db;
+      if (ha) load_conn<D>(conn, ea / B, da);
+      if (hb) load_conn<D>(conn, eb / B, db);
+      if (ha) load_geometry<T, D, FORM>(pts, coeff, ea / B, da);
+      if (hb) load_geometry<T, D, FORM>(pts, coeff, eb / B, db);
+      __syncthreads();  // s_rp, s_ip
+      build_entry_rows(s_rp, n_rows, s_erow);
+      if (ha) {
+        double Kr[B];
+        row_from<D, FORM>(da, c, ia, Kr);
+#pragma unroll
+        for (int j = 0; j < B; ++j) s_val[ta * B + j] = (T)Kr[j];
+        s_pos[ta] = pa;
+      }
+      if (hb) {
+        double Kr[B];
+        row_from<D, FORM>(db, c, ib, Kr);
+#pragma unroll
+        for (int j = 0; j < B; ++j) s_val[tb * B + j] = (T)Kr[j];
+        s_pos[tb] = pb;
+      }
+    }
     __syncthreads();
     // phase 2: one thread per entry, incidences in ascending element order
-    for (int32_t kl = threadIdx.x; kl < n_ent; kl += kBlock) {
-      const int32_t lr = s_erow[kl];
-      const uint32_t p = (uint32_t)(kl - s_rp[lr]);
-      const int32_t ta = s_ip[lr], tz = s_ip[lr + 1];
-      T acc = T(0);
-      if (entry_mask && tz - ta <= 32) {
-        // only the incidences that contribute, lowest bit (= lowest element) first
-        uint32_t m = entry_mask[k0 + kl];
-        while (m) {
-          const int32_t t = ta + __builtin_ctz(m);
-          m &= m - 1u;
-          const uint32_t cp = s_pos[t];
-          int jj = 0;
-#pragma unroll
-          for (int j = 1; j < B; ++j) jj = (((cp >> (8 * j)) & 0xffu) == p) ? j : jj;
-          acc += s_val[t * B + jj];
-        }
-      } else {
-        for (int32_t t = ta; t < tz; ++t) {
-          const uint32_t cp = s_pos[t];
-#pragma unroll
-          for (int j = 0; j < B; ++j)
-            if (((cp >> (8 * j)) & 0xffu) == p) acc += s_val[t * B + j];
-        }
-      }
-      values[k0 + kl] = acc;
-    }
+    sum_entries<T, B, kRot>(s_val, s_pos, s_rp, s_ip, s_erow, entry_mask, k0, n_ent, m0, m1, values);
     return;
   }
   // oversized run: every entry walks global memory, recomputing local rows
@@ -231,7 +537,8 @@ __global__ void __launch_bounds__(kBlock)
     p1_rows_reduce_kernel(const T* __restrict__ slot, const int32_t* __restrict__ row_ptr,
                           const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
                           const uint32_t* __restrict__ colpos, const uint32_t* __restrict__ entry_mask,
-                          const int32_t* __restrict__ blk_ptr, int64_t n_blocks, T* __restrict__ values) {
+                          const int32_t* __restrict__ blk_ptr, const int32_t* __restrict__ blk_info,
+                          int64_t n_blocks, T* __restrict__ values) {
   constexpr int B = D + 1;
   __shared__ T s_val[kCapInc * B];
   __shared__ uint32_t s_pos[kCapInc];
@@ -239,11 +546,16 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ int32_t s_ip[kMaxRows + 1];
   __shared__ uint16_t s_erow[kMaxEnt];
   const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
-  const int32_t r0 = blk_ptr[blk], r1 = blk_ptr[blk + 1];
-  const int32_t t0 = inc_ptr[r0], t1 = inc_ptr[r1];
-  const int32_t k0 = row_ptr[r0], k1 = row_ptr[r1];
-  const int32_t n_inc = t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
+  const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
+  const int32_t r0 = bb.r0, r1 = bb.r1, t0 = bb.t0, k0 = bb.k0, k1 = bb.k1;
+  const int32_t n_inc = bb.t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
   if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
+    stage_row_maps(row_ptr, inc_ptr, bb, s_rp, s_ip);
+    uint32_t m0 = 0, m1 = 0;
+    if (entry_mask) {
+      if ((int32_t)threadIdx.x < n_ent) m0 = entry_mask[k0 + threadIdx.x];
+      if ((int32_t)threadIdx.x + kBlock < n_ent) m1 = entry_mask[k0 + threadIdx.x + kBlock];
+    }
     for (int32_t t = threadIdx.x; t < n_inc; t += kBlock) {
       // element-ordered slots: row (e, i) starts at (e*B + i)*B; incidence-ordered: at (t0 + t)*B
       const T* src = INC_ORDER ? slot + (int64_t)(t0 + t) * B : slot + (int64_t)inc[t0 + t] * B;
@@ -262,41 +574,11 @@ __global__ void __launch_bounds__(kBlock)
       }
       s_pos[t] = colpos[t0 + t];
     }
-    for (int32_t lr = threadIdx.x; lr <= n_rows; lr += kBlock) {
-      s_rp[lr] = row_ptr[r0 + lr] - k0;
-      s_ip[lr] = inc_ptr[r0 + lr] - t0;
-    }
     __syncthreads();
     for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock)
       for (int32_t k = s_rp[lr]; k < s_rp[lr + 1]; ++k) s_erow[k] = (uint16_t)lr;
     __syncthreads();
-    for (int32_t kl = threadIdx.x; kl < n_ent; kl += kBlock) {
-      const int32_t lr = s_erow[kl];
-      const uint32_t p = (uint32_t)(kl - s_rp[lr]);
-      const int32_t ta = s_ip[lr], tz = s_ip[lr + 1];
-      T acc = T(0);
-      if (entry_mask && tz - ta <= 32) {
-        // only the incidences that contribute, lowest bit (= lowest element) first
-        uint32_t m = entry_mask[k0 + kl];
-        while (m) {
-          const int32_t t = ta + __builtin_ctz(m);
-          m &= m - 1u;
-          const uint32_t cp = s_pos[t];
-          int jj = 0;
-#pragma unroll
-          for (int j = 1; j < B; ++j) jj = (((cp >> (8 * j)) & 0xffu) == p) ? j : jj;
-          acc += s_val[t * B + jj];
-        }
-      } else {
-        for (int32_t t = ta; t < tz; ++t) {
-          const uint32_t cp = s_pos[t];
-#pragma unroll
-          for (int j = 0; j < B; ++j)
-            if (((cp >> (8 * j)) & 0xffu) == p) acc += s_val[t * B + j];
-        }
-      }
-      values[k0 + kl] = acc;
-    }
+    sum_entries<T, B, false>(s_val, s_pos, s_rp, s_ip, s_erow, entry_mask, k0, n_ent, m0, m1, values);
     return;
   }
   for (int32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
@@ -339,7 +621,7 @@ int launch(const tm_mesh_view* m, const tm_form_desc* f, const tm_rows_view* rv,
   if (nb == 0) return TM_OK;
 #define TM_ROWS_LAUNCH(FORM)                                                                              \
   p1_rows_kernel<T, D, FORM><<<nb, kBlock, 0, s>>>(m->conn, pts, coeff, c, rv->row_ptr, rv->inc_ptr, rv->inc, \
-                                                   rv->colpos, rv->entry_mask, rv->blk_ptr, nb, out)
+                                                   rv->colpos, rv->entry_mask, rv->blk_ptr, rv->blk_info, nb, out)
   switch (f->form) {
     case TM_FORM_LAPLACE: TM_ROWS_LAUNCH(TM_FORM_LAPLACE); break;
     case TM_FORM_DIFFUSION_ELEM:
@@ -393,8 +675,8 @@ int tm_rows_reduce_launch(int32_t dtype, int32_t cell, const tm_rows_view* rows,
     return tmh::fail(TM_ERR_ARG, "NULL buffer");
 #define TM_RR(T, D, IO)                                                                                          \
   p1_rows_reduce_kernel<T, D, IO><<<nb, kBlock, 0, stream>>>((const T*)slot, rows->row_ptr, rows->inc_ptr, rows->inc, \
-                                                             rows->colpos, rows->entry_mask, rows->blk_ptr, nb,      \
-                                                             (T*)values)
+                                                             rows->colpos, rows->entry_mask, rows->blk_ptr,          \
+                                                             rows->blk_info, nb, (T*)values)
   if (dtype == TM_F64) {
     if (cell == TM_TET4) { if (inc_order) TM_RR(double, 3, true); else TM_RR(double, 3, false); }
     else { if (inc_order) TM_RR(double, 2, true); else TM_RR(double, 2, false); }

@@ -9,12 +9,15 @@
 //     dot product is reduced per system from per-workgroup partials in a fixed
 //     order -- results are bitwise reproducible;
 //   * all scalars (rz, alpha, beta, |r|^2, |b|^2, active flag) live in device
-//     memory, so an iteration is five launches with no host round trip, and a
-//     chunk of iterations can be captured in one graph;
+//     memory, so an iteration needs no host round trip, and a chunk of
+//     iterations can be captured in one graph;
 //   * a system stops updating once |r| <= rtol |b| (its active flag clears),
-//     the host only polls between chunks.
-// Scalars layout per system s (double): sc[s*8 + {0:rz, 1:pq, 2:alpha, 3:beta,
-// 4:rr, 5:bb, 6:active, 7:iterations}].
+//     the host only polls between chunks;
+//   * SpMV rows are summed in ascending entry order (CSR-stream through LDS).
+// Two iteration variants: the textbook one, five launches per iteration with
+// scalars sc[s*8 + {0:rz, 1:pq, 2:alpha, 3:beta, 4:rr, 5:bb, 6:active,
+// 7:iterations}], and a two-launch Chronopoulos-Gear one (described where its
+// kernels are defined).
 #include <hip/hip_runtime.h>
 
 #include "tm_internal.h"
@@ -25,27 +28,77 @@ constexpr int kBlock = 256;
 
 enum { kRZ = 0, kPQ = 1, kAlpha = 2, kBeta = 3, kRR = 4, kBB = 5, kActive = 6, kIter = 7, kNS = 8 };
 
-template <typename T>
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-  // deterministic tree sum over the workgroup
-  sh[threadIdx.x] = v;
+// Deterministic workgroup sums of N values at once: a butterfly within each
+// 64-lane wavefront (every lane ends with the same bits: each step adds the
+// same two operands, commuted), then the four wavefront sums in wave order.
+// sh needs N * (kBlock / 64) doubles.
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] += __shfl_xor(v[n], off, 64);
+  constexpr int W = kBlock / 64;
+  const int wv = threadIdx.x / 64;
+  if (threadIdx.x % 64 == 0)
+#pragma unroll
+    for (int n = 0; n < N; ++n) sh[wv * N + n] = v[n];
   __syncthreads();
 #pragma unroll
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
+  for (int n = 0; n < N; ++n) {
+    double t = sh[n];
+#pragma unroll
+    for (int w = 1; w < W; ++w) t += sh[w * N + n];
+    v[n] = t;
   }
-  const double r = sh[0];
   __syncthreads();
-  return r;
 }
 
 template <typename T>
-__device__ __forceinline__ T row_dot(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                     const T* __restrict__ av, const T* __restrict__ x, int32_t i) {
-  T s = T(0);
-  for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += av[k] * x[ci[k]];
-  return s;
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  double a[1] = {v};
+  block_sum_n<1>(a, sh);
+  return a[0];
+}
+
+// Rows [row0, row1) of A x, CSR-stream: the workgroup walks the rows'
+// contiguous entry range [rp[row0], rp[row1]) in chunks of kChunk entries,
+// each thread loading kPer entries of a chunk at once (coalesced values and
+// column ids, then the x gathers), and stages the products in LDS; then the
+// thread of row i (= row0 + threadIdx.x) adds its row's products in
+// ascending entry order.  Returns that row sum (0 for threads past row1).
+constexpr int kPer = 8;
+constexpr int kChunk = kPer * kBlock;
+
+template <typename T>
+__device__ __forceinline__ T rows_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                       const T* __restrict__ av, const T* __restrict__ x, int32_t row0,
+                                       int32_t row1, T* s_prod) {
+  const int32_t i = row0 + threadIdx.x;
+  const int32_t k0 = rp[row0], k1 = rp[row1];
+  const int32_t ra = i < row1 ? rp[i] : 0, rb = i < row1 ? rp[i + 1] : 0;
+  T acc = T(0);
+  for (int32_t c0 = k0; c0 < k1; c0 += kChunk) {
+    const int32_t n = k1 - c0 < kChunk ? k1 - c0 : kChunk;
+    T a[kPer];
+    int32_t c[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int32_t e = threadIdx.x + j * kBlock;
+      a[j] = e < n ? av[c0 + e] : T(0);
+      c[j] = e < n ? ci[c0 + e] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int32_t e = threadIdx.x + j * kBlock;
+      if (e < n) s_prod[e] = a[j] * x[c[j]];
+    }
+    __syncthreads();
+    const int32_t lo = ra > c0 ? ra : c0, hi = rb < c0 + n ? rb : c0 + n;
+    for (int32_t e = lo; e < hi; ++e) acc += s_prod[e - c0];
+    __syncthreads();
+  }
+  return acc;
 }
 
 // r = b - A x, z = dinv r, p = z; partials: rz, bb, rr
@@ -57,11 +110,14 @@ __global__ void __launch_bounds__(kBlock)
                     const T* __restrict__ dinv, T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
                     double* __restrict__ part) {
   __shared__ double sh[kBlock];
+  __shared__ T s_prod[kChunk];
   const int32_t blk = blockIdx.x;
-  const int32_t i = blk_row0[blk] + threadIdx.x;
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const T ax = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
+  const int32_t i = row0 + threadIdx.x;
   double rz = 0.0, bb = 0.0, rr = 0.0;
-  if (i < blk_row1[blk]) {
-    const T ri = b[i] - row_dot(rp, ci, av, x, i);
+  if (i < row1) {
+    const T ri = b[i] - ax;
     const T zi = dinv[i] * ri;
     r[i] = ri;
     z[i] = zi;
@@ -92,10 +148,13 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t blk = blockIdx.x;
   const int32_t s = blk_sys[blk];
   if (sc[s * kNS + kActive] == 0.0) return;  // uniform per workgroup
-  const int32_t i = blk_row0[blk] + threadIdx.x;
+  __shared__ T s_prod[kChunk];
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const T qv = rows_spmv<T>(rp, ci, av, p, row0, row1, s_prod);
+  const int32_t i = row0 + threadIdx.x;
   double pq = 0.0;
-  if (i < blk_row1[blk]) {
-    const T qi = row_dot(rp, ci, av, p, i);
+  if (i < row1) {
+    const T qi = qv;
     q[i] = qi;
     pq = (double)p[i] * (double)qi;
   }
@@ -205,43 +264,51 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// Two-launch PCG iteration (Chronopoulos-Gear recurrences): per iteration one
-// elementwise kernel (p, s, x, r, u updates and gamma = (r, u), |r|^2) and
-// one SpMV kernel (w = A u and delta = (w, u)); the per-system sums are closed
-// by the last workgroup of each system to finish (atomic ticket), which adds
-// the partials in workgroup order -- deterministic -- and publishes the next
-// alpha / beta.  Recurrences (u = M r, w = A u, s = A p):
-//   p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s, u = M r
-//   beta' = gamma' / gamma,  alpha' = gamma' / (delta' - beta' gamma' / alpha)
-// Extra scalars (after the 8 of the five-launch variant are reused):
-//   sc[s*8 + 0] gamma, 1 delta, 2 alpha, 3 beta, 4 |r|^2, 5 |b|^2, 6 active, 7 iterations.
+// Two-launch PCG iteration (Chronopoulos-Gear recurrences, u = M r, w = A u,
+// s = A p):
+//   vec  (V_k): beta_k = gamma_k / gamma_{k-1},
+//               alpha_k = gamma_k / (delta_k - beta_k gamma_k / alpha_{k-1})
+//               (k = 0: beta = 0, alpha = gamma / delta);
+//               p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s,
+//               u = M r; partials of gamma_{k+1} = (r, u) and |r|^2
+//   spmv (S_k): |r|^2 test (clears the system's active flag), w = A u,
+//               partials of delta_{k+1} = (w, u)
+// No kernel waits on another workgroup: every workgroup of the consuming
+// kernel sums its system's partials itself, in workgroup order, so all of
+// them derive bit-identical scalars without fences or atomics.  What one
+// kernel both reads and writes is double-buffered by the iteration's parity
+// P (V_k reads slot 1-P, writes slot P):
+//   part[(0|1)*nb + w] gamma partials, part[2*nb + w] |r|^2, part[3*nb + w]
+//   delta, part[4*nb + w] |b|^2;
+//   sc[s*16 + 8 + 4*P + {0: alpha, 1: gamma, 2: iterations}].
+// Host-visible copies (written by the system's first workgroup only):
+//   sc[s*16 + {4: |r|^2, 5: |b|^2, 6: active, 7: iterations}].
 // ---------------------------------------------------------------------------
+constexpr int kNS2 = 16;
 
-// Sums part[b*3 + c] over the system's workgroups in order (only the last
-// workgroup of a system calls it); loads bypass the non-coherent L1.
-__device__ __forceinline__ double system_sum(const double* part, int32_t b0, int32_t b1, int c, double* sh) {
+// sum over the system's workgroups [b0, b1) of part[w], in w order (every
+// thread of the workgroup gets the same value)
+__device__ __forceinline__ double system_sum(const double* __restrict__ part, int32_t b0, int32_t b1,
+                                             double* sh) {
   double v = 0.0;
-  for (int32_t b = b0 + threadIdx.x; b < b1; b += kBlock)
-    v += __hip_atomic_load(&part[b * 3 + c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int32_t w = b0 + threadIdx.x; w < b1; w += kBlock) v += part[w];
   return block_sum<double>(v, sh);
 }
 
-// Ticket for "am I the last workgroup of system s"; resets the counter.
-__device__ __forceinline__ bool last_block(int32_t* counter, int32_t n_blocks_sys, int* sh_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int32_t t = atomicAdd(counter, 1);
-    *sh_flag = (t == n_blocks_sys - 1) ? 1 : 0;
-    if (*sh_flag) atomicExch(counter, 0);
+// two system sums in one workgroup reduction
+__device__ __forceinline__ void system_sum2(const double* __restrict__ pa, const double* __restrict__ pb, int32_t b0,
+                                            int32_t b1, double* sh, double& a, double& b) {
+  double v[2] = {0.0, 0.0};
+  for (int32_t w = b0 + threadIdx.x; w < b1; w += kBlock) {
+    v[0] += pa[w];
+    v[1] += pb[w];
   }
-  __syncthreads();
-  const bool last = *sh_flag != 0;
-  if (last) __threadfence();
-  return last;
+  block_sum_n<2>(v, sh);
+  a = v[0];
+  b = v[1];
 }
 
-// r = b - A x, u = dinv r, p = s = 0; gamma = (r, u), |b|^2, |r|^2
+// r = b - A x, u = dinv r, p = s = 0; partials gamma_0 (slot 1), |r|^2, |b|^2
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     cg2_init_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
@@ -249,15 +316,17 @@ __global__ void __launch_bounds__(kBlock)
                     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ av,
                     const T* __restrict__ b, const T* __restrict__ x, const T* __restrict__ dinv, T* __restrict__ r,
                     T* __restrict__ u, T* __restrict__ p, T* __restrict__ sv, double* __restrict__ part,
-                    double* __restrict__ sc, int32_t* __restrict__ counters, double tol2) {
+                    double* __restrict__ sc, int32_t nb) {
   __shared__ double sh[kBlock];
-  __shared__ int flag;
+  __shared__ T s_prod[kChunk];
   const int32_t blk = blockIdx.x;
   const int32_t sy = blk_sys[blk];
-  const int32_t i = blk_row0[blk] + threadIdx.x;
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const T ax = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
+  const int32_t i = row0 + threadIdx.x;
   double g = 0.0, bb = 0.0, rr = 0.0;
-  if (i < blk_row1[blk]) {
-    const T ri = b[i] - row_dot(rp, ci, av, x, i);
+  if (i < row1) {
+    const T ri = b[i] - ax;
     const T ui = dinv[i] * ri;
     r[i] = ri;
     u[i] = ui;
@@ -267,112 +336,126 @@ __global__ void __launch_bounds__(kBlock)
     bb = (double)b[i] * (double)b[i];
     rr = (double)ri * (double)ri;
   }
-  g = block_sum<T>(g, sh);
-  bb = block_sum<T>(bb, sh);
-  rr = block_sum<T>(rr, sh);
-  if (threadIdx.x == 0) {
-    part[blk * 3 + 0] = g;
-    part[blk * 3 + 1] = bb;
-    part[blk * 3 + 2] = rr;
+  {
+    double v[3] = {g, bb, rr};
+    block_sum_n<3>(v, sh);
+    g = v[0];
+    bb = v[1];
+    rr = v[2];
   }
-  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
-  if (!last_block(&counters[sy], b1 - b0, &flag)) return;
-  const double G = system_sum(part, b0, b1, 0, sh);
-  const double BB = system_sum(part, b0, b1, 1, sh);
-  const double RR = system_sum(part, b0, b1, 2, sh);
   if (threadIdx.x == 0) {
-    double* c = sc + sy * kNS;
-    c[kRZ] = G;
-    c[kBB] = BB;
-    c[kRR] = RR;
-    c[kBeta] = 0.0;
-    c[kIter] = 0.0;
-    c[kActive] = (RR > tol2 * BB && BB > 0.0) ? 1.0 : 0.0;
+    part[1 * nb + blk] = g;
+    part[2 * nb + blk] = rr;
+    part[4 * nb + blk] = bb;
+    if (blk == sys_blk[sy]) {
+      sc[sy * kNS2 + 8 + 4 + 2] = 0.0;  // iterations in slot 1 (read by V_0)
+      sc[sy * kNS2 + 7] = 0.0;
+    }
   }
 }
 
-// w = A u; delta = (w, u); last workgroup: alpha (first=1: alpha = gamma / delta)
+// |r|^2 test, then w = A u and partials of delta = (w, u).  first: also sums
+// |b|^2 (published for the later S launches, which read it from sc).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     cg2_spmv_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
                     const int32_t* __restrict__ blk_row1, const int32_t* __restrict__ sys_blk,
                     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ av,
                     const T* __restrict__ u, T* __restrict__ w, double* __restrict__ part, double* __restrict__ sc,
-                    int32_t* __restrict__ counters, int first) {
+                    int32_t nb, double tol2, int first) {
   __shared__ double sh[kBlock];
-  __shared__ int flag;
   const int32_t blk = blockIdx.x;
   const int32_t sy = blk_sys[blk];
-  if (sc[sy * kNS + kActive] == 0.0) return;  // uniform per workgroup and per system
-  const int32_t i = blk_row0[blk] + threadIdx.x;
+  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
+  double* c = sc + sy * kNS2;
+  double rr, bb;
+  if (first) {
+    system_sum2(part + 2 * nb, part + 4 * nb, b0, b1, sh, rr, bb);
+  } else {
+    rr = system_sum(part + 2 * nb, b0, b1, sh);
+    bb = c[5];
+  }
+  const bool active = rr > tol2 * bb && bb > 0.0;
+  if (threadIdx.x == 0 && blk == b0) {
+    c[4] = rr;
+    if (first) c[5] = bb;
+    c[6] = active ? 1.0 : 0.0;
+  }
+  if (!active) return;  // uniform per workgroup
+  __shared__ T s_prod[kChunk];
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const T wv = rows_spmv<T>(rp, ci, av, u, row0, row1, s_prod);
+  const int32_t i = row0 + threadIdx.x;
   double d = 0.0;
-  if (i < blk_row1[blk]) {
-    const T wi = row_dot(rp, ci, av, u, i);
+  if (i < row1) {
+    const T wi = wv;
     w[i] = wi;
     d = (double)wi * (double)u[i];
   }
   d = block_sum<T>(d, sh);
-  if (threadIdx.x == 0) part[blk * 3 + 0] = d;
-  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
-  if (!last_block(&counters[sy], b1 - b0, &flag)) return;
-  const double D = system_sum(part, b0, b1, 0, sh);
-  if (threadIdx.x == 0) {
-    double* c = sc + sy * kNS;
-    const double g = c[kRZ];
-    double den = first ? D : D - c[kBeta] * g / c[kAlpha];
-    c[kPQ] = D;
-    c[kAlpha] = den != 0.0 ? g / den : 0.0;
-  }
+  if (threadIdx.x == 0) part[3 * nb + blk] = d;
 }
 
-// p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s; u = dinv r;
-// gamma' = (r, u), |r|^2; last workgroup: beta' = gamma'/gamma, convergence.
+// vector updates of iteration k (parity P); partials of gamma_{k+1}, |r|^2
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
     cg2_vec_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
                    const int32_t* __restrict__ blk_row1, const int32_t* __restrict__ sys_blk,
                    const T* __restrict__ dinv, const T* __restrict__ w, T* __restrict__ x, T* __restrict__ r,
                    T* __restrict__ u, T* __restrict__ p, T* __restrict__ sv, double* __restrict__ part,
-                   double* __restrict__ sc, int32_t* __restrict__ counters, double tol2) {
+                   double* __restrict__ sc, int32_t nb, int parity) {
   __shared__ double sh[kBlock];
-  __shared__ int flag;
   const int32_t blk = blockIdx.x;
   const int32_t sy = blk_sys[blk];
-  const double* c0 = sc + sy * kNS;
-  if (c0[kActive] == 0.0) return;
-  const T alpha = (T)c0[kAlpha], beta = (T)c0[kBeta];
+  double* c = sc + sy * kNS2;
+  if (c[6] == 0.0) return;  // cleared by the preceding S launch; uniform per system
+  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
+  const int P = parity & 1, Q = P ^ 1;
+  double gamma, delta;
+  system_sum2(part + Q * nb, part + 3 * nb, b0, b1, sh, gamma, delta);
+  const double* prev = c + 8 + 4 * Q;
+  const double iter = prev[2];
+  double alpha, beta;
+  if (iter == 0.0) {
+    beta = 0.0;
+    alpha = delta != 0.0 ? gamma / delta : 0.0;
+  } else {
+    beta = prev[1] != 0.0 ? gamma / prev[1] : 0.0;
+    const double den = delta - beta * gamma / prev[0];
+    alpha = den != 0.0 ? gamma / den : 0.0;
+  }
+  if (threadIdx.x == 0 && blk == b0) {
+    double* cur = c + 8 + 4 * P;
+    cur[0] = alpha;
+    cur[1] = gamma;
+    cur[2] = iter + 1.0;
+    c[7] = iter + 1.0;
+  }
+  const T al = (T)alpha, be = (T)beta;
   const int32_t i = blk_row0[blk] + threadIdx.x;
   double g = 0.0, rr = 0.0;
   if (i < blk_row1[blk]) {
-    const T pi = u[i] + beta * p[i];
-    const T si = w[i] + beta * sv[i];
+    const T pi = u[i] + be * p[i];
+    const T si = w[i] + be * sv[i];
     p[i] = pi;
     sv[i] = si;
-    x[i] += alpha * pi;
-    const T ri = r[i] - alpha * si;
+    x[i] += al * pi;
+    const T ri = r[i] - al * si;
     const T ui = dinv[i] * ri;
     r[i] = ri;
     u[i] = ui;
     g = (double)ri * (double)ui;
     rr = (double)ri * (double)ri;
   }
-  g = block_sum<T>(g, sh);
-  rr = block_sum<T>(rr, sh);
-  if (threadIdx.x == 0) {
-    part[blk * 3 + 0] = g;
-    part[blk * 3 + 2] = rr;
+  {
+    double v[2] = {g, rr};
+    block_sum_n<2>(v, sh);
+    g = v[0];
+    rr = v[1];
   }
-  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
-  if (!last_block(&counters[sy], b1 - b0, &flag)) return;
-  const double G = system_sum(part, b0, b1, 0, sh);
-  const double RR = system_sum(part, b0, b1, 2, sh);
   if (threadIdx.x == 0) {
-    double* c = sc + sy * kNS;
-    c[kBeta] = c[kRZ] != 0.0 ? G / c[kRZ] : 0.0;
-    c[kRZ] = G;
-    c[kRR] = RR;
-    c[kIter] += 1.0;
-    c[kActive] = RR > tol2 * c[kBB] ? 1.0 : 0.0;
+    part[P * nb + blk] = g;
+    part[2 * nb + blk] = rr;
   }
 }
 
@@ -380,9 +463,13 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock)
     spmv_kernel(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ av,
                 const T* __restrict__ x, T* __restrict__ y, int64_t n_rows) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n_rows) return;
-  y[i] = row_dot(rp, ci, av, x, (int32_t)i);
+  // 256 rows per workgroup, CSR-stream
+  __shared__ T s_prod[kChunk];
+  const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+  const int32_t row0 = (int32_t)r0, row1 = (int32_t)(r0 + kBlock < n_rows ? r0 + kBlock : n_rows);
+  const T v = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
+  const int32_t i = row0 + threadIdx.x;
+  if (i < row1) y[i] = v;
 }
 
 template <typename T>
@@ -395,11 +482,6 @@ __global__ void __launch_bounds__(kBlock)
   for (int32_t k = rp[i]; k < rp[i + 1]; ++k)
     if (ci[k] == (int32_t)i) d += av[k];
   dinv[i] = d != T(0) ? T(1) / d : T(1);
-}
-
-__global__ void fill_counters(int32_t* c, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) c[i] = 0;
 }
 
 int hip_status(const char* what) {
@@ -442,30 +524,28 @@ int pcg_iterate(const tm_pcg_view* v, void* x, int32_t n_iter, hipStream_t s) {
 
 template <typename T>
 int cg2_init(const tm_pcg_view* v, const void* b, const void* x, hipStream_t s) {
-  const int64_t nb = v->n_blocks;
+  const int32_t nb = (int32_t)v->n_blocks;
   const double tol2 = v->rtol * v->rtol;
-  fill_counters<<<(v->n_sys + 255) / 256, 256, 0, s>>>(v->counters, v->n_sys);
   cg2_init_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, v->row_ptr, v->col_idx,
                                            (const T*)v->values, (const T*)b, (const T*)x, (const T*)v->dinv,
-                                           (T*)v->r, (T*)v->z, (T*)v->p, (T*)v->s, v->partials, v->scalars,
-                                           v->counters, tol2);
+                                           (T*)v->r, (T*)v->z, (T*)v->p, (T*)v->s, v->partials, v->scalars, nb);
   cg2_spmv_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, v->row_ptr, v->col_idx,
                                            (const T*)v->values, (const T*)v->z, (T*)v->q, v->partials, v->scalars,
-                                           v->counters, 1);
+                                           nb, tol2, 1);
   return hip_status("pcg init");
 }
 
 template <typename T>
 int cg2_iterate(const tm_pcg_view* v, void* x, int32_t n_iter, hipStream_t s) {
-  const int64_t nb = v->n_blocks;
+  const int32_t nb = (int32_t)v->n_blocks;
   const double tol2 = v->rtol * v->rtol;
   for (int32_t it = 0; it < n_iter; ++it) {
     cg2_vec_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, (const T*)v->dinv,
                                             (const T*)v->q, (T*)x, (T*)v->r, (T*)v->z, (T*)v->p, (T*)v->s,
-                                            v->partials, v->scalars, v->counters, tol2);
+                                            v->partials, v->scalars, nb, (v->parity + it) & 1);
     cg2_spmv_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, v->row_ptr,
                                              v->col_idx, (const T*)v->values, (const T*)v->z, (T*)v->q, v->partials,
-                                             v->scalars, v->counters, 0);
+                                             v->scalars, nb, tol2, 0);
   }
   return hip_status("pcg iterate");
 }
@@ -524,7 +604,7 @@ extern "C" int tm_pcg_init(const tm_pcg_view* view, const void* b, const void* x
   if (st) return st;
   if (!b || !x) return tmh::fail(TM_ERR_ARG, "NULL b / x");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
-  const bool two = view->s && view->counters;
+  const bool two = view->two_launch != 0;
   if (view->dtype == TM_F64) return two ? cg2_init<double>(view, b, x, s) : pcg_init<double>(view, b, x, s);
   if (view->dtype == TM_F32) return two ? cg2_init<float>(view, b, x, s) : pcg_init<float>(view, b, x, s);
   return tmh::fail(TM_ERR_ARG, "unknown dtype");
@@ -537,7 +617,7 @@ extern "C" int tm_pcg_iterate(const tm_pcg_view* view, void* x, int32_t n_iter, 
   if (st) return st;
   if (!x || n_iter < 0) return tmh::fail(TM_ERR_ARG, "bad x / n_iter");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
-  const bool two = view->s && view->counters;
+  const bool two = view->two_launch != 0;
   if (view->dtype == TM_F64) return two ? cg2_iterate<double>(view, x, n_iter, s) : pcg_iterate<double>(view, x, n_iter, s);
   if (view->dtype == TM_F32) return two ? cg2_iterate<float>(view, x, n_iter, s) : pcg_iterate<float>(view, x, n_iter, s);
   return tmh::fail(TM_ERR_ARG, "unknown dtype");

@@ -29,6 +29,15 @@ def layout_hash(row: np.ndarray, col: np.ndarray) -> str:
                           + np.ascontiguousarray(col, dtype=np.int64).tobytes()).hexdigest()
 
 
+def repeated_vertex(conn: torch.Tensor) -> bool:
+    """True if some element lists a node twice (the row kernels stage one
+    diagonal value per incidence, which assumes distinct vertices)."""
+    c = conn.detach()
+    B = c.shape[1]
+    eq = (c.unsqueeze(2) == c.unsqueeze(1)).sum(dim=(1, 2))
+    return bool((eq > B).any().item())
+
+
 @dataclass
 class Pattern:
     n_points: int
@@ -50,14 +59,18 @@ class Pattern:
         self._plans = {}
         self._colpos = None
 
-    ROWS_CAP = 512    # incidences staged per workgroup by the row-centric P1 kernels (tm_rows.hip)
+    # blocks of the row-centric P1 kernels (tm_rows.hip): at most ROWS_CAP
+    # incidences and ROWS_MAX rows per workgroup
+    ROWS_CAP = 512
+    ROWS_MAX = 128
 
     def colpos(self, conn: torch.Tensor) -> Optional[torch.Tensor]:
         """Packed per-incidence column positions for the fused P1 path, or
-        None when the element has more than 4 nodes or a row has more than 255
-        entries (the two-stage path then serves the call)."""
+        None when the element has more than 4 nodes, a row has more than 255
+        entries or an element repeats a vertex (the two-stage path then serves
+        the call)."""
         if self._colpos is None:
-            if self.n_basis > 4 or self.inc is None:
+            if self.n_basis > 4 or self.inc is None or repeated_vertex(conn):
                 self._colpos = False
             else:
                 c = np.ascontiguousarray(conn.detach().cpu().numpy().astype(np.int32))
@@ -88,17 +101,33 @@ class Pattern:
         cp = self.colpos(conn)
         if cp is None:
             return None
-        bp = self.plan(self.ROWS_CAP, node=True)
+        bp = self.plan(self.ROWS_CAP, node=True, max_rows=self.ROWS_MAX)
+        info = self.block_info(self.ROWS_CAP)
         return nat.RowsView(self.nnz, self.row_ptr.data_ptr(), self.inc_ptr.data_ptr(), self.inc.data_ptr(),
-                            cp.data_ptr(), self._entry_mask.data_ptr(), bp.numel() - 1, bp.data_ptr())
+                            cp.data_ptr(), self._entry_mask.data_ptr(), bp.numel() - 1, bp.data_ptr(),
+                            info.data_ptr())
 
-    def plan(self, cap: int, node: bool = False) -> torch.Tensor:
+    def block_info(self, cap: int) -> torch.Tensor:
+        """[n_blocks + 1, 4] int32: first row, first incidence and first CSR
+        entry of every block of the node plan (one 16-byte load per bound)."""
+        key = ("info", cap)
+        if key not in self._plans:
+            bp = self.plan(cap, node=True, max_rows=self.ROWS_MAX).long()
+            info = torch.zeros(bp.numel(), 4, dtype=torch.int32, device=bp.device)
+            info[:, 0] = bp.to(torch.int32)
+            info[:, 1] = self.inc_ptr[bp]
+            info[:, 2] = self.row_ptr[bp]
+            self._plans[key] = info.contiguous()
+        return self._plans[key]
+
+    def plan(self, cap: int, node: bool = False, max_rows: Optional[int] = None) -> torch.Tensor:
         """Reduction plan (blk_ptr, int32, on the pattern's device) for entries
-        (node=False) or for the node incidence lists (node=True)."""
-        key = (cap, node)
+        (node=False) or for the node incidence lists (node=True); max_rows
+        caps the segments (rows) per block."""
+        key = (cap, node, max_rows)
         if key not in self._plans:
             seg = (self.inc_ptr if node else self.seg_ptr).cpu().numpy()
-            self._plans[key] = torch.from_numpy(nat.reduce_plan(seg, cap)).to(self.seg_ptr.device)
+            self._plans[key] = torch.from_numpy(nat.reduce_plan(seg, cap, max_rows)).to(self.seg_ptr.device)
         return self._plans[key]
 
     @staticmethod
@@ -166,7 +195,7 @@ class Pattern:
         pat = Pattern(n_points, E, B, nnz, row_ptr, col_idx, seg_ptr, seg_slot, slot2nnz, row, col, h, inc_ptr, inc)
         if colpos is not None:
             too_long = bool(((row_ptr[1:] - row_ptr[:-1]) > 255).any().item())
-            if not too_long:
+            if not too_long and not repeated_vertex(conn):
                 pat._colpos, pat._entry_mask = colpos, mask
             else:
                 pat._colpos = False

@@ -64,7 +64,9 @@ class BatchedPCG:
     """Jacobi-PCG over a block-diagonal stack of CSR systems on the device.
 
     ``two_launch`` (default True) uses the two-launch Chronopoulos-Gear
-    iteration; False the five-launch textbook PCG.
+    iteration for stacks whose largest system spans at most
+    ``TWO_LAUNCH_MAX_BLOCKS`` workgroups (every workgroup re-sums its
+    system's partials); otherwise, or with False, the five-launch textbook PCG.
 
     Parameters
     ----------
@@ -75,6 +77,7 @@ class BatchedPCG:
     """
 
     two_launch: bool = True
+    TWO_LAUNCH_MAX_BLOCKS = 2048
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor,
                  sys_rows: Sequence[int], rtol: float = 1e-10):
@@ -97,11 +100,14 @@ class BatchedPCG:
         t = lambda a: torch.tensor(a, dtype=torch.int32, device=self.dev)
         self.blk_sys, self.blk_row0, self.blk_row1, self.sys_blk = t(blk_sys), t(blk0), t(blk1), t(sys_blk)
         self.n_blocks = len(blk_sys)
+        max_blk = max((sys_blk[i + 1] - sys_blk[i] for i in range(self.n_sys)), default=0)
+        self.two = bool(self.two_launch) and max_blk <= self.TWO_LAUNCH_MAX_BLOCKS
+        self.stride = 16 if self.two else 8
+        self._parity = 0
         e = lambda: torch.empty(self.n, dtype=self.dtype, device=self.dev)
         self.dinv, self.r, self.z, self.p, self.q, self.x, self.s = e(), e(), e(), e(), e(), e(), e()
-        self.counters = torch.zeros(self.n_sys, dtype=torch.int32, device=self.dev)
-        self.partials = torch.zeros(self.n_blocks * 3, dtype=torch.float64, device=self.dev)
-        self.scalars = torch.zeros(self.n_sys * 8, dtype=torch.float64, device=self.dev)
+        self.partials = torch.zeros(self.n_blocks * (5 if self.two else 3), dtype=torch.float64, device=self.dev)
+        self.scalars = torch.zeros(self.n_sys * self.stride, dtype=torch.float64, device=self.dev)
         self.rtol = rtol
         self._graphs = {}
         self.refresh_values(None)
@@ -117,14 +123,14 @@ class BatchedPCG:
 
     def _view(self) -> nat.PcgView:
         v = nat.PcgView()
-        v.dtype, v.reserved, v.n_rows, v.n_sys, v.n_blocks = nat.dtype_code(self.dtype), 0, self.n, self.n_sys, \
-            self.n_blocks
+        v.dtype, v.reserved = nat.dtype_code(self.dtype), 0
+        v.n_rows, v.n_sys, v.n_blocks = self.n, self.n_sys, self.n_blocks
         for name in ("row_ptr", "col_idx", "values", "dinv", "r", "z", "p", "q", "partials", "scalars",
                      "blk_sys", "blk_row0", "blk_row1", "sys_blk"):
             setattr(v, name, getattr(self, name).data_ptr())
         v.rtol = self.rtol
-        if self.two_launch:
-            v.s, v.counters = self.s.data_ptr(), self.counters.data_ptr()
+        if self.two:
+            v.s, v.two_launch, v.parity = self.s.data_ptr(), 1, self._parity
         return v
 
     def residual(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
@@ -135,12 +141,15 @@ class BatchedPCG:
         xx = x.to(self.dtype).contiguous()
         nat.check(nat.lib().tm_pcg_init(ctypes.byref(view), bb.data_ptr(), xx.data_ptr(),
                                         nat.stream_of(self.dev)), "tm_pcg_init")
-        sc = self.scalars.view(self.n_sys, 8)
+        self._parity = 0
+        sc = self.scalars.view(self.n_sys, self.stride)
         return torch.sqrt(sc[:, 4] / torch.clamp(sc[:, 5], min=1e-300))
 
-    def _iterate(self, view, n):
+    def _iterate(self, n):
+        view = self._view()
         nat.check(nat.lib().tm_pcg_iterate(ctypes.byref(view), self.x.data_ptr(), n, nat.stream_of(self.dev)),
                   "tm_pcg_iterate")
+        self._parity = (self._parity + n) & 1
 
     def solve(self, b: torch.Tensor, x0: Optional[torch.Tensor] = None, max_iter: int = 20000, chunk: int = 32,
               graph: bool = True):
@@ -153,6 +162,9 @@ class BatchedPCG:
         bb = b.to(self.dtype).contiguous()
         nat.check(nat.lib().tm_pcg_init(ctypes.byref(view), bb.data_ptr(), self.x.data_ptr(),
                                         nat.stream_of(self.dev)), "tm_pcg_init")
+        self._parity = 0
+        if self.two:
+            chunk += chunk & 1  # graphs replay with fixed parities: keep chunks even
         done = 0
         g = None
         if graph:
@@ -163,20 +175,21 @@ class BatchedPCG:
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.stream(s):
                     with torch.cuda.graph(g, stream=s):
-                        self._iterate(view, chunk)
+                        self._iterate(chunk)
                 torch.cuda.current_stream(self.dev).wait_stream(s)
                 self._graphs[chunk] = g
                 # capturing does not execute: start from the initial state again
+                self._parity = 0
         while done < max_iter:
             if g is not None:
                 g.replay()
             else:
-                self._iterate(view, chunk)
+                self._iterate(chunk)
             done += chunk
-            sc = self.scalars.view(self.n_sys, 8)
+            sc = self.scalars.view(self.n_sys, self.stride)
             if not bool((sc[:, 6] != 0).any().item()):
                 break
-        sc = self.scalars.view(self.n_sys, 8)
+        sc = self.scalars.view(self.n_sys, self.stride)
         iters = sc[:, 7].to(torch.int64)
         rel = torch.sqrt(sc[:, 4] / torch.clamp(sc[:, 5], min=1e-300))
         return self.x.clone(), iters, rel

@@ -218,5 +218,6 @@ def test_row_centric_summation_reproduces_oracle_on_cpu():
             vals2[k] = acc
     assert np.array_equal(vals2, ref.numpy())
     # the plan covers every row exactly once, in order
-    bp = pat.plan(Pattern.ROWS_CAP, node=True).numpy()
+    bp = pat.plan(Pattern.ROWS_CAP, node=True, max_rows=Pattern.ROWS_MAX).numpy()
     assert bp[0] == 0 and bp[-1] == m.n_points and np.all(np.diff(bp) > 0)
+    assert np.diff(bp).max() <= Pattern.ROWS_MAX

@@ -294,8 +294,11 @@ def hyperelastic_bench(n=32, reps=5):
     t0 = time.perf_counter()
     model = tm.NeoHookeanModel.from_mesh(m, quadrature_order=2, E=1.0, nu=0.3)
     out["pattern_setup_s"] = time.perf_counter() - t0
-    for dt, mfma in ((torch.float64, False), (torch.float32, False), (torch.float32, True)):
+    for dt, mfma, path in ((torch.float64, False, "slots"), (torch.float64, False, "rows"),
+                           (torch.float32, False, "slots"), (torch.float32, False, "rows"),
+                           (torch.float32, True, "slots")):
         model.use_mfma = mfma
+        model.nh_path = path
         pts = m.points.to(dt)
         u = torch.zeros_like(pts)
         u[:, 0] = 0.1 * pts[:, 0]
@@ -315,7 +318,8 @@ def hyperelastic_bench(n=32, reps=5):
         byts = (4 * 10 * E + s * 3 * 2 * m.n_points + s * E * 930) + (4 * 100 * E + 4 * (nnzb + 1) + s * E * 900
                                                                        + s * nnzb * 9) \
             + (4 * 10 * E + 4 * (m.n_points + 1) + s * E * 30 + s * 3 * m.n_points)
-        out["f64" if dt == torch.float64 else ("f32_mfma" if mfma else "f32")] = {
+        key = ("f64" if dt == torch.float64 else ("f32_mfma" if mfma else "f32")) + ("_rows" if path == "rows" else "")
+        out[key] = {
             "ms_per_newton_assembly": ms, "elements_per_s": E / (ms * 1e-3), "block_entries": nnzb,
             "scalar_nnz": nnzb * 9, "algorithmic_bytes": byts, "GBps": byts / (ms * 1e-3) / 1e9,
             "hbm_frac": byts / (ms * 1e-3) / PEAK_HBM}
@@ -416,6 +420,26 @@ def profile_kernels(reps=5):
                                    "cg2_spmv_kernel", "copy 268435456 B"], "reps": reps}))
 
 
+def profile_hyper(reps=3):
+    """Launch the Neo-Hookean residual + tangent assembly a few times per
+    precision and reduction path, for rocprofv3 runs."""
+    import tensormesh_amd as tm
+    from tensormesh_amd.mesh import unit_cube_tet
+    m = unit_cube_tet(32, 0.0, 0, order=2).to("cuda")
+    model = tm.NeoHookeanModel.from_mesh(m, quadrature_order=2, E=1.0, nu=0.3)
+    for dt, mfma, path in ((torch.float64, False, "slots"), (torch.float64, False, "rows"),
+                           (torch.float32, False, "slots"), (torch.float32, True, "slots")):
+        model.use_mfma = mfma
+        model.nh_path = path
+        pts = m.points.to(dt)
+        u = torch.zeros_like(pts)
+        u[:, 0] = 0.1 * pts[:, 0]
+        for _ in range(reps):
+            model.residual_and_tangent(u, points=pts)
+    torch.cuda.synchronize()
+    print(json.dumps({"profiled": "neo-hookean tetra10 32^3x6", "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -427,6 +451,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the Neo-Hookean and batched legs")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only launch each assembly kernel a few times (for rocprofv3 counter runs)")
+    ap.add_argument("--profile-hyper", action="store_true",
+                    help="only launch the Neo-Hookean assembly a few times (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -438,6 +464,9 @@ def main():
 
     if args.profile_kernels:
         profile_kernels()
+        return
+    if args.profile_hyper:
+        profile_hyper()
         return
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     mesh, kappa, A, setup_s, host = build_workload(dtype)
@@ -520,6 +549,15 @@ def main():
                       "traffic": None if traffic is None else {k: traffic.get(k) for k in ("G", "S")}},
         "pattern_setup_s": setup_s,
         "roofline": roof,
+        # the two-stage pipeline's unique bytes per call (G + S, every array once, int32 indices) and
+        # the compulsory floor (conn, coords, kappa in; values out), each over the whole call's time
+        "roofline_pipeline_denominator": {"bytes": gb + sb, "achieved": (gb + sb) / dt / 1e9,
+                                          "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                          "frac": (gb + sb) / dt / PEAK_HBM},
+        "roofline_compulsory_floor": {"bytes": 4 * 4 * E + s * 3 * N + s * E + s * nnz,
+                                      "achieved": (4 * 4 * E + s * 3 * N + s * E + s * nnz) / dt / 1e9,
+                                      "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                                      "frac": (4 * 4 * E + s * 3 * N + s * E + s * nnz) / dt / PEAK_HBM},
     }
     if not args.no_grad_step:
         out["grad_step"] = grad_step_bench()

@@ -128,6 +128,23 @@ typedef struct tm_rows_view {
                                 or NULL (then read through blk_ptr) */
 } tm_rows_view;
 
+/* Block rows of a node-pair pattern for the block-row reduction: runs of
+ * whole rows (nodes) per workgroup, and for every contribution (in seg_ptr /
+ * seg_slot order: ascending slot within each entry) its workgroup-local
+ * (incidence, column) pair. */
+typedef struct tm_block_rows_view {
+  int64_t nnz;               /* block entries */
+  const int32_t* row_ptr;    /* [n_points + 1] */
+  const int32_t* seg_ptr;    /* [nnz + 1] contributions per entry */
+  const uint16_t* pairs;     /* [n_elem*B*B] from tm_block_rows_pairs */
+  const int32_t* inc_ptr;    /* [n_points + 1] node incidence lists (ascending e*B + i) */
+  const int32_t* inc;        /* [n_elem*B] */
+  int64_t n_blocks;
+  const int32_t* blk_info;   /* [n_blocks + 1][4] {first row r, inc_ptr[r], row_ptr[r], seg_ptr[row_ptr[r]]};
+                                at most 256 / D rows and 6553 incidences per block; blocks of more
+                                than 1024 / D^2 entries take several passes */
+} tm_block_rows_view;
+
 /* Block-diagonal stack of SPD CSR systems for the batched Jacobi-PCG.
  * Workgroups are system-aligned: workgroup w covers rows
  * [blk_row0[w], blk_row1[w]) of system blk_sys[w] (at most 256 rows);
@@ -190,6 +207,15 @@ int tm_reduce_plan(const int32_t* seg_ptr, int64_t n_entries, int32_t cap, int32
 int tm_colpos_build(const int32_t* conn, int64_t n_elem, int32_t n_basis, int64_t n_points,
                     const int32_t* row_ptr, const int32_t* col_idx, const int32_t* inc_ptr, const int32_t* inc,
                     uint32_t* colpos, uint32_t* entry_mask);
+
+/* Workgroup-local (incidence, column) pair of every contribution (device,
+ * stream-ordered): for contribution c of entry k of row r, with slot
+ * s = seg_slot[c] = e*B*B + a*B + b, pairs[c] = (inc_inv[e*B + a] - row_t0[r])*B + b,
+ * where row_t0[r] is the first incidence of r's block.  *overflow is set to 1
+ * (device int32, caller-zeroed) if a pair does not fit 16 bits. */
+int tm_block_rows_pairs(int32_t n_basis, int64_t n_points, const int32_t* row_ptr, const int32_t* seg_ptr,
+                        const int32_t* seg_slot, const int32_t* inc_inv, const int32_t* row_t0, uint16_t* pairs,
+                        int32_t* overflow, tm_stream_t stream);
 
 /* Device pattern builder (stream-ordered; all pointers device).  Same arrays as
  * tm_incidence_build + tm_pattern_build + tm_colpos_build.
@@ -259,6 +285,23 @@ int tm_assemble_p1_adjoint(const tm_mesh_view* mesh, const tm_form_desc* form, c
 int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
                            const tm_pattern_view* pat, const tm_incidence_view* inc, void* tslot_ws,
                            void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
+
+/* Neo-Hookean residual and tangent through incidence-ordered slot rows: the
+ * element kernel writes the block row (e, a) (B blocks of D x D values, and D
+ * residual values) at position inc_inv[e*B + a] of the node-sorted incidence
+ * list, then a block-row reduction sums every entry's contributions in
+ * ascending element order (as tm_assemble_neohookean, whose segment
+ * reduction gathers the same values slot by slot).  Outputs as
+ * tm_assemble_neohookean: tangent [nnz*D*D] in BSR(D x D) node-pair CSR
+ * order, residual [n_points*D]; either may be NULL.  tslot_ws: [n_elem*B*B*D*D],
+ * rslot_ws: [n_elem*B*D], in the mesh dtype.  TM_FLAG_MFMA selects the
+ * matrix-core tangent (float32).  Replaces the reference's autograd residual /
+ * Hessian of NeoHookeanModel.energy (builtin.py:219-333,
+ * element_assembler.py:447-659) with the ReduceProjector sum
+ * (projector.py:95-123). */
+int tm_assemble_neohookean_rows(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
+                                const tm_block_rows_view* rows, const int32_t* inc_inv, void* tslot_ws,
+                                void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
 
 /* Row-centric stage S for P1 (TRI3 / TET4): the entries of tm_assemble_bilinear
  * from a stage-G slot buffer, each CSR row reading its incidences' local rows

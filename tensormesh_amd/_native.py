@@ -49,6 +49,12 @@ class PatternView(ctypes.Structure):
                 ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
 
 
+class BlockRowsView(ctypes.Structure):
+    _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("seg_ptr", ctypes.c_void_p),
+                ("pairs", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p),
+                ("n_blocks", ctypes.c_int64), ("blk_info", ctypes.c_void_p)]
+
+
 class RowsView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p),
                 ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("entry_mask", ctypes.c_void_p),
@@ -119,6 +125,10 @@ SIGNATURES = {
     "tm_assemble_p1_incidence": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                                 ctypes.POINTER(RowsView), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_block_rows_pairs": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64] + [ctypes.c_void_p] * 8),
+    "tm_assemble_neohookean_rows": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                                   ctypes.c_void_p, ctypes.POINTER(BlockRowsView)]
+                                    + [ctypes.c_void_p] * 6),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 

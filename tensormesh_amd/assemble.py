@@ -334,6 +334,20 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
                             None, float(young), float(poisson),
                             nat.TM_FLAG_MFMA if (getattr(self, "use_mfma", False) and dtype == torch.float32) else 0)
+        use_rows = getattr(self, "nh_path", "slots") == "rows"
+        brv = self._block_rows(pat, conn, dev, D) if use_rows else None
+        if brv is not None:
+            tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None
+            rslot = torch.empty(E * B * D, dtype=dtype, device=dev) if want_residual else None
+            tan = torch.empty(pat.nnz * D * D, dtype=dtype, device=dev) if want_tangent else None
+            res = torch.empty(self.n_points * D, dtype=dtype, device=dev) if want_residual else None
+            p = lambda t: None if t is None else t.data_ptr()
+            nat.check(L.tm_assemble_neohookean_rows(ctypes.byref(mesh), ctypes.byref(form), u.data_ptr(),
+                                                    ctypes.byref(brv), pat.inc_inv().data_ptr(), p(tslot), p(tan),
+                                                    p(rslot), p(res), nat.stream_of(dev)),
+                      "tm_assemble_neohookean_rows")
+            K = self._wrap_output(tan.reshape(pat.nnz, D, D), dev) if want_tangent else None
+            return res, K
         pv = pat.view(dtype, D * D)
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
         tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None
@@ -346,6 +360,45 @@ class ElementAssembler(_AssemblerBase):
                                            nat.stream_of(dev)), "tm_assemble_neohookean")
         K = self._wrap_output(tan.reshape(pat.nnz, D, D), dev) if want_tangent else None
         return res, K
+
+    def _block_rows(self, pat, conn: torch.Tensor, dev, D: int):
+        """Block-row plan and per-contribution pairs for the block-row
+        reduction (once per pattern), or None when it cannot serve this mesh
+        (a repeated vertex, or a block whose pairs overflow 16 bits)."""
+        def make():
+            from .pattern import repeated_vertex
+            if pat.inc is None or repeated_vertex(conn):
+                return None
+            E, B = conn.shape
+            rp = pat.row_ptr.cpu().numpy()
+            ip = pat.inc_ptr.cpu().numpy().astype(np.int64)
+            # (entry, component) accumulators per workgroup: 1024 (one pass of the
+            # block-row reduction); residual threads: 256 // D rows
+            bp = nat.reduce_plan(rp, 1024 // (D * D), max_segments=256 // D)
+            t_first = ip[bp[:-1]]
+            if int((ip[bp[1:]] - t_first).max(initial=0)) * B > 0xffff:
+                return None
+            info = np.zeros((bp.size, 4), np.int32)
+            info[:, 0] = bp
+            info[:, 1] = ip[bp]
+            info[:, 2] = rp[bp]
+            info[:, 3] = pat.seg_ptr.cpu().numpy()[rp[bp]]
+            row_t0 = np.repeat(t_first, np.diff(bp)).astype(np.int32)
+            info_d = torch.from_numpy(info).to(dev)
+            row_t0_d = torch.from_numpy(row_t0).to(dev)
+            pairs = torch.empty(E * B * B, dtype=torch.int16, device=dev)
+            flag = torch.zeros(1, dtype=torch.int32, device=dev)
+            nat.check(nat.lib().tm_block_rows_pairs(B, self.n_points, pat.row_ptr.data_ptr(), pat.seg_ptr.data_ptr(),
+                                                    pat.seg_slot.data_ptr(), pat.inc_inv().data_ptr(),
+                                                    row_t0_d.data_ptr(), pairs.data_ptr(), flag.data_ptr(),
+                                                    nat.stream_of(dev)), "tm_block_rows_pairs")
+            if int(flag.item()) != 0:
+                return None
+            view = nat.BlockRowsView(pat.nnz, pat.row_ptr.data_ptr(), pat.seg_ptr.data_ptr(), pairs.data_ptr(),
+                                     pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), bp.size - 1, info_d.data_ptr())
+            view._keep = (pairs, info_d)
+            return view
+        return self._cached("nh_rows", dev, make)
 
     def _native_bilinear(self, match, points, coeff) -> SparseMatrix:
         return self._wrap_output(self._native_values(match, points, coeff), points.device)

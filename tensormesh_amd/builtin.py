@@ -63,10 +63,15 @@ class LinearElasticityElementAssembler(ElementAssembler):
 class NeoHookeanModel(ElementAssembler):
     """psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2 (builtin.py:219-333).
 
-    ``use_mfma = True`` contracts the float32 tangent on the matrix cores
+    ``nh_path = "slots"`` (default) writes every element's block rows in
+    element order and sums them with the segment reduction
+    (tm_assemble_neohookean); ``"rows"`` writes them in node-incidence order
+    and sums them by block rows (tm_assemble_neohookean_rows).
+    ``use_mfma = True`` (float32) contracts the tangent on the matrix cores
     (v_mfma_f32_32x32x2f32) instead of the closed-form VALU loop."""
 
     use_mfma: bool = False
+    nh_path: str = "slots"
 
     def __post_init__(self, E=1.0, nu=0.3):
         self.E = E

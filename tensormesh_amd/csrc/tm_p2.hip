@@ -193,12 +193,18 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 // residual slots [e][i][a] summed into [N][D] by the node incidence reduction.
 typedef float tm_floatx16 __attribute__((ext_vector_type(16)));
 
-template <typename T, int D, int B, bool MFMA>
+// MQ: largest number of quadrature points the LDS arrays hold.  The tangent
+// of each element is staged in LDS and written as whole slot rows (B blocks of
+// D x D values each) with vector stores; row (e, i) goes to e*B + i, or with
+// inc_inv to its position inc_inv[e*B + i] in the node-sorted incidence list.
+template <typename T, int D, int B, bool MFMA, int MQ>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem,
-                      T* __restrict__ tslot, T* __restrict__ rslot) {
-  __shared__ double s_tab[kMaxTab];
+                      T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv) {
+  constexpr int kMaxQ = MQ;
+  constexpr int kRowVals = B * D * D;  // values of one slot row
+  __shared__ double s_tab[MQ * (1 + B + D * B)];
   __shared__ double s_x[kWavesPerBlock][B * D];
   __shared__ double s_u[kWavesPerBlock][B * D];
   __shared__ double s_inv[kWavesPerBlock][kMaxQ][D * D];
@@ -209,6 +215,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   __shared__ double s_c2[kWavesPerBlock][kMaxQ];
   __shared__ double s_Fi[kWavesPerBlock][kMaxQ][D * D];
   __shared__ int s_v[kWavesPerBlock][B];
+  __shared__ T s_out[kWavesPerBlock][B * kRowVals];
 
   const int lane = threadIdx.x % kWave;
   const int wv = threadIdx.x / kWave;
@@ -336,8 +343,8 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     }
   }
   __syncthreads();
-  if (!active) return;
-  if (rslot) {
+  // (no early return for inactive waves: the block meets once more below)
+  if (rslot && active) {
     for (int t = lane; t < B * D; t += kWave) {
       const int i = t / D, a = t % D;
       double acc = 0.0;
@@ -348,7 +355,9 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
         for (int A = 0; A < D; ++A) s += s_P[wv][q][a * D + A] * g[A];
         acc += s * s_jxw[wv][q];
       }
-      rslot[e * (B * D) + t] = (T)acc;
+      // element order: row (e, i) at e*B + i; incidence order: at inc_inv[e*B + i]
+      const int64_t row = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
+      rslot[row * D + a] = (T)acc;
     }
   }
   if constexpr (MFMA) {
@@ -385,26 +394,23 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
       }
-      T* out = tslot + e * (int64_t)(B * B * DD);
       const int n = m;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = (r / 4) * 8 + kh * 4 + (r % 4);
-        if (row < M && n < M) {
+        if (row < M && n < M && active) {
           const int ii = row / D, aa = row % D, jj = n / D, bb = n % D;
-          out[(ii * B + jj) * DD + aa * D + bb] = (T)acc[r];
+          s_out[wv][(ii * B + jj) * DD + aa * D + bb] = (T)acc[r];
         }
       }
     }
-    return;
-  }
-  if (tslot) {
+  } else if (tslot && active) {
     // one lane per node pair (i, j): the whole D x D block from one read of
     // g_i, g_j, h_i, h_j per quadrature point
     constexpr int DD = D * D;
-    T* out = tslot + e * (int64_t)(B * B * DD);
     for (int ij = lane; ij < B * B; ij += kWave) {
       const int i = ij / B, j = ij % B;
+      T* out = &s_out[wv][(i * B + j) * DD];
       double acc[DD];
 #pragma unroll
       for (int k = 0; k < DD; ++k) acc[k] = 0.0;
@@ -434,14 +440,30 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
             acc[a * D + b] += lw * vhi[a] * vhj[b] - cw * vhi[b] * vhj[a] + (a == b ? mw : 0.0);
       }
 #pragma unroll
-      for (int k = 0; k < DD; ++k) out[ij * DD + k] = (T)acc[k];
+      for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
+    }
+  }
+  __syncthreads();
+  if (tslot && active) {
+    // whole slot rows with vector stores: 16 bytes when a row's size allows,
+    // else 8 (slot buffers are 256-byte aligned)
+    constexpr int kRowBytes = kRowVals * (int)sizeof(T);
+    constexpr int kVec = kRowBytes % 16 == 0 ? 16 : 8;
+    static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
+    constexpr int kPerRow = kRowBytes / kVec;
+    typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
+    for (int v = lane; v < B * kPerRow; v += kWave) {
+      const int i = v / kPerRow, o = v % kPerRow;
+      const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
+      const vec_t x = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
+      reinterpret_cast<vec_t*>(tslot + srow * kRowVals)[o] = x;
     }
   }
 }
 
 template <typename T, int D, int B>
 int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
-                 hipStream_t stream) {
+                 const int32_t* inc_inv, hipStream_t stream) {
   const int Q = f->n_quad;
   if (Q <= 0 || Q > kMaxQ) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
@@ -450,20 +472,24 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   const double mu = f->young / (2.0 * (1.0 + f->poisson));
   const double lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
   const bool mfma = (f->flags & TM_FLAG_MFMA) != 0;
+  if (mfma && !(std::is_same<T, float>::value && B * D <= 32))
+    return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
+  if (mfma && (Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
+  const dim3 g((unsigned)grid), blk(kWave * kWavesPerBlock);
+#define TM_HYPER(MF, MQ)                                                                                   \
+  hyper_wave_kernel<T, D, B, MF, MQ><<<g, blk, 0, stream>>>(m->conn, (const T*)m->points, (const T*)disp, \
+                                                            f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot,  \
+                                                            (T*)rslot, inc_inv)
   if constexpr (std::is_same<T, float>::value && B * D <= 32) {
     if (mfma) {
-      if ((Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
-      hyper_wave_kernel<T, D, B, true><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
-          m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+      if (Q <= 4) TM_HYPER(true, 4); else TM_HYPER(true, 16);
     } else {
-      hyper_wave_kernel<T, D, B, false><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
-          m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+      if (Q <= 4) TM_HYPER(false, 4); else TM_HYPER(false, 16);
     }
   } else {
-    if (mfma) return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
-    hyper_wave_kernel<T, D, B, false><<<grid, kWave * kWavesPerBlock, 0, stream>>>(
-        m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, (T*)rslot);
+    if (Q <= 4) TM_HYPER(false, 4); else TM_HYPER(false, 16);
   }
+#undef TM_HYPER
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     tmh::set_error("neo-hookean kernel launch: %s", hipGetErrorString(err));
@@ -474,12 +500,13 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
 
 template <typename T>
 int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
+                   const int32_t* inc_inv,
                    hipStream_t s) {
   switch (m->cell) {
-    case TM_TRI3: return launch_hyper<T, 2, 3>(m, f, disp, tslot, rslot, s);
-    case TM_TET4: return launch_hyper<T, 3, 4>(m, f, disp, tslot, rslot, s);
-    case TM_TRI6: return launch_hyper<T, 2, 6>(m, f, disp, tslot, rslot, s);
-    case TM_TET10: return launch_hyper<T, 3, 10>(m, f, disp, tslot, rslot, s);
+    case TM_TRI3: return launch_hyper<T, 2, 3>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_TET4: return launch_hyper<T, 3, 4>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_TRI6: return launch_hyper<T, 2, 6>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_TET10: return launch_hyper<T, 3, 10>(m, f, disp, tslot, rslot, inc_inv, s);
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }
@@ -543,6 +570,12 @@ int dispatch_cell(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, h
 
 }  // namespace
 
+int tm_hyper_element_rows(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp, void* tslot,
+                          void* rslot, const int32_t* inc_inv, hipStream_t stream) {
+  return mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, tslot, rslot, inc_inv, stream)
+                               : dispatch_hyper<float>(mesh, form, disp, tslot, rslot, inc_inv, stream);
+}
+
 int tm_p2_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
                            hipStream_t stream, int* hh_out) {
   if (mesh->dtype == TM_F64) return dispatch_cell<double>(mesh, form, slot_ws, stream, hh_out);
@@ -562,8 +595,8 @@ extern "C" int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_de
   if (residual && (!inc || !rslot_ws)) return tmh::fail(TM_ERR_ARG, "residual needs incidence and a workspace");
   void* ts = tangent ? tslot_ws : nullptr;
   void* rs = residual ? rslot_ws : nullptr;
-  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, stream)
-                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, stream);
+  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, nullptr, stream)
+                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, nullptr, stream);
   if (st) return st;
   if (tangent) {
     st = tm_segmented_reduce(mesh->dtype, pat->nnz, (int32_t)(D * D), pat->seg_ptr, pat->seg_slot, pat->n_blocks,

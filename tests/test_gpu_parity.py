@@ -194,8 +194,9 @@ def test_cfg2_fp32_error_bound_against_fp64_oracle(path):
 
 
 @pytest.mark.parametrize("etype", ["tetra", "tetra10", "triangle6"])
-def test_neohookean_residual_tangent_against_oracle_autograd(etype):
-    say(f"case neo-hookean {etype}")
+@pytest.mark.parametrize("nh_path", ["rows", "slots"])
+def test_neohookean_residual_tangent_against_oracle_autograd(etype, nh_path):
+    say(f"case neo-hookean {etype} {nh_path}")
     from tensormesh_amd.mesh import unit_square_tri
     order = 2 if etype in ("tetra10", "triangle6") else 1
     m = unit_square_tri(3, 0.1, 2, order=order) if etype.startswith("tri") else unit_cube_tet(2, 0.1, 2, order=order)
@@ -214,6 +215,7 @@ def test_neohookean_residual_tangent_against_oracle_autograd(etype):
     res = torch.zeros(m.n_points, D, dtype=torch.float64).index_add(0, m.elements().reshape(-1),
                                                                      res_e.reshape(-1, D)).reshape(-1)
     model = tm.NeoHookeanModel.from_mesh(m.to(DEV), quadrature_order=2, E=1.0, nu=0.3)
+    model.nh_path = nh_path
     R, K = model.residual_and_tangent(u.to(DEV))
     ok, err = close64(R, res)
     assert ok, err
@@ -247,3 +249,29 @@ def test_neohookean_fp32_tangent_against_fp64(mfma):
     assert normwise < 1e-6 and maxrel < 1e-6
     dr = (R32.double() - R64).norm() / R64.norm()
     assert dr.item() < 1e-5
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_neohookean_rows_path_matches_slots_path(dtype):
+    """The fused block-row path against the slot-buffer path on a 4^3 tetra10
+    mesh (many block rows per workgroup, chunked incidence lists)."""
+    say(f"case neo-hookean rows vs slots {dtype}")
+    m = unit_cube_tet(4, 0.1, 7, order=2)
+    P = m.points
+    u = 0.05 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, 2], 0.5 * P[:, 0] ** 2], 1)
+    md = tm.Mesh(P.to(dtype), {"tetra10": m.elements()}).to(DEV)
+    out = {}
+    for path in ("rows", "slots"):
+        model = tm.NeoHookeanModel.from_mesh(md, quadrature_order=2, E=1.0, nu=0.3)
+        model.nh_path = path
+        out[path] = model.residual_and_tangent(u.to(dtype).to(DEV))
+    (Rr, Kr), (Rs, Ks) = out["rows"], out["slots"]
+    assert torch.equal(Kr.row, Ks.row) and torch.equal(Kr.col, Ks.col)
+    tol = 1e-11 if dtype == torch.float64 else 2e-5
+    scale = Ks.values.abs().max()
+    assert ((Kr.values - Ks.values).abs() <= tol * Ks.values.abs() + tol * 1e-2 * scale).all()
+    assert ((Rr - Rs).abs() <= tol * Rs.abs().max()).all()
+    # reproducible bit for bit
+    model = tm.NeoHookeanModel.from_mesh(md, quadrature_order=2, E=1.0, nu=0.3)
+    R2, K2 = model.residual_and_tangent(u.to(dtype).to(DEV))
+    assert torch.equal(K2.values, Kr.values) and torch.equal(R2, Rr)

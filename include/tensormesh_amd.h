@@ -145,6 +145,15 @@ typedef struct tm_block_rows_view {
                                 than 1024 / D^2 entries take several passes */
 } tm_block_rows_view;
 
+/* Selected boundary facets (FacetAssembler): facet i is local facet local[i]
+ * of element elem[i] (triangle: edges (1,2), (0,2), (0,1); tetrahedron: faces
+ * (1,2,3), (0,2,3), (0,1,3), (0,1,2)), in the reference's selection order. */
+typedef struct tm_facet_view {
+  int64_t n_sel;
+  const int32_t* elem;       /* [n_sel] */
+  const int32_t* local;      /* [n_sel] */
+} tm_facet_view;
+
 /* Block-diagonal stack of SPD CSR systems for the batched Jacobi-PCG.
  * Workgroups are system-aligned: workgroup w covers rows
  * [blk_row0[w], blk_row1[w]) of system blk_sys[w] (at most 256 rows);
@@ -361,6 +370,18 @@ int tm_condense(int32_t dtype, int64_t n_inner, int64_t n_kept, const void* valu
                 const int32_t* inner_idx, const int32_t* ou_rp, const int32_t* ou_idx, const int32_t* ou_col,
                 const int32_t* inner_dof, const void* rhs, const void* u_o, void* kii, void* f_i,
                 tm_stream_t stream);
+
+/* Boundary-facet load vector (FacetAssembler.__call__, facet_assembler.py:123-270)
+ * for the integrands f * v (data: nodal f [n_points * n_comp]) or c * v
+ * (cval_dev: n_comp device doubles; data NULL).  facet_tab_dev: float64
+ * w [Qf] | phi [nF][Qf][B] | dphi [nF][Qf][D][B] | dgamma [nF][D-1][D] (the
+ * cell basis at the facet points mapped into the reference cell, and the
+ * reference tangents of each facet).  inc: the facet-node incidence lists of
+ * the selected facets' cell nodes (tm_incidence_build on elem's rows, entry
+ * facet*B + b), slot_ws [n_sel*B*n_comp], out [n_points*n_comp].  n_comp <= 4. */
+int tm_assemble_facet_linear(const tm_mesh_view* mesh, const tm_facet_view* facets, const double* facet_tab_dev,
+                             int32_t n_quad, int32_t n_comp, const void* data, const double* cval_dev,
+                             const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream);
 
 /* ---------------------------------------------------------------------
  * Sparse solve.  The reference inherits solve / @ from torch_sla (not

@@ -38,6 +38,15 @@ reduce_project              brute-force loop (NaiveProjector semantics of
                             tests/assemble/test_projector.py)
 block_coo                   tests/sparse/test_block_coo.py naive example
 node_assemble (load)        sum b = integral of f_h; closed-form P1 load
+facet loads                 Gauss-Legendre exactness on monomials; facet
+                            selection against plain loops; unit square:
+                            sum b = 4 + 4 sqrt(2) h (f = 1), 2 + 2 sqrt(2) h
+                            (f = x), the two corner diagonals included as the
+                            reference selects them; one edge: h per inner
+                            node, h/2 at its ends; P2 Simpson weights; P2
+                            face: vertex functions integrate to 0; cube: sum
+                            of selected triangle areas by cross products;
+                            face x = 0: area 1, integral of y = 1/2
 condense / recover          tests/operator/test_condenser.py 4x4 system
 adjoint gradients           central finite differences; dense solve autograd
 ==========================  =================================================

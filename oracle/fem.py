@@ -272,6 +272,114 @@ def assemble_linear(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder
     return torch.zeros(points.shape[0], dtype=points.dtype).index_add(0, conn.reshape(-1), elem.reshape(-1))
 
 
+# ----------------------------------------------------------------- facets ---
+# Gauss-Legendre n-point rules on [-1, 1] (tensormesh/element/quadrature.py:36-54)
+GAUSS_LEGENDRE = {
+    1: ([2.0], [0.0]),
+    2: ([1.0, 1.0], [-0.57735026918962576451, 0.57735026918962576451]),
+    3: ([0.55555555555555555556, 0.88888888888888888889, 0.55555555555555555556],
+        [-0.77459666924148337704, 0.0, 0.77459666924148337704]),
+    4: ([0.34785484513745385737, 0.65214515486254614263, 0.65214515486254614263, 0.34785484513745385737],
+        [-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526]),
+}
+# facet vertices per cell (element.py:1684-1685 edges of the triangle, :1950 faces of the tetrahedron)
+FACET_VERTS = {"tri": [(1, 2), (0, 2), (0, 1)], "tet": [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]}
+
+
+def facet_quadrature(etype: str, order: int):
+    """Reference-facet rule in facet coordinates: line facets use the n-point
+    Gauss-Legendre rule mapped to [0, 1] with halved weights
+    (quadrature.py:13-66, n = order); triangle facets the triangle rule
+    (quadrature.py:362-376)."""
+    fam = FAMILY[etype][0]
+    if fam == "tri":
+        w, p = GAUSS_LEGENDRE[order]
+        return (torch.tensor(w, dtype=torch.float64) / 2,
+                (torch.tensor(p, dtype=torch.float64)[:, None] + 1) / 2)
+    return quadrature("triangle", order)
+
+
+def facet_local_nodes(etype: str):
+    """Local nodes of each facet (vertices, then P2 edge midpoints on it)."""
+    fam, dim, order = FAMILY[etype]
+    edges = TRI_EDGE if fam == "tri" else TET_EDGE
+    out = []
+    for fv in FACET_VERTS[fam]:
+        extra = [dim + 1 + m for m, (a, b) in enumerate(edges) if a in fv and b in fv] if order == 2 else []
+        out.append(list(fv) + extra)
+    return np.array(out)
+
+
+def facet_select(conn: np.ndarray, boundary_mask: np.ndarray, etype: str):
+    """``FacetAssembler.from_elements`` for cells with one facet type
+    (facet_assembler.py:488-527): the boundary elements are those with any
+    flagged node; a facet of one of them is selected iff all its nodes are
+    flagged (also an interior facet whose nodes all lie on the boundary);
+    selected facets in row-major order of the [boundary element, facet] mask.
+    Returns (element index, local facet index) of each selected facet."""
+    conn = np.asarray(conn)
+    is_be = boundary_mask[conn].any(-1)
+    be_idx = np.nonzero(is_be)[0]
+    faces = conn[be_idx][:, facet_local_nodes(etype)]        # [n_be, n_facet, n_node_per_facet]
+    m = boundary_mask[faces].all(-1)
+    e_sel, f_sel = np.nonzero(m)
+    return be_idx[e_sel], f_sel
+
+
+def facet_geometry(points: torch.Tensor, conn: torch.Tensor, elem: np.ndarray, fac: np.ndarray, etype: str,
+                   order: int):
+    """Cell basis values phi[F, Qf, B], physical gradients grad[F, Qf, B, D]
+    and FxW[F, Qf] at the facet quadrature points of the selected facets.
+
+    Restates Transformation.facet_shape_val / facet_shape_grad / FxW
+    (transformation.py:516-713, 985-1097): the reference facet is mapped
+    affinely onto facet f of the reference cell (element.py:972-1067), the
+    cell Jacobian J[i][j] = sum_b x_b[j] dphi_b/dxi_i is evaluated there,
+    the facet tangents are F = (d gamma / d r) J and FxW = w |sqrt(det(F F^T))|."""
+    fam, dim, _ = FAMILY[etype]
+    w, r = facet_quadrature(etype, order)
+    verts = lagrange_nodes(etype)[:dim + 1]
+    tabs = []
+    for fv in FACET_VERTS[fam]:
+        a = verts[fv[0]]
+        dg = np.stack([verts[fv[k + 1]] - a for k in range(dim - 1)])   # [D-1, D]
+        xi = a[None] + r.numpy() @ dg
+        tabs.append((shape_values(etype, xi), shape_ref_grads(etype, xi), dg))
+    phi = torch.tensor(np.stack([t[0] for t in tabs]))[torch.from_numpy(fac)]     # [F, Qf, B]
+    dphi = torch.tensor(np.stack([t[1] for t in tabs]))[torch.from_numpy(fac)]    # [F, Qf, D, B]
+    dg = torch.tensor(np.stack([t[2] for t in tabs]))[torch.from_numpy(fac)]      # [F, D-1, D]
+    x = points[conn[torch.from_numpy(elem)]]                                       # [F, B, D]
+    J = torch.einsum("fbj,fqib->fqij", x, dphi)
+    T = torch.einsum("fki,fqij->fqkj", dg, J)
+    detF = torch.sqrt(torch.linalg.det(T @ T.transpose(-1, -2)))
+    fxw = w[None, :] * detF.abs()
+    grad = torch.einsum("fqib,fqji->fqbj", dphi, torch.inverse(J))
+    return phi, grad, fxw
+
+
+def assemble_facet_linear(points: torch.Tensor, conn: torch.Tensor, boundary_mask: torch.Tensor, etype: str,
+                          order: int, f_nodal: Optional[torch.Tensor] = None,
+                          const: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """``FacetAssembler.__call__`` (facet_assembler.py:123-270) with the
+    integrand ``f * v`` (f nodal, [N] or [N, H], interpolated at the facet
+    points) or ``c * v`` (c a constant scalar or [H] vector): the integral
+    ``einsum('fqb...,fq->fb...')`` against FxW, scattered with ``index_add_``
+    over the selected facets' cell nodes (facet_assembler.py:520-525).
+    Returns the flattened [N] or [N*H] vector."""
+    elem, fac = facet_select(conn.numpy(), boundary_mask.numpy(), etype)
+    phi, _, fxw = facet_geometry(points, conn, elem, fac, etype, order)
+    dofs = conn[torch.from_numpy(elem)]                                             # [F, B]
+    if f_nodal is not None:
+        fq = torch.einsum("fb...,fqb->fq...", f_nodal[dofs], phi)                   # [F, Qf, (H)]
+    else:
+        fq = const.to(torch.float64).expand(phi.shape[:2] + tuple(const.shape))
+    integrand = phi[..., None] * fq[:, :, None, :] if fq.dim() == 3 else phi * fq[:, :, None]
+    integral = torch.einsum("fqb...,fq->fb...", integrand, fxw)
+    n = points.shape[0]
+    out = torch.zeros((n,) + tuple(integral.shape[2:]), dtype=torch.float64)
+    return out.index_add(0, dofs.reshape(-1), integral.reshape((-1,) + tuple(integral.shape[2:]))).reshape(-1)
+
+
 # -------------------------------------------------------------- condenser ---
 def condense(values, row, col, n_dof, mask, dvalue=None, rhs=None):
     """``Condenser._compute_layout`` + ``__call__`` (tensormesh/operator/condense.py:182-296).

@@ -10,13 +10,14 @@ from .builtin import (DiffusionElementAssembler, LaplaceElementAssembler, Linear
                       MassElementAssembler, NeoHookeanModel, SourceNodeAssembler, const_node_assembler,
                       func_node_assembler)
 from .condense import Condenser
+from .facet import FacetAssembler
 from .mesh import Mesh
 from .sparse import SparseMatrix
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "ElementAssembler", "NodeAssembler", "Condenser", "SparseMatrix", "Mesh",
+    "ElementAssembler", "NodeAssembler", "FacetAssembler", "Condenser", "SparseMatrix", "Mesh",
     "LaplaceElementAssembler", "DiffusionElementAssembler", "MassElementAssembler",
     "LinearElasticityElementAssembler", "NeoHookeanModel", "SourceNodeAssembler",
     "const_node_assembler", "func_node_assembler",

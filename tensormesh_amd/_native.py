@@ -55,6 +55,10 @@ class BlockRowsView(ctypes.Structure):
                 ("n_blocks", ctypes.c_int64), ("blk_info", ctypes.c_void_p)]
 
 
+class FacetView(ctypes.Structure):
+    _fields_ = [("n_sel", ctypes.c_int64), ("elem", ctypes.c_void_p), ("local", ctypes.c_void_p)]
+
+
 class RowsView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p),
                 ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("entry_mask", ctypes.c_void_p),
@@ -129,6 +133,10 @@ SIGNATURES = {
     "tm_assemble_neohookean_rows": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                                    ctypes.c_void_p, ctypes.POINTER(BlockRowsView)]
                                     + [ctypes.c_void_p] * 6),
+    "tm_assemble_facet_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FacetView), ctypes.c_void_p,
+                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.POINTER(IncidenceView), ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 12),
 }
 

@@ -15,6 +15,18 @@
 ``reference_tables`` packs weights, values and reference gradients into the
 flat float64 layout that the HIP kernels read (documented in
 ``include/tensormesh_amd.h``).
+
+Facets (boundary integrals, tensormesh/assemble/facet_assembler.py):
+
+* triangle facets are the edges ``[[1,2],[0,2],[0,1]]`` and tetrahedron
+  facets the faces ``((1,2,3),(0,2,3),(0,1,3),(0,1,2))``
+  (tensormesh/element/element.py:1684-1685, :1949-1950), each with the
+  element's edge-midpoint nodes on it for P2;
+* a facet is the affine image gamma_f(r) = P_a + sum_k r_k (P_{k+1} - P_a) of the
+  reference facet (reference-element vertices P, element.py:972-1067);
+* triangle facets use an n-point Gauss-Legendre rule on [0, 1] with n equal
+  to ``quadrature_order`` (1-7; quadrature.py:13-66), tetrahedron facets the
+  triangle rule of that order (quadrature.py:362-376).
 """
 
 from __future__ import annotations
@@ -64,6 +76,28 @@ TET_QUAD = {
          [0.3994035761667992, 0.1005964238332008, 0.1005964238332008],
          [0.1005964238332008, 0.3994035761667992, 0.1005964238332008],
          [0.1005964238332008, 0.1005964238332008, 0.3994035761667992]]),
+}
+
+# n-point Gauss-Legendre on [-1, 1] (quadrature.py:36-54), mapped to [0, 1] with
+# weights halved (quadrature.py:59-61)
+_GL = {
+    1: ([2.0], [0.0]),
+    2: ([1.0, 1.0], [-0.57735026918962576451, 0.57735026918962576451]),
+    3: ([0.55555555555555555556, 0.88888888888888888889, 0.55555555555555555556],
+        [-0.77459666924148337704, 0.0, 0.77459666924148337704]),
+    4: ([0.34785484513745385737, 0.65214515486254614263, 0.65214515486254614263, 0.34785484513745385737],
+        [-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526]),
+    5: ([0.23692688505618908751, 0.47862867049936646804, 0.56888888888888888889, 0.47862867049936646804,
+         0.23692688505618908751],
+        [-0.9061798459386640, -0.5384693101056831, 0.0, 0.5384693101056831, 0.9061798459386640]),
+    6: ([0.17132449237917034504, 0.36076157304813860757, 0.46791393457269104739, 0.46791393457269104739,
+         0.36076157304813860757, 0.17132449237917034504],
+        [-0.9324695142031521, -0.6612093864662645, -0.2386191860831969, 0.2386191860831969, 0.6612093864662645,
+         0.9324695142031521]),
+    7: ([0.12948496616886969327, 0.27970539148927666790, 0.38183005050511894495, 0.41795918367346938776,
+         0.38183005050511894495, 0.27970539148927666790, 0.12948496616886969327],
+        [-0.9491079123427585, -0.7415311855993945, -0.4058451513773972, 0.0, 0.4058451513773972,
+         0.7415311855993945, 0.9491079123427585]),
 }
 
 ELEMENT_SHAPE = {
@@ -154,3 +188,65 @@ def torch_tables(element_type: str, qorder: int, dtype, device):
     return (torch.tensor(w, dtype=dtype, device=device),
             torch.tensor(shape_values(element_type, p), dtype=dtype, device=device),
             torch.tensor(shape_ref_grads(element_type, p), dtype=dtype, device=device))
+
+
+def line_quadrature(order: int) -> Tuple[np.ndarray, np.ndarray]:
+    """(weights[Q], points[Q, 1]) of the n-point Gauss-Legendre rule on [0, 1]."""
+    if order not in _GL:
+        raise NotImplementedError(f"facet quadrature_order={order} on lines: supported orders are 1-7")
+    w, p = _GL[order]
+    return np.asarray(w, np.float64) / 2.0, ((np.asarray(p, np.float64) + 1.0) / 2.0)[:, None]
+
+
+_REF_VERTS = {"tri": np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]),
+              "tet": np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])}
+_FACET_VERTS = {"tri": ((1, 2), (0, 2), (0, 1)), "tet": ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))}
+_EDGES = {"tri": ((1, 2), (0, 2), (0, 1)), "tet": ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))}
+
+
+def facet_nodes(element_type: str) -> np.ndarray:
+    """[n_facet, n_node_per_facet] local nodes of each facet: its vertices,
+    then (P2) the midpoint nodes of the element edges lying on it."""
+    dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    out = []
+    for fv in _FACET_VERTS[fam]:
+        nodes = list(fv)
+        if order == 2:
+            nodes += [dim + 1 + m for m, (a, b) in enumerate(_EDGES[fam]) if a in fv and b in fv]
+        out.append(nodes)
+    return np.asarray(out, np.int64)
+
+
+@functools.lru_cache(maxsize=None)
+def facet_tables(element_type: str, qorder: int):
+    """Facet quadrature data in cell reference coordinates:
+    (w[Qf], phi[nF, Qf, B], dphi[nF, Qf, D, B], dgamma[nF, D-1, D]) with
+    dgamma[f][k] = d gamma_f / d r_k (the facet's reference tangents)."""
+    dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    if fam == "tri":
+        w, r = line_quadrature(qorder)
+    else:
+        w, r = quadrature("triangle", qorder)
+    P = _REF_VERTS[fam]
+    phi, dphi, dgam = [], [], []
+    for fv in _FACET_VERTS[fam]:
+        a = P[fv[0]]
+        tang = np.stack([P[fv[k + 1]] - a for k in range(dim - 1)])   # [D-1, D]
+        xi = a[None, :] + r @ tang                                     # [Qf, D]
+        phi.append(shape_values(element_type, xi))
+        dphi.append(shape_ref_grads(element_type, xi))
+        dgam.append(tang)
+    return (np.asarray(w, np.float64), np.stack(phi), np.stack(dphi), np.stack(dgam))
+
+
+@functools.lru_cache(maxsize=None)
+def facet_reference_tables(element_type: str, qorder: int) -> np.ndarray:
+    """Flat float64 facet table read by the facet kernel::
+
+        [0, Qf)                                w[q]
+        then nF*Qf*B                           phi[f][q][b]
+        then nF*Qf*D*B                         dphi[f][q][d][b]
+        then nF*(D-1)*D                        dgamma[f][k][d]
+    """
+    w, phi, dphi, dgam = facet_tables(element_type, qorder)
+    return np.concatenate([w, phi.ravel(), dphi.ravel(), dgam.ravel()]).astype(np.float64)

@@ -1,0 +1,283 @@
+"""Boundary-facet integrals: ``FacetAssembler``.
+
+Mirrors tensormesh/assemble/facet_assembler.py:17-567 for meshes of one
+simplex type (triangle, triangle6, tetra, tetra10):
+
+* ``from_mesh(mesh, boundary_mask=None, quadrature_order=2, project='reduce')``
+  / ``from_elements`` / ``from_assembler`` build the boundary topology: the
+  boundary elements are those with any flagged node, and a facet of one of
+  them is selected iff *all* its nodes are flagged -- as in the reference,
+  this also selects an interior facet whose nodes all lie on the boundary
+  (facet_assembler.py:488-527); selected facets keep the reference's order
+  (boundary elements in mesh order, then local facet index);
+* ``__call__(points=None, func=None, point_data=None)`` integrates the
+  per-point integrand ``forward`` over the selected facets and scatters it
+  onto the nodes: ``u``/``v`` are the cell basis values at the facet point
+  (``[B]``), ``gradu``/``gradv`` their physical gradients (``[B, D]``), nodal
+  fields of ``point_data`` (``x`` included) are interpolated there and
+  ``"grad" + key`` gives their gradients.  Returns the flattened ``[N]`` or
+  ``[N*H]`` vector (facet_assembler.py:123-270).
+
+Geometry follows Transformation.facet_shape_val / facet_shape_grad / FxW
+(transformation.py:516-713, 985-1097): each reference facet is mapped
+affinely onto the reference cell's facet, the cell Jacobian is evaluated
+there, and FxW = w sqrt(det(F F^T)) with F the physical facet tangents.
+
+The integrands ``f * v`` (``f`` a nodal field, scalar or ``[N, H]``) and
+``c * v`` (constant scalar or ``[H]`` traction) run on the HIP kernel
+``tm_assemble_facet_linear`` for device tensors; any other integrand runs
+through the torch path below (the reference algorithm, vmapped).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import inspect
+from dataclasses import dataclass
+from typing import Callable, Dict, Optional
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from . import _native as nat
+from .element import ELEMENT_SHAPE, facet_nodes, facet_reference_tables, facet_tables
+
+
+@dataclass
+class _FacetTopology:
+    etype: str
+    conn: torch.Tensor            # [E, B] int64, the whole mesh
+    points: torch.Tensor          # [N, D] default coordinates
+    elem: torch.Tensor            # [F] int64 element of each selected facet
+    local: torch.Tensor           # [F] int64 local facet index
+    quadrature_order: int
+    incidence: tuple              # (inc_ptr, inc) int32 over the selected facets' cell nodes
+
+
+@dataclass
+class FacetMatch:
+    key: Optional[str]            # nodal field multiplying v, or None for a constant
+    const: Optional[torch.Tensor]  # [H] constant (float64) when key is None
+    n_comp: int
+
+
+def _rand(gen, *shape):
+    return torch.rand(tuple(shape), generator=gen, dtype=torch.float64) * 2.0 - 1.0
+
+
+def match_facet(fn: Callable, point_data: Dict[str, torch.Tensor], n_basis: int) -> Optional[FacetMatch]:
+    """Recognise ``f * v`` / ``c * v`` (scalar or vector) by evaluating the
+    integrand on random inputs; None for anything else."""
+    try:
+        names = list(inspect.signature(fn).parameters)
+    except (TypeError, ValueError):
+        return None
+    gen = torch.Generator().manual_seed(8765)
+    try:
+        with torch.no_grad():
+            if names == ["v"]:
+                v = _rand(gen, n_basis)
+                out = torch.as_tensor(fn(v), dtype=torch.float64)
+                if out.shape == (n_basis,):
+                    c = (out / v).mean()
+                    const = c.reshape(1)
+                elif out.dim() == 2 and out.shape[0] == n_basis and 1 <= out.shape[1] <= 4:
+                    const = (out / v[:, None]).mean(0)
+                else:
+                    return None
+                for _ in range(3):
+                    v = _rand(gen, n_basis)
+                    ref = v * const[0] if out.dim() == 1 else v[:, None] * const[None, :]
+                    got = torch.as_tensor(fn(v), dtype=torch.float64)
+                    if got.shape != ref.shape or not torch.allclose(got, ref, rtol=1e-12, atol=1e-14):
+                        return None
+                return FacetMatch(None, const, int(const.numel()))
+            if len(names) == 2 and "v" in names:
+                key = [n for n in names if n != "v"][0]
+                if key not in point_data:
+                    return None
+                data = point_data[key]
+                if data.dim() == 1:
+                    H, fshape = 1, ()
+                elif data.dim() == 2 and 1 <= data.shape[1] <= 4:
+                    H, fshape = int(data.shape[1]), (int(data.shape[1]),)
+                else:
+                    return None
+                for _ in range(3):
+                    v, f = _rand(gen, n_basis), _rand(gen, *fshape)
+                    got = torch.as_tensor(fn(**{"v": v, key: f}), dtype=torch.float64)
+                    ref = f * v if H == 1 and not fshape else v[:, None] * f[None, :]
+                    if got.shape != ref.shape or not torch.allclose(got, ref, rtol=1e-12, atol=1e-14):
+                        return None
+                return FacetMatch(key, None, H)
+    except Exception:
+        return None
+    return None
+
+
+class FacetAssembler(nn.Module):
+    """Boundary-facet linear-form assembler (facet_assembler.py:17-567)."""
+
+    def __init__(self, topology: _FacetTopology, *args, **kwargs):
+        super().__init__()
+        self._topo = topology
+        self.element_types = [topology.etype]
+        self.dimension = ELEMENT_SHAPE[topology.etype][0]
+        self.n_points = int(topology.points.shape[0])
+        self._dev_cache: Dict[str, object] = {}
+        self.__post_init__(*args, **kwargs)
+
+    def __post_init__(self, *args, **kwargs):
+        pass
+
+    def forward(self, *args):
+        raise NotImplementedError("forward is not implemented")
+
+    # ---- construction (facet_assembler.py:393-567) ----
+    @classmethod
+    def from_mesh(cls, mesh, boundary_mask=None, quadrature_order: int = 2, project: str = "reduce",
+                  *args, **kwargs):
+        points = mesh.points
+        elements = mesh.elements()
+        if isinstance(elements, torch.Tensor):
+            elements = {mesh.default_element_type: elements}
+        if boundary_mask is None:
+            boundary_mask = mesh.boundary_mask
+        elif isinstance(boundary_mask, str):
+            boundary_mask = mesh.point_data[boundary_mask]
+        assert boundary_mask.dim() == 1 and boundary_mask.shape[0] == points.shape[0]
+        return cls.from_elements(points, elements, boundary_mask, quadrature_order, points.device, points.dtype,
+                                 project, *args, **kwargs)
+
+    @classmethod
+    def from_elements(cls, points, elements, boundary_mask, quadrature_order: int = 2, device="cpu",
+                      dtype=torch.float32, project: str = "reduce", *args, **kwargs):
+        if project not in ("reduce", "sparse"):
+            raise ValueError(f"project should be either 'reduce' or 'sparse', but got {project}")
+        if len(elements) != 1:
+            raise NotImplementedError("tensormesh_amd assembles one element type per mesh")
+        etype, conn = next(iter(elements.items()))
+        if etype not in ELEMENT_SHAPE:
+            raise NotImplementedError(f"element type {etype} has no facet tables here")
+        c = conn.detach().cpu().long()
+        bm = boundary_mask.detach().cpu().bool()
+        is_be = bm[c].any(-1)
+        be_idx = torch.nonzero(is_be).flatten()
+        faces = c[be_idx][:, torch.from_numpy(facet_nodes(etype))]            # [n_be, n_facet, n_per_facet]
+        sel = bm[faces].all(-1)
+        e_sel, f_sel = torch.nonzero(sel, as_tuple=True)
+        elem = be_idx[e_sel]
+        dofs = np.ascontiguousarray(c[elem].numpy().astype(np.int32))
+        n = int(points.shape[0])
+        F, B = dofs.shape[0], c.shape[1]
+        inc_ptr = np.empty(n + 1, np.int32)
+        inc = np.empty(max(F * B, 1), np.int32)
+        nat.check(nat.lib().tm_incidence_build(dofs.ctypes.data, F, B, n, inc_ptr.ctypes.data, inc.ctypes.data),
+                  "tm_incidence_build")
+        dev = torch.device(device)
+        topo = _FacetTopology(etype, c.to(dev), points.detach().to(device=dev, dtype=dtype), elem.to(dev),
+                              f_sel.to(dev), quadrature_order,
+                              (torch.from_numpy(inc_ptr).to(dev), torch.from_numpy(inc[:F * B]).to(dev)))
+        return cls(topo, *args, **kwargs)
+
+    @classmethod
+    def from_assembler(cls, obj, *args, **kwargs):
+        assert isinstance(obj, FacetAssembler), f"the object {obj} should inheritate from NodeAssembler"
+        return cls(obj._topo, *args, **kwargs)
+
+    # ---- evaluation (facet_assembler.py:123-270) ----
+    def __call__(self, points: Optional[torch.Tensor] = None, func: Optional[Callable] = None,
+                 point_data: Optional[Dict[str, torch.Tensor]] = None) -> torch.Tensor:
+        point_data = {} if point_data is None else dict(point_data)
+        if points is None:
+            points = self._topo.points
+        point_data["x"] = points
+        for key, value in point_data.items():
+            assert value.shape[0] == self.n_points, f"the shape of {key} should be [n_point, ...], but got {value.shape}"
+        fn = self.forward if func is None else func
+        B = ELEMENT_SHAPE[self._topo.etype][2]
+        match = match_facet(fn, point_data, B) if points.is_cuda else None
+        if match is not None:
+            f = None if match.key is None else point_data[match.key]
+            needs_grad = torch.is_grad_enabled() and (points.requires_grad or (f is not None and f.requires_grad))
+            if not needs_grad:
+                return self._native(points, f, match)
+        return self._torch(fn, points, point_data)
+
+    def _geometry(self, points: torch.Tensor):
+        """phi [F, Qf, B], physical gradients [F, Qf, B, D], FxW [F, Qf]."""
+        t = self._topo
+        dev, dt = points.device, points.dtype
+        w, phi, dphi, dg = (torch.as_tensor(a, dtype=dt, device=dev) for a in facet_tables(t.etype,
+                                                                                          t.quadrature_order))
+        loc = t.local.to(dev)
+        phi_f, dphi_f, dg_f = phi[loc], dphi[loc], dg[loc]
+        x = points[t.conn.to(dev)[t.elem.to(dev)]]                                   # [F, B, D]
+        J = torch.einsum("fbj,fqib->fqij", x, dphi_f)
+        T = torch.einsum("fki,fqij->fqkj", dg_f, J)
+        detF = torch.sqrt(torch.linalg.det(T @ T.transpose(-1, -2)))
+        fxw = w[None, :] * detF.abs()
+        grad = torch.einsum("fqib,fqji->fqbj", dphi_f, torch.inverse(J))
+        return phi_f, grad, fxw
+
+    def _torch(self, fn, points, point_data) -> torch.Tensor:
+        t = self._topo
+        dev = points.device
+        phi, grad, fxw = self._geometry(points)
+        dofs = t.conn.to(dev)[t.elem.to(dev)]                                          # [F, B]
+        args = []
+        for key in inspect.signature(fn).parameters:
+            if key in ("u", "v"):
+                args.append(phi)
+            elif key in ("gradu", "gradv"):
+                args.append(grad)
+            elif key in point_data:
+                args.append(torch.einsum("fb...,fqb->fq...", point_data[key][dofs], phi))
+            elif key.startswith("grad") and key[4:] in point_data:
+                args.append(torch.einsum("fb...,fqbd->fq...d", point_data[key[4:]][dofs], grad))
+            else:
+                raise NotImplementedError(f"key {key} is not implemented")
+        integral = torch.vmap(torch.vmap(fn))(*args)                                  # [F, Qf, B, ...]
+        integral = torch.einsum("fqb...,fq->fb...", integral, fxw)
+        out = torch.zeros((self.n_points,) + tuple(integral.shape[2:]), dtype=integral.dtype, device=dev)
+        out = out.index_add(0, dofs.reshape(-1), integral.reshape((-1,) + tuple(integral.shape[2:])))
+        return out.flatten()
+
+    def _native(self, points, f, match: FacetMatch) -> torch.Tensor:
+        nat.require_device(points, "points")
+        t = self._topo
+        dev = points.device
+        points = points.detach().contiguous()
+        dtype = points.dtype
+        H = match.n_comp
+        if f is not None:
+            nat.require_device(f, "point data")
+            f = f.detach().to(dtype).contiguous()
+        cache = self._dev_cache
+        k = f"fac@{dev}"
+        if k not in cache:
+            tab = torch.from_numpy(facet_reference_tables(t.etype, t.quadrature_order)).to(dev)
+            conn32 = t.conn.to(device=dev, dtype=torch.int32).contiguous()
+            elem32 = t.elem.to(device=dev, dtype=torch.int32).contiguous()
+            loc32 = t.local.to(device=dev, dtype=torch.int32).contiguous()
+            inc_ptr, inc = (x.to(dev) for x in t.incidence)
+            cache[k] = (tab, conn32, elem32, loc32, inc_ptr, inc)
+        tab, conn32, elem32, loc32, inc_ptr, inc = cache[k]
+        F = int(elem32.numel())
+        B = int(conn32.shape[1])
+        nq = int(facet_tables(t.etype, t.quadrature_order)[0].shape[0])
+        cval = None if match.const is None else match.const.to(device=dev, dtype=torch.float64).contiguous()
+        slot = torch.empty(max(F * B * H, 1), dtype=dtype, device=dev)
+        out = torch.empty(self.n_points * H, dtype=dtype, device=dev)
+        cell = nat.CELL[t.etype]
+        mesh = nat.MeshView(cell, nat.dtype_code(dtype), conn32.shape[0], points.shape[0], conn32.data_ptr(),
+                            points.data_ptr())
+        fv = nat.FacetView(F, elem32.data_ptr(), loc32.data_ptr())
+        iv = nat.IncidenceView(inc_ptr.data_ptr(), inc.data_ptr(), 0, None)
+        nat.check(nat.lib().tm_assemble_facet_linear(ctypes.byref(mesh), ctypes.byref(fv), tab.data_ptr(), nq, H,
+                                                     None if f is None else f.data_ptr(),
+                                                     None if cval is None else cval.data_ptr(), ctypes.byref(iv),
+                                                     slot.data_ptr(), out.data_ptr(), nat.stream_of(dev)),
+                  "tm_assemble_facet_linear")
+        return out

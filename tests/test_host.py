@@ -221,3 +221,84 @@ def test_row_centric_summation_reproduces_oracle_on_cpu():
     bp = pat.plan(Pattern.ROWS_CAP, node=True, max_rows=Pattern.ROWS_MAX).numpy()
     assert bp[0] == 0 and bp[-1] == m.n_points and np.all(np.diff(bp) > 0)
     assert np.diff(bp).max() <= Pattern.ROWS_MAX
+
+
+# ------------------------------------------------------------------ facets ---
+def _facet_mesh(kind, order=1, mask="all"):
+    from tensormesh_amd.mesh import quadratic_arrays, unit_cube_tet_arrays, unit_square_tri_arrays
+    if kind == "tri":
+        p, c, bm = unit_square_tri_arrays(5, 0.2, 1)
+        et = "triangle" if order == 1 else "triangle6"
+    else:
+        p, c, bm = unit_cube_tet_arrays(2, 0.1, 1)
+        et = "tetra" if order == 1 else "tetra10"
+    if order == 2:
+        p, c, bm = quadratic_arrays(p, c, bm)
+    if mask == "face":
+        bm = p[:, 0] <= 1e-12
+    m = tm.Mesh(torch.from_numpy(p), {et: torch.from_numpy(c)}, torch.from_numpy(bm))
+    return m, et
+
+
+@pytest.mark.parametrize("kind,order", [("tri", 1), ("tri", 2), ("tet", 1), ("tet", 2)])
+def test_facet_topology_and_torch_path_match_oracle(kind, order):
+    from oracle import fem
+    m, et = _facet_mesh(kind, order)
+    c = m.elements()
+    elem, fac = fem.facet_select(c.numpy(), m.boundary_mask.numpy(), et)
+
+    class Load(tm.FacetAssembler):
+        def forward(self, v, f):
+            return f * v
+
+    A = Load.from_mesh(m, quadrature_order=2)
+    assert A._topo.elem.tolist() == elem.tolist() and A._topo.local.tolist() == fac.tolist()
+    f = torch.sin(3 * m.points[:, 0]) + m.points[:, 1]
+    b = A(point_data={"f": f})
+    ref = fem.assemble_facet_linear(m.points, c, m.boundary_mask, et, 2, f_nodal=f)
+    assert torch.allclose(b, ref, rtol=1e-12, atol=1e-14)
+
+    class Traction(tm.FacetAssembler):
+        def forward(self, v):
+            t = torch.tensor([0.25, -1.0, 0.5][:m.points.shape[1]], dtype=v.dtype)
+            return v[:, None] * t[None, :]
+
+    bt = Traction.from_assembler(A)()
+    tvec = torch.tensor([0.25, -1.0, 0.5][:m.points.shape[1]], dtype=torch.float64)
+    reft = fem.assemble_facet_linear(m.points, c, m.boundary_mask, et, 2, const=tvec)
+    assert torch.allclose(bt, reft, rtol=1e-12, atol=1e-14)
+
+
+def test_facet_torch_path_custom_integrand_uses_gradients():
+    """forward(v, gradv, x) = (|x|^2 + gradv[:, 0]) v-shaped: checked against
+    the oracle's facet geometry (values, gradients, FxW) directly."""
+    from oracle import fem
+    m, et = _facet_mesh("tri", 2)
+
+    class Custom(tm.FacetAssembler):
+        def forward(self, v, gradv, x):
+            return (x * x).sum() * v + gradv[:, 0]
+
+    b = Custom.from_mesh(m, quadrature_order=3)()
+    elem, fac = fem.facet_select(m.elements().numpy(), m.boundary_mask.numpy(), et)
+    phi, grad, fxw = fem.facet_geometry(m.points, m.elements(), elem, fac, et, 3)
+    dofs = m.elements()[torch.from_numpy(elem)]
+    xq = torch.einsum("fbd,fqb->fqd", m.points[dofs], phi)
+    integrand = (xq * xq).sum(-1)[..., None] * phi + grad[..., 0]
+    ref = torch.zeros(m.n_points, dtype=torch.float64).index_add(
+        0, dofs.reshape(-1), torch.einsum("fqb,fq->fb", integrand, fxw).reshape(-1))
+    assert torch.allclose(b, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_facet_builtin_recognition():
+    from tensormesh_amd.facet import match_facet
+    pd = {"f": torch.zeros(5), "g": torch.zeros(5, 3)}
+    assert match_facet(lambda v, f: f * v, pd, 4).key == "f"
+    mg = match_facet(lambda v, g: v[:, None] * g[None, :], pd, 4)
+    assert mg.key == "g" and mg.n_comp == 3
+    mc = match_facet(lambda v: 2.5 * v, pd, 4)
+    assert mc.key is None and abs(float(mc.const[0]) - 2.5) < 1e-14
+    mt = match_facet(lambda v: v[:, None] * torch.tensor([0.0, -1.0], dtype=v.dtype), pd, 4)
+    assert mt.n_comp == 2 and torch.allclose(mt.const, torch.tensor([0.0, -1.0], dtype=torch.float64))
+    assert match_facet(lambda v, f: f * f * v, pd, 4) is None
+    assert match_facet(lambda v: v * v, pd, 4) is None

@@ -282,3 +282,118 @@ def test_neohookean_finite_differences():
     res_m, _ = fem.neohookean_element_tangent(P, C, "tetra", 2, u - du)
     fd_t = (res_p - res_m)[e] / (2 * h)               # d R[e,i,a] / d u[node1, 2]
     assert torch.allclose(fd_t, tan[e, :, 1, :, 2], atol=1e-7)
+
+
+# ------------------------------------------------------------------ facets ---
+def test_facet_line_rule_exact_on_monomials():
+    """n-point Gauss-Legendre on [0, 1] integrates x^k exactly for k <= 2n - 1."""
+    for n in (1, 2, 3, 4):
+        w, p = fem.facet_quadrature("triangle", n)
+        assert abs(float(w.sum()) - 1.0) < 1e-15
+        for k in range(2 * n):
+            assert abs(float((w * p[:, 0] ** k).sum()) - 1.0 / (k + 1)) < 1e-14
+
+
+def test_facet_select_brute_force():
+    """Selection against plain loops: every facet (local node set) of every
+    element with a flagged node, kept iff all its nodes are flagged."""
+    from tensormesh_amd.mesh import unit_cube_tet_arrays, quadratic_arrays
+    for order in (1, 2):
+        p, c, bm = unit_cube_tet_arrays(2, 0.1, 4)
+        et = "tetra"
+        if order == 2:
+            p, c, bm = quadratic_arrays(p, c, bm)
+            et = "tetra10"
+        faces = {1: [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]}[1]
+        mids = {(2, 3): 4, (1, 3): 5, (1, 2): 6, (0, 3): 7, (0, 2): 8, (0, 1): 9}
+        expect = []
+        for e in range(c.shape[0]):
+            if not any(bm[c[e]]):
+                continue
+            for f, fv in enumerate(faces):
+                nodes = list(fv)
+                if order == 2:
+                    nodes += [mids[(a, b)] for (a, b) in mids if a in fv and b in fv]
+                if all(bm[c[e, k]] for k in nodes):
+                    expect.append((e, f))
+        elem, fac = fem.facet_select(c, bm, et)
+        assert list(zip(elem.tolist(), fac.tolist())) == expect
+
+
+def test_facet_square_closed_forms():
+    """P1 triangles of the unit square, f = 1 and f = x over all boundary
+    facets: 4 + 4 sqrt(2) h and 2 + 2 sqrt(2) h -- the reference also selects
+    the two corner-square diagonals, whose end nodes are both on the boundary
+    (once from each of their two triangles); one edge only: |edge| = 1,
+    traction sums to t."""
+    from tensormesh_amd.mesh import unit_square_tri_arrays
+    n = 8
+    h = 1.0 / n
+    p, c, bm = unit_square_tri_arrays(n, 0.2, 3)
+    P, C, BM = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(bm)
+    for q in (1, 2, 3):
+        b1 = fem.assemble_facet_linear(P, C, BM, "triangle", q, const=torch.tensor(1.0))
+        bx = fem.assemble_facet_linear(P, C, BM, "triangle", q, f_nodal=P[:, 0].clone())
+        assert abs(float(b1.sum()) - (4 + 4 * math.sqrt(2) * h)) < 1e-13
+        assert abs(float(bx.sum()) - (2 + 2 * math.sqrt(2) * h)) < 1e-13
+    left = torch.from_numpy(p[:, 0] <= 1e-12)
+    b = fem.assemble_facet_linear(P, C, left, "triangle", 2, const=torch.tensor(1.0))
+    assert abs(float(b.sum()) - 1.0) < 1e-14
+    # interior nodes of the edge get h, its two ends h / 2
+    ys = p[left.numpy(), 1]
+    vals = b[left].numpy()
+    assert np.allclose(vals[(ys > 1e-12) & (ys < 1 - 1e-12)], h, rtol=0, atol=1e-14)
+    assert np.allclose(vals[(ys <= 1e-12) | (ys >= 1 - 1e-12)], h / 2, rtol=0, atol=1e-14)
+    bt = fem.assemble_facet_linear(P, C, left, "triangle", 2, const=torch.tensor([0.5, -1.0])).reshape(-1, 2)
+    assert torch.allclose(bt.sum(0), torch.tensor([0.5, -1.0], dtype=torch.float64), atol=1e-14)
+
+
+def test_facet_p2_simpson_and_vertex_zero():
+    """P2: on an edge, f = 1 gives Simpson weights (vertex h/6, midpoint 2h/3);
+    on a triangular face the quadratic vertex functions integrate to 0 and
+    each midpoint function to |face|/3."""
+    from tensormesh_amd.mesh import unit_square_tri_arrays, unit_cube_tet_arrays, quadratic_arrays
+    n = 4
+    h = 1.0 / n
+    p, c, bm = quadratic_arrays(*unit_square_tri_arrays(n, 0.0, 0))
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    left = torch.from_numpy(p[:, 0] <= 1e-12)
+    b = fem.assemble_facet_linear(P, C, left, "triangle6", 2, const=torch.tensor(1.0)).numpy()
+    ys = p[:, 1]
+    onl = left.numpy()
+    is_mid = np.arange(p.shape[0]) >= (n + 1) ** 2
+    assert np.allclose(b[onl & is_mid], 2 * h / 3, atol=1e-14)
+    inner_v = onl & ~is_mid & (ys > 1e-12) & (ys < 1 - 1e-12)
+    assert np.allclose(b[inner_v], h / 3, atol=1e-14)
+    p3, c3, bm3 = quadratic_arrays(*unit_cube_tet_arrays(2, 0.0, 0))
+    P3, C3 = torch.from_numpy(p3), torch.from_numpy(c3)
+    face = torch.from_numpy(p3[:, 0] <= 1e-12)
+    b3 = fem.assemble_facet_linear(P3, C3, face, "tetra10", 2, const=torch.tensor(1.0)).numpy()
+    nv = 27
+    assert np.abs(b3[:nv]).max() < 1e-15
+    assert abs(b3.sum() - 1.0) < 1e-14
+
+
+def test_facet_cube_areas_brute_force():
+    """All boundary facets of a jittered Kuhn cube (with the interior faces
+    whose three vertices lie on the boundary): sum of b for f = 1 equals the
+    sum of the selected triangles' areas computed with cross products; the
+    face x = 0 alone has area 1 and integral of y = 1/2."""
+    from tensormesh_amd.mesh import unit_cube_tet_arrays
+    p, c, bm = unit_cube_tet_arrays(3, 0.1, 2)
+    P, C, BM = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(bm)
+    faces = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    area = 0.0
+    for e in range(c.shape[0]):
+        for fv in faces:
+            nodes = [c[e, k] for k in fv]
+            if all(bm[nodes]):
+                a, b_, d = p[nodes[0]], p[nodes[1]], p[nodes[2]]
+                area += 0.5 * np.linalg.norm(np.cross(b_ - a, d - a))
+    for q in (1, 2, 3):
+        s = float(fem.assemble_facet_linear(P, C, BM, "tetra", q, const=torch.tensor(1.0)).sum())
+        assert abs(s - area) < 1e-13 * max(1.0, area)
+    face = torch.from_numpy(p[:, 0] <= 1e-12)
+    assert abs(float(fem.assemble_facet_linear(P, C, face, "tetra", 2, const=torch.tensor(1.0)).sum()) - 1.0) < 1e-14
+    y = fem.assemble_facet_linear(P, C, face, "tetra", 2, f_nodal=P[:, 1].clone())
+    assert abs(float(y.sum()) - 0.5) < 1e-14

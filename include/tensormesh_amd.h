@@ -312,6 +312,55 @@ int tm_assemble_neohookean_rows(const tm_mesh_view* mesh, const tm_form_desc* fo
                                 const tm_block_rows_view* rows, const int32_t* inc_inv, void* tslot_ws,
                                 void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
 
+/* ---------------------------------------------------------------------
+ * J2 (von Mises) plasticity with linear isotropic hardening.
+ * Replaces J2Plasticity (tensormesh/assemble/builtin.py:335-561) through
+ * ElementAssembler.energy (element_assembler.py:447-659): the algorithmic
+ * incremental potential
+ *   psi = K/2 tr(eps_tr)^2 + mu |dev eps_tr|^2 - (2 mu + 2H/3) dg^2 / 2,
+ *   eps_tr = sym(grad u) (3 x 3, zero-padded in 2D) - eps_p_n,
+ *   dg = max(2 mu |dev eps_tr| - sqrt(2/3)(sig0 + H alpha_n), 0) / (2 mu + 2H/3),
+ * its gradient (the reference's autograd residual), its consistent Hessian
+ * and the radial-return update of update_state (builtin.py:513-561).
+ * ------------------------------------------------------------------- */
+typedef struct tm_j2_material {
+  double young;              /* E */
+  double poisson;            /* nu */
+  double sig0;               /* initial yield stress */
+  double hardening;          /* H, linear isotropic hardening modulus */
+} tm_j2_material;
+
+/* Per-quadrature-point history, in the mesh dtype (J2Plasticity.history). */
+typedef struct tm_j2_state {
+  void* eps_p;               /* [n_elem][n_quad][3][3] plastic strain eps_p_n */
+  void* alpha;               /* [n_elem][n_quad] equivalent plastic strain alpha_n */
+} tm_j2_state;
+
+#define TM_J2_ENERGY_PARTS 1024
+
+/* Energy, residual and tangent at displacement disp [n_points*D] for the
+ * history in state (read only).  Each output may be NULL:
+ *   energy [1]             sum over elements and points of psi * jxw; needs
+ *                          energy_ws: double [n_elem + TM_J2_ENERGY_PARTS],
+ *                          which also receives the per-element energies [n_elem]
+ *   tangent [nnz*D*D]      BSR(D x D) node-pair CSR order; needs pat and
+ *                          tslot_ws [n_elem*B*B*D*D]
+ *   residual [n_points*D]  dE/du; needs inc and rslot_ws [n_elem*B*D]
+ * form: tables (tab_dev) and n_quad of the element rule (form->form unused).
+ * Sums are deterministic (fixed order); where |dev eps_tr| = 0 the tangent is
+ * the elastic limit (K - 2mu/3, mu). */
+int tm_assemble_j2(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_j2_material* mat,
+                   const void* disp, const tm_j2_state* state, void* energy_ws, void* energy,
+                   const tm_pattern_view* pat, const tm_incidence_view* inc, void* tslot_ws,
+                   void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
+
+/* Radial-return update of state in place at the converged displacement disp
+ * (J2Plasticity.update_state, builtin.py:513-561): where the trial yield
+ * value is positive, eps_p += dg n and alpha += sqrt(2/3) dg, with
+ * n = dev eps_tr / |dev eps_tr| (a norm below 1e-12 divides by 1). */
+int tm_j2_update_state(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_j2_material* mat,
+                       const void* disp, const tm_j2_state* state, tm_stream_t stream);
+
 /* Row-centric stage S for P1 (TRI3 / TET4): the entries of tm_assemble_bilinear
  * from a stage-G slot buffer, each CSR row reading its incidences' local rows
  * (B contiguous values each) and routing them by colpos; same per-entry order

@@ -478,3 +478,80 @@ def neohookean_element_tangent(points, conn, etype, qorder, u, E=1.0, nu=0.3):
     res = torch.func.vmap(torch.func.grad(e_energy))(ue, grad, jxw)
     hes = torch.func.vmap(torch.func.hessian(e_energy))(ue, grad, jxw)  # [E,B,H,B,H]
     return res, hes.permute(0, 1, 3, 2, 4).contiguous()
+
+
+# --------------------------------------------------------- J2 plasticity ---
+def j2_moduli(E: float, nu: float):
+    """mu, bulk = lam + 2/3 mu (builtin.py:436-440)."""
+    mu, lam = lame(E, nu)
+    return mu, lam + 2.0 / 3.0 * mu
+
+
+def _j2_strain3(G: torch.Tensor) -> torch.Tensor:
+    """sym(grad u), padded to 3 x 3 in 2D (builtin.py:483-488, 530-537)."""
+    eps = 0.5 * (G + G.transpose(-1, -2))
+    if G.shape[-1] == 2:
+        eps = torch.nn.functional.pad(eps, (0, 1, 0, 1))
+    return eps
+
+
+def j2_psi(G, eps_p, alpha, E, nu, sig0, H):
+    """Algorithmic incremental potential at quadrature points, batched over
+    leading dimensions: builtin.py:464-511 (trial strain, deviator, trial
+    yield value, clamp, vol + dev energy)."""
+    mu, bulk = j2_moduli(E, nu)
+    eps_tr = _j2_strain3(G) - eps_p
+    tr = eps_tr.diagonal(dim1=-2, dim2=-1).sum(-1)
+    dev = eps_tr - (tr / 3.0)[..., None, None] * torch.eye(3, dtype=G.dtype)
+    nrm = torch.linalg.norm(dev, dim=(-2, -1))
+    f_tr = 2.0 * mu * nrm - np.sqrt(2.0 / 3.0) * (sig0 + H * alpha)
+    denom = 2.0 * mu + (2.0 / 3.0) * H
+    dg = torch.clamp(f_tr, min=0.0) / denom
+    return 0.5 * bulk * tr * tr + mu * nrm * nrm - 0.5 * denom * dg * dg
+
+
+def j2_grad_u(points, conn, etype, qorder, u):
+    """graddisplacement [E,Q,D,D] (element_assembler.py:631-633): row = component, column = derivative."""
+    _, grad, jxw, _ = shape_grad_jxw(points, conn, etype, qorder)
+    return torch.einsum("eba,eqbd->eqad", u[conn], grad), grad, jxw
+
+
+def j2_energy(points, conn, etype, qorder, u, eps_p, alpha, E=200e9, nu=0.3, sig0=250e6, H=1e9):
+    """``J2Plasticity.energy`` = sum(psi * jxw) over elements and points
+    (element_assembler.py:641-657 with builtin.py:464-511)."""
+    G, _, jxw = j2_grad_u(points, conn, etype, qorder, u)
+    return (j2_psi(G, eps_p, alpha, E, nu, sig0, H) * jxw).sum()
+
+
+def j2_element_derivatives(points, conn, etype, qorder, u, eps_p, alpha, E=200e9, nu=0.3, sig0=250e6, H=1e9):
+    """Per-element residual [E,B,D] and tangent [E,B,B,D,D] by autograd of the
+    element energy -- what ``energy().backward()`` (and a Newton solve's
+    autograd Jacobian) produce (element_assembler.py:447-659).  Autograd's
+    Hessian is 0 or NaN where the trial deviator vanishes (torch.norm's
+    derivative at 0), so callers keep away from that point."""
+    _, grad, jxw, _ = shape_grad_jxw(points, conn, etype, qorder)
+
+    def e_energy(ue_, g_, w_, ep_, al_):
+        G = torch.einsum("ba,qbd->qad", ue_, g_)
+        return (j2_psi(G, ep_, al_, E, nu, sig0, H) * w_).sum()
+
+    ue = u[conn]
+    res = torch.func.vmap(torch.func.grad(e_energy))(ue, grad, jxw, eps_p, alpha)
+    hes = torch.func.vmap(torch.func.hessian(e_energy))(ue, grad, jxw, eps_p, alpha)
+    return res, hes.permute(0, 1, 3, 2, 4).contiguous()
+
+
+def j2_update(points, conn, etype, qorder, u, eps_p, alpha, E=200e9, nu=0.3, sig0=250e6, H=1e9):
+    """Radial-return history update of ``J2Plasticity.update_state``
+    (builtin.py:513-561): returns new (eps_p [E,Q,3,3], alpha [E,Q])."""
+    mu, _ = j2_moduli(E, nu)
+    G, _, _ = j2_grad_u(points, conn, etype, qorder, u)
+    eps_tr = _j2_strain3(G) - eps_p
+    tr = eps_tr.diagonal(dim1=-2, dim2=-1).sum(-1)
+    dev = eps_tr - (tr / 3.0)[..., None, None] * torch.eye(3, dtype=u.dtype)
+    nrm = torch.linalg.norm(dev, dim=(-2, -1))
+    f_tr = 2.0 * mu * nrm - np.sqrt(2.0 / 3.0) * (sig0 + H * alpha)
+    dg = torch.clamp(f_tr, min=0.0) / (2.0 * mu + (2.0 / 3.0) * H)
+    safe = torch.where(nrm < 1e-12, torch.ones_like(nrm), nrm)
+    dgm = dg * (f_tr > 0).to(dg.dtype)
+    return eps_p + dgm[..., None, None] * dev / safe[..., None, None], alpha + np.sqrt(2.0 / 3.0) * dgm

@@ -6,8 +6,8 @@ gfx950 HIP kernels behind the C-ABI of libtmhip.so.
 """
 
 from .assemble import ElementAssembler, NodeAssembler
-from .builtin import (DiffusionElementAssembler, LaplaceElementAssembler, LinearElasticityElementAssembler,
-                      MassElementAssembler, NeoHookeanModel, SourceNodeAssembler, const_node_assembler,
+from .builtin import (DiffusionElementAssembler, J2Plasticity, LaplaceElementAssembler,
+                      LinearElasticityElementAssembler, MassElementAssembler, NeoHookeanModel, SourceNodeAssembler, const_node_assembler,
                       func_node_assembler)
 from .condense import Condenser
 from .facet import FacetAssembler
@@ -19,6 +19,6 @@ __version__ = "0.1.0"
 __all__ = [
     "ElementAssembler", "NodeAssembler", "FacetAssembler", "Condenser", "SparseMatrix", "Mesh",
     "LaplaceElementAssembler", "DiffusionElementAssembler", "MassElementAssembler",
-    "LinearElasticityElementAssembler", "NeoHookeanModel", "SourceNodeAssembler",
+    "LinearElasticityElementAssembler", "NeoHookeanModel", "J2Plasticity", "SourceNodeAssembler",
     "const_node_assembler", "func_node_assembler",
 ]

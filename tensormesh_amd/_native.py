@@ -76,6 +76,18 @@ class PcgView(ctypes.Structure):
                  ("parity", ctypes.c_int32)]
 
 
+class J2Material(ctypes.Structure):
+    _fields_ = [("young", ctypes.c_double), ("poisson", ctypes.c_double), ("sig0", ctypes.c_double),
+                ("hardening", ctypes.c_double)]
+
+
+class J2State(ctypes.Structure):
+    _fields_ = [("eps_p", ctypes.c_void_p), ("alpha", ctypes.c_void_p)]
+
+
+J2_ENERGY_PARTS = 1024
+
+
 class IncidenceView(ctypes.Structure):
     _fields_ = [("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p),
                 ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
@@ -138,6 +150,13 @@ SIGNATURES = {
                                                 ctypes.POINTER(IncidenceView), ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
     "tm_condense": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 12),
+    "tm_assemble_j2": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.POINTER(J2Material),
+                                      ctypes.c_void_p, ctypes.POINTER(J2State), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.POINTER(PatternView), ctypes.POINTER(IncidenceView)]
+                       + [ctypes.c_void_p] * 5),
+    "tm_j2_update_state": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                          ctypes.POINTER(J2Material), ctypes.c_void_p, ctypes.POINTER(J2State),
+                                          ctypes.c_void_p]),
 }
 
 _LIB: Optional[ctypes.CDLL] = None

@@ -496,7 +496,11 @@ class ElementAssembler(_AssemblerBase):
     def energy(self, points=None, func=None, point_data=None, element_data=None, scalar_data=None,
                batch_size: int = -1):
         point_data = {} if point_data is None else dict(point_data)
-        element_data = {} if element_data is None else dict(element_data)
+        # element_data as {key: tensor} or {key: {etype: tensor}} (element_assembler.py:533-546)
+        element_data = {} if element_data is None else {
+            k: (v[self.etype] if isinstance(v, dict) else v) for k, v in element_data.items()}
+        for k, v in element_data.items():
+            assert v.shape[0] == self.elements[self.etype].shape[0], f"element_data[{k!r}] has {v.shape[0]} rows"
         scalar_data = {} if scalar_data is None else {k: torch.as_tensor(v) for k, v in scalar_data.items()}
         points = self._points if points is None else points
         point_data["x"] = points

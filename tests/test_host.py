@@ -302,3 +302,52 @@ def test_facet_builtin_recognition():
     assert mt.n_comp == 2 and torch.allclose(mt.const, torch.tensor([0.0, -1.0], dtype=torch.float64))
     assert match_facet(lambda v, f: f * f * v, pd, 4) is None
     assert match_facet(lambda v: v * v, pd, 4) is None
+
+
+# --------------------------------------------------------- J2 plasticity ---
+_J2 = dict(E=200.0, nu=0.3, sig0=0.25, H=1.0)
+
+
+@pytest.mark.parametrize("etype", ["triangle", "tetra10"])
+def test_j2_host_torch_path_matches_oracle(etype):
+    """On host tensors J2Plasticity.energy runs the generic vmap path with the
+    reference's element_energy and update_state its torch restatement; both
+    against the oracle, with history given as {etype: tensor}."""
+    order = 2 if etype == "tetra10" else 1
+    m = unit_square_tri(2, 0.1, 3, order=order) if etype == "triangle" else unit_cube_tet(1, 0.1, 3, order=order)
+    model = tm.J2Plasticity.from_mesh(m, quadrature_order=2, **_J2)
+    h = model.history[etype]
+    E = m.elements().shape[0]
+    Q = fem.quadrature(etype, 2)[0].shape[0]
+    assert h["eps_p"].shape == (E, Q, 3, 3) and h["alpha"].shape == (E, Q)
+    g = torch.Generator().manual_seed(1)
+    ep = 1e-4 * torch.randn(E, Q, 3, 3, generator=g, dtype=torch.float64)
+    ep = 0.5 * (ep + ep.transpose(-1, -2))
+    al = 1e-4 * torch.rand(E, Q, generator=g, dtype=torch.float64)
+    u = (6e-4 * torch.randn(m.points.shape, generator=g, dtype=torch.float64)).requires_grad_(True)
+    e = model.energy(point_data={"displacement": u}, element_data={"eps_p_n": {etype: ep}, "alpha_n": {etype: al}})
+    ref = fem.j2_energy(m.points, m.elements(), etype, 2, u.detach(), ep, al, **_J2)
+    assert abs(e.item() - ref.item()) <= 1e-12 * abs(ref.item())
+    e.backward()
+    res_e, _ = fem.j2_element_derivatives(m.points, m.elements(), etype, 2, u.detach(), ep, al, **_J2)
+    D = m.points.shape[1]
+    res = torch.zeros(m.n_points, D, dtype=torch.float64).index_add(0, m.elements().reshape(-1),
+                                                                     res_e.reshape(-1, D))
+    assert torch.allclose(u.grad, res, rtol=1e-10, atol=1e-13)
+    h["eps_p"], h["alpha"] = ep.clone(), al.clone()
+    model.update_state(u.detach())
+    ep1, al1 = fem.j2_update(m.points, m.elements(), etype, 2, u.detach(), ep, al, **_J2)
+    assert torch.allclose(model.history[etype]["eps_p"], ep1, rtol=1e-12, atol=1e-18)
+    assert torch.allclose(model.history[etype]["alpha"], al1, rtol=1e-12, atol=1e-18)
+
+
+def test_j2_material_object_and_native_requires_device():
+    class Steel:
+        E, nu, sigma_y, H = 200e9, 0.3, 250e6, 1e9
+
+    m = unit_cube_tet(1, 0.0, 0)
+    model = tm.J2Plasticity.from_mesh(m, material=Steel())
+    assert (model.sig0, model.H) == (250e6, 1e9)
+    assert abs(model.bulk - (model.lam + 2 / 3 * model.mu)) < 1e-3
+    with pytest.raises(nat.NativeError):
+        model.residual(torch.zeros_like(m.points))

@@ -397,3 +397,124 @@ def test_facet_cube_areas_brute_force():
     assert abs(float(fem.assemble_facet_linear(P, C, face, "tetra", 2, const=torch.tensor(1.0)).sum()) - 1.0) < 1e-14
     y = fem.assemble_facet_linear(P, C, face, "tetra", 2, f_nodal=P[:, 1].clone())
     assert abs(float(y.sum()) - 0.5) < 1e-14
+
+
+# --------------------------------------------------------- J2 plasticity ---
+_J2 = dict(E=200.0, nu=0.3, sig0=0.25, H=1.0)
+
+
+def _j2_state(E, Q, seed, scale=0.0):
+    g = torch.Generator().manual_seed(seed)
+    ep = scale * torch.randn(E, Q, 3, 3, generator=g)
+    ep = 0.5 * (ep + ep.transpose(-1, -2))
+    ep = ep - ep.diagonal(dim1=-2, dim2=-1).sum(-1)[..., None, None] / 3.0 * torch.eye(3)
+    al = scale * torch.rand(E, Q, generator=g)
+    return ep, al
+
+
+@pytest.mark.parametrize("etype", ["triangle", "tetra"])
+def test_j2_elastic_limit_is_linear_elasticity(etype):
+    """Below yield (sig0 -> inf) the incremental potential is the linear
+    elastic energy 1/2 u^T K u, and its Hessian the linear-elasticity
+    element matrices (plane strain in 2D: the strain is padded with zeros)."""
+    p, c, _ = (unit_square_tri_arrays(2, 0.1, 5) if etype == "triangle" else unit_cube_tet_arrays(1, 0.1, 5))
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    Q = fem.quadrature(etype, 2)[0].shape[0]
+    ep, al = _j2_state(C.shape[0], Q, 1)
+    u = 0.01 * torch.randn(P.shape, generator=torch.Generator().manual_seed(2))
+    prm = dict(_J2, sig0=1e9)
+    lin = fem.element_integrals(P, C, etype, 2, "linear_elasticity", E=prm["E"], nu=prm["nu"])  # [E,B,B,D,D]
+    ue = u[C]
+    e_lin = 0.5 * torch.einsum("eia,eijab,ejb->", ue, lin, ue)
+    e_j2 = fem.j2_energy(P, C, etype, 2, u, ep, al, **prm)
+    assert abs(e_j2.item() - e_lin.item()) < 1e-12 * abs(e_lin.item())
+    res, tan = fem.j2_element_derivatives(P, C, etype, 2, u, ep, al, **prm)
+    assert torch.allclose(tan, lin, rtol=1e-12, atol=1e-10)
+    assert torch.allclose(res, torch.einsum("eijab,ejb->eia", lin, ue), rtol=1e-11, atol=1e-12)
+
+
+def test_j2_simple_shear_closed_form():
+    """Homogeneous simple shear u = (gamma y, 0, 0) on the unit cube: the
+    trial deviator has norm gamma / sqrt 2, so the energy is
+    V (mu gamma^2 / 2 - (2 mu + 2H/3) dg^2 / 2) with
+    dg = max(sqrt(2) mu gamma - sqrt(2/3) sig0, 0) / (2 mu + 2H/3)."""
+    p, c, _ = unit_cube_tet_arrays(2, 0.15, 7)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    Q = fem.quadrature("tetra", 2)[0].shape[0]
+    ep, al = torch.zeros(C.shape[0], Q, 3, 3), torch.zeros(C.shape[0], Q)
+    mu = _J2["E"] / (2 * (1 + _J2["nu"]))
+    den = 2 * mu + 2.0 / 3.0 * _J2["H"]
+    for gamma in (1e-4, 2e-3, 1e-2):
+        u = torch.zeros_like(P)
+        u[:, 0] = gamma * P[:, 1]
+        dg = max(math.sqrt(2.0) * mu * gamma - math.sqrt(2.0 / 3.0) * _J2["sig0"], 0.0) / den
+        want = mu * gamma ** 2 / 2 - 0.5 * den * dg ** 2
+        got = fem.j2_energy(P, C, "tetra", 2, u, ep, al, **_J2).item()
+        assert abs(got - want) < 1e-12 * abs(want), (gamma, got, want)
+    assert dg > 0  # the last case yields
+
+
+@pytest.mark.parametrize("etype", ["triangle6", "tetra"])
+def test_j2_finite_differences_plastic(etype):
+    """Residual = central difference of the energy and tangent = central
+    difference of the residual, with a strain large enough that most
+    quadrature points are past yield and a nonzero prior plastic state."""
+    if etype == "tetra":
+        p, c, _ = unit_cube_tet_arrays(1, 0.1, 2)
+    else:
+        p, c, _ = quadratic_arrays(*unit_square_tri_arrays(1, 0.1, 2))
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    Q = fem.quadrature(etype, 2)[0].shape[0]
+    ep, al = _j2_state(C.shape[0], Q, 4, 1e-3)
+    u = 0.01 * torch.randn(P.shape, generator=torch.Generator().manual_seed(5))
+    G, _, _ = fem.j2_grad_u(P, C, etype, 2, u)
+    mu, _ = fem.j2_moduli(_J2["E"], _J2["nu"])
+    res, tan = fem.j2_element_derivatives(P, C, etype, 2, u, ep, al, **_J2)
+    e, node = 0, 1
+    h = 1e-7
+    for a in range(P.shape[1]):
+        du = torch.zeros_like(u)
+        du[C[e, node], a] = h
+        fd = (fem.j2_energy(P, C[e:e + 1], etype, 2, u + du, ep[e:e + 1], al[e:e + 1], **_J2)
+              - fem.j2_energy(P, C[e:e + 1], etype, 2, u - du, ep[e:e + 1], al[e:e + 1], **_J2)) / (2 * h)
+        assert abs(fd.item() - res[e, node, a].item()) < 1e-6 * res[e].abs().max().item()
+        rp, _ = fem.j2_element_derivatives(P, C, etype, 2, u + du, ep, al, **_J2)
+        rm, _ = fem.j2_element_derivatives(P, C, etype, 2, u - du, ep, al, **_J2)
+        fd_t = (rp - rm)[e] / (2 * h)
+        assert torch.allclose(fd_t, tan[e, :, node, :, a], rtol=1e-5, atol=1e-5 * tan[e].abs().max().item())
+    # the case is plastic: most points have a positive trial yield value
+    eps_tr = fem._j2_strain3(G) - ep
+    dev = eps_tr - eps_tr.diagonal(dim1=-2, dim2=-1).sum(-1)[..., None, None] / 3 * torch.eye(3)
+    f_tr = 2 * mu * torch.linalg.norm(dev, dim=(-2, -1)) - math.sqrt(2 / 3) * (_J2["sig0"] + _J2["H"] * al)
+    assert (f_tr > 0).float().mean() > 0.5
+
+
+def test_j2_update_returns_to_the_yield_surface():
+    """After the radial return, a point that yielded lies on the updated
+    yield surface (2 mu |dev(eps - eps_p')| = sqrt(2/3)(sig0 + H alpha')),
+    the plastic strain stays trace-free, alpha grows by sqrt(2/3) |d eps_p|,
+    and points below yield keep their state."""
+    p, c, _ = unit_cube_tet_arrays(2, 0.1, 9)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    Q = fem.quadrature("tetra", 2)[0].shape[0]
+    ep, al = _j2_state(C.shape[0], Q, 6, 1e-4)
+    u = 0.0002 * torch.randn(P.shape, generator=torch.Generator().manual_seed(8))
+    mu, _ = fem.j2_moduli(_J2["E"], _J2["nu"])
+    ep1, al1 = fem.j2_update(P, C, "tetra", 2, u, ep, al, **_J2)
+    G, _, _ = fem.j2_grad_u(P, C, "tetra", 2, u)
+    dev = lambda x: x - x.diagonal(dim1=-2, dim2=-1).sum(-1)[..., None, None] / 3 * torch.eye(3)
+    f0 = 2 * mu * torch.linalg.norm(dev(fem._j2_strain3(G) - ep), dim=(-2, -1)) \
+        - math.sqrt(2 / 3) * (_J2["sig0"] + _J2["H"] * al)
+    yielded = f0 > 0
+    assert 0 < int(yielded.sum()) < yielded.numel()
+    f1 = 2 * mu * torch.linalg.norm(dev(fem._j2_strain3(G) - ep1), dim=(-2, -1)) \
+        - math.sqrt(2 / 3) * (_J2["sig0"] + _J2["H"] * al1)
+    assert f1[yielded].abs().max() < 1e-12
+    assert torch.equal(ep1[~yielded], ep[~yielded]) and torch.equal(al1[~yielded], al[~yielded])
+    assert ep1.diagonal(dim1=-2, dim2=-1).sum(-1).abs().max() < 1e-15
+    d = torch.linalg.norm(ep1 - ep, dim=(-2, -1))
+    assert torch.allclose(al1 - al, math.sqrt(2 / 3) * d, rtol=1e-12, atol=1e-18)
+    # the energy after the update is the elastic energy of the new state
+    e1 = fem.j2_energy(P, C, "tetra", 2, u, ep1, al1, **_J2)
+    e_el = fem.j2_energy(P, C, "tetra", 2, u, ep1, al1, **dict(_J2, sig0=1e9))
+    assert abs(e1.item() - e_el.item()) < 1e-9 * abs(e_el.item())

@@ -365,6 +365,66 @@ def facet_bench(n=64, reps=20):
             "speedup": ms_ref / ms, "max_rel_diff": float((b - b_ref).abs().max() / b_ref.abs().max())}
 
 
+def plasticity_bench(n=48, reps=10):
+    """J2 plasticity (J2Plasticity, E = 200, nu = 0.3, sig0 = 0.25, H = 1) on
+    the jittered n^3 x 6 Kuhn tetrahedral cube (48^3 -> 663,552 tets), P1,
+    4-point rule, float64, at a displacement past yield at most points with a
+    nonzero prior plastic state.  Device time per call (HIP events) of the
+    native energy, energy + backward (the reference's residual workflow),
+    residual + consistent tangent and update_state, beside the reference
+    algorithm (generic vmapped torch energy + autograd backward) on the same
+    GPU."""
+    import tensormesh_amd as tm
+    from tensormesh_amd.mesh import unit_cube_tet
+    m = unit_cube_tet(n, 0.1, 0).to("cuda")
+    model = tm.J2Plasticity.from_mesh(m, quadrature_order=2, E=200.0, nu=0.3, sig0=0.25, H=1.0)
+    P = m.points
+    u = torch.stack([3e-3 * P[:, 0] + 2e-4 * torch.sin(6 * P[:, 1]), -1e-3 * P[:, 1],
+                     5e-4 * torch.cos(5 * P[:, 0]) * P[:, 2]], 1)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    h = model.history["tetra"]
+    h["eps_p"].copy_(1e-4 * torch.randn(h["eps_p"].shape, generator=g, device="cuda", dtype=torch.float64))
+    h["eps_p"].copy_(0.5 * (h["eps_p"] + h["eps_p"].transpose(-1, -2)))
+    h["alpha"].copy_(1e-4 * torch.rand(h["alpha"].shape, generator=g, device="cuda", dtype=torch.float64))
+    hist = {"eps_p_n": h["eps_p"], "alpha_n": h["alpha"]}
+
+    def timed(call, n_rep=reps):
+        out = call()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        for _ in range(n_rep):
+            out = call()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return out, ev[0].elapsed_time(ev[1]) / n_rep
+
+    def energy_and_grad():
+        x = u.clone().requires_grad_(True)
+        e = model.energy(point_data={"displacement": x}, element_data=hist)
+        e.backward()
+        return e.detach(), x.grad
+
+    e_nat, ms_e = timed(lambda: model.energy(point_data={"displacement": u}, element_data=hist))
+    (e1, g_nat), ms_eg = timed(energy_and_grad)
+    _, ms_rt = timed(lambda: model.residual_and_tangent(u))
+    model.reference_path = True
+    (e_ref, g_ref), ms_ref = timed(energy_and_grad, 3)
+    model.reference_path = False
+    saved = (h["eps_p"].clone(), h["alpha"].clone())
+    _, ms_up = timed(lambda: model.update_state(u), 1)
+    h["eps_p"].copy_(saved[0])
+    h["alpha"].copy_(saved[1])
+    E = m.elements().shape[0]
+    return {"workload": f"J2 plasticity P1 tets {n}^3x6 = {E} elements, {m.n_points} nodes, qorder 2 (4 points), f64",
+            "ms_energy": ms_e, "ms_energy_and_backward": ms_eg, "ms_residual_and_tangent": ms_rt,
+            "ms_update_state": ms_up, "reference_algorithm_ms_energy_and_backward": ms_ref,
+            "speedup_energy_and_backward": ms_ref / ms_eg,
+            "elements_per_s_residual_and_tangent": E / (ms_rt * 1e-3),
+            "energy_rel_diff": abs(e_nat.item() - e_ref.item()) / abs(e_ref.item()),
+            "grad_max_rel_diff": float((g_nat - g_ref).abs().max() / g_ref.abs().max())}
+
+
 def batched_bench(n_meshes=256, seed=0):
     """256 jittered 2-D P1 meshes, side N ~ U{16..128} (seeded), own log-normal
     kappa and a sinusoidal source each; one assembly launch for all stiffness
@@ -604,6 +664,7 @@ def main():
         out["neo_hookean"] = hyperelastic_bench()
         out["batched"] = batched_bench()
         out["facet"] = facet_bench()
+        out["plasticity"] = plasticity_bench()
     if not args.no_cpu_baseline:
         cb = cpu_baseline(*host)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -361,6 +361,37 @@ int tm_assemble_j2(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_
 int tm_j2_update_state(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_j2_material* mat,
                        const void* disp, const tm_j2_state* state, tm_stream_t stream);
 
+/* ---------------------------------------------------------------------
+ * Penalty contact against a plane on the selected boundary facets.
+ * Replaces ContactAssembler.energy (tensormesh/assemble/builtin.py:609-728)
+ * with the penalty density of its example (builtin.py:588-597) for the plane
+ * n . x = offset, and the autograd derivatives of that energy:
+ *   x_q = sum_b phi_b(q) x_b, g_q = n . x_q - offset,
+ *   energy = sum_facets sum_q kappa/2 max(-g_q, 0)^2 FxW_q,
+ * with FxW of the configuration mesh->points and x the positions.
+ * ------------------------------------------------------------------- */
+typedef struct tm_contact_plane {
+  double kappa;              /* penalty stiffness */
+  double normal[3];          /* unit obstacle normal n (entries past D unused) */
+  double offset;             /* g = n . x - offset; contact where g <= 0 */
+} tm_contact_plane;
+
+/* x [n_points*D]: positions in the mesh dtype.  Each output may be NULL:
+ *   energy [1]           needs energy_ws: double [n_sel + 1024] (per-facet
+ *                        energies, then partial sums)
+ *   grad [n_points*D]    dE/dx; needs inc (the selected facets' cell nodes,
+ *                        entries f*B + b) and rslot_ws [n_sel*B*D]
+ *   hess [nnz*D*D]       d2E/dx2 in the BSR(D x D) node-pair order of the
+ *                        cell mesh's pattern; fpat->seg_ptr [nnz + 1] and
+ *                        fpat->seg_slot list, per pattern entry, the facet
+ *                        slots f*B*B + b*B + c that land on it (ascending);
+ *                        entries with none come out 0.  tslot_ws [n_sel*B*B*D*D]
+ * facet_tab_dev / n_quad as tm_assemble_facet_linear. */
+int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view* facets, const double* facet_tab_dev,
+                       int32_t n_quad, const tm_contact_plane* plane, const void* x, void* energy_ws,
+                       void* energy, const tm_incidence_view* inc, void* rslot_ws, void* grad,
+                       const tm_pattern_view* fpat, void* tslot_ws, void* hess, tm_stream_t stream);
+
 /* Row-centric stage S for P1 (TRI3 / TET4): the entries of tm_assemble_bilinear
  * from a stage-G slot buffer, each CSR row reading its incidences' local rows
  * (B contiguous values each) and routing them by colpos; same per-entry order

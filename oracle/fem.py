@@ -555,3 +555,43 @@ def j2_update(points, conn, etype, qorder, u, eps_p, alpha, E=200e9, nu=0.3, sig
     safe = torch.where(nrm < 1e-12, torch.ones_like(nrm), nrm)
     dgm = dg * (f_tr > 0).to(dg.dtype)
     return eps_p + dgm[..., None, None] * dev / safe[..., None, None], alpha + np.sqrt(2.0 / 3.0) * dgm
+
+
+# ------------------------------------------------------- penalty contact ---
+def contact_penalty_psi(x: torch.Tensor, kappa: float, normal, offset: float) -> torch.Tensor:
+    """kappa/2 <-(n . x - offset)>_+^2 at points x[..., D]: the penalty density
+    of the ContactAssembler example (builtin.py:588-597) for a plane n . x = offset."""
+    n = torch.as_tensor(normal, dtype=x.dtype)
+    pen = torch.clamp(-((x * n).sum(-1) - offset), min=0.0)
+    return 0.5 * kappa * pen * pen
+
+
+def contact_energy(points_ref, x_nodes, conn, boundary_mask, etype, order, kappa, normal, offset):
+    """``ContactAssembler.energy`` (builtin.py:609-728) for the penalty
+    density: the nodal positions x are interpolated at the facet points of
+    the selected facets with the cell basis (einsum 'sbd,sqb->sqd'), and the
+    density is integrated against FxW of the configuration points_ref."""
+    elem, fac = facet_select(conn.numpy(), boundary_mask.numpy(), etype)
+    phi, _, fxw = facet_geometry(points_ref, conn, elem, fac, etype, order)
+    dofs = conn[torch.from_numpy(elem)]
+    xq = torch.einsum("sbd,sqb->sqd", x_nodes[dofs], phi)
+    return (contact_penalty_psi(xq, kappa, normal, offset) * fxw).sum()
+
+
+def contact_facet_derivatives(points_ref, x_nodes, conn, boundary_mask, etype, order, kappa, normal, offset):
+    """Per-facet gradient [F, B, D] and Hessian [F, B, B, D, D] of the penalty
+    contact energy with respect to the nodal positions, by autograd (what
+    ``energy().backward()`` and an autograd Jacobian give), plus the facets'
+    (element, local facet) lists."""
+    elem, fac = facet_select(conn.numpy(), boundary_mask.numpy(), etype)
+    phi, _, fxw = facet_geometry(points_ref, conn, elem, fac, etype, order)
+    dofs = conn[torch.from_numpy(elem)]
+
+    def f_energy(xe, ph, w):
+        xq = torch.einsum("bd,qb->qd", xe, ph)
+        return (contact_penalty_psi(xq, kappa, normal, offset) * w).sum()
+
+    xe = x_nodes[dofs]
+    res = torch.func.vmap(torch.func.grad(f_energy))(xe, phi, fxw)
+    hes = torch.func.vmap(torch.func.hessian(f_energy))(xe, phi, fxw)
+    return res, hes.permute(0, 1, 3, 2, 4).contiguous(), elem, fac

@@ -10,6 +10,7 @@ from .builtin import (DiffusionElementAssembler, J2Plasticity, LaplaceElementAss
                       LinearElasticityElementAssembler, MassElementAssembler, NeoHookeanModel, SourceNodeAssembler, const_node_assembler,
                       func_node_assembler)
 from .condense import Condenser
+from .contact import ContactAssembler, PenaltyContact
 from .facet import FacetAssembler
 from .mesh import Mesh
 from .sparse import SparseMatrix
@@ -17,7 +18,7 @@ from .sparse import SparseMatrix
 __version__ = "0.1.0"
 
 __all__ = [
-    "ElementAssembler", "NodeAssembler", "FacetAssembler", "Condenser", "SparseMatrix", "Mesh",
+    "ElementAssembler", "NodeAssembler", "FacetAssembler", "ContactAssembler", "PenaltyContact", "Condenser", "SparseMatrix", "Mesh",
     "LaplaceElementAssembler", "DiffusionElementAssembler", "MassElementAssembler",
     "LinearElasticityElementAssembler", "NeoHookeanModel", "J2Plasticity", "SourceNodeAssembler",
     "const_node_assembler", "func_node_assembler",

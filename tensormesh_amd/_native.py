@@ -88,6 +88,10 @@ class J2State(ctypes.Structure):
 J2_ENERGY_PARTS = 1024
 
 
+class ContactPlane(ctypes.Structure):
+    _fields_ = [("kappa", ctypes.c_double), ("normal", ctypes.c_double * 3), ("offset", ctypes.c_double)]
+
+
 class IncidenceView(ctypes.Structure):
     _fields_ = [("inc_ptr", ctypes.c_void_p), ("inc", ctypes.c_void_p),
                 ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
@@ -154,6 +158,11 @@ SIGNATURES = {
                                       ctypes.c_void_p, ctypes.POINTER(J2State), ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.POINTER(PatternView), ctypes.POINTER(IncidenceView)]
                        + [ctypes.c_void_p] * 5),
+    "tm_contact_penalty": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FacetView), ctypes.c_void_p,
+                                          ctypes.c_int32, ctypes.POINTER(ContactPlane), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(IncidenceView),
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(PatternView),
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "tm_j2_update_state": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                           ctypes.POINTER(J2Material), ctypes.c_void_p, ctypes.POINTER(J2State),
                                           ctypes.c_void_p]),

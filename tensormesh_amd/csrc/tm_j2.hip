@@ -36,8 +36,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
-constexpr int kEnergyParts = 1024;
-constexpr int kSumBlock = 256;
+static_assert(tmh::kSumParts == TM_J2_ENERGY_PARTS, "energy workspace size");
 
 struct J2Consts {
   double mu, bulk, sig0, hard, denom, r23;
@@ -283,40 +282,6 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   }
 }
 
-// Fixed-shape sum of n doubles: block p of kEnergyParts sums a contiguous
-// chunk (strided per thread, then an LDS tree), and one block sums the
-// partials in order, so the result does not depend on scheduling.
-__device__ double block_tree_sum(double v, double* s_red) {
-  s_red[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = kSumBlock / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
-    __syncthreads();
-  }
-  return s_red[0];
-}
-
-__global__ void __launch_bounds__(kSumBlock)
-    energy_partials_kernel(const double* __restrict__ v, int64_t n, double* __restrict__ part) {
-  __shared__ double s_red[kSumBlock];
-  const int64_t chunk = (n + kEnergyParts - 1) / kEnergyParts;
-  const int64_t lo = (int64_t)blockIdx.x * chunk;
-  const int64_t hi = lo + chunk < n ? lo + chunk : n;
-  double s = 0.0;
-  for (int64_t i = lo + threadIdx.x; i < hi; i += kSumBlock) s += v[i];
-  const double tot = block_tree_sum(s, s_red);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kSumBlock) energy_final_kernel(const double* __restrict__ part, T* __restrict__ out) {
-  __shared__ double s_red[kSumBlock];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < kEnergyParts; i += kSumBlock) s += part[i];
-  const double tot = block_tree_sum(s, s_red);
-  if (threadIdx.x == 0) out[0] = (T)tot;
-}
-
 template <typename T, int D, int B>
 int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, const void* disp,
               const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s) {
@@ -396,17 +361,8 @@ extern "C" int tm_assemble_j2(const tm_mesh_view* mesh, const tm_form_desc* form
                              : dispatch_j2<float>(mesh, form, k, disp, state, 0, e_out, rs, ts, s);
   if (st) return st;
   if (energy) {
-    double* part = e_out + mesh->n_elem;
-    energy_partials_kernel<<<kEnergyParts, kSumBlock, 0, s>>>(e_out, mesh->n_elem, part);
-    if (mesh->dtype == TM_F64)
-      energy_final_kernel<double><<<1, kSumBlock, 0, s>>>(part, static_cast<double*>(energy));
-    else
-      energy_final_kernel<float><<<1, kSumBlock, 0, s>>>(part, static_cast<float*>(energy));
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) {
-      tmh::set_error("energy sum launch: %s", hipGetErrorString(err));
-      return TM_ERR_HIP;
-    }
+    st = tmh::sum_fixed(e_out, mesh->n_elem, e_out + mesh->n_elem, mesh->dtype, energy, stream_);
+    if (st) return st;
   }
   if (tangent) {
     st = tm_segmented_reduce(mesh->dtype, pat->nnz, (int32_t)(D * D), pat->seg_ptr, pat->seg_slot, pat->n_blocks,

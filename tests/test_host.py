@@ -1,6 +1,7 @@
 """CPU tests of the host logic: the C-ABI library, the host pattern builder,
 form recognition, the generic torch form path and the API containers."""
 import ctypes
+import math
 import re
 from pathlib import Path
 
@@ -351,3 +352,48 @@ def test_j2_material_object_and_native_requires_device():
     assert abs(model.bulk - (model.lam + 2 / 3 * model.mu)) < 1e-3
     with pytest.raises(nat.NativeError):
         model.residual(torch.zeros_like(m.points))
+
+
+# ------------------------------------------------------- contact energies ---
+@pytest.mark.parametrize("kind,order", [("tri", 2), ("tet", 1)])
+def test_penalty_contact_torch_path_matches_oracle(kind, order):
+    """PenaltyContact.energy on host tensors (the generic vmapped path with
+    the reference's example density) against the oracle, for a deformed
+    configuration pressing into a tilted plane; FxW of the construction
+    configuration."""
+    m, et = _facet_mesh(kind, order)
+    D = m.points.shape[1]
+    nrm = [0.2, 1.0] if D == 2 else [0.1, 1.0, -0.2]
+    model = tm.PenaltyContact.from_mesh(m, quadrature_order=2, kappa=3.0, normal=nrm, offset=0.05)
+    g = torch.Generator().manual_seed(4)
+    x = m.points + 0.02 * torch.randn(m.points.shape, generator=g, dtype=torch.float64)
+    x[:, 1] -= 0.1
+    e = model.energy(points=x)
+    s = math.sqrt(sum(v * v for v in nrm))
+    ref = fem.contact_energy(m.points, x, m.elements(), m.boundary_mask, et, 2, 3.0, [v / s for v in nrm], 0.05)
+    assert ref.item() > 0
+    assert abs(e.item() - ref.item()) <= 1e-12 * ref.item()
+
+
+def test_contact_assembler_generic_arguments():
+    """A user density with per-element data, a scalar and interpolated nodal
+    data (the reference's broadcast rules), against explicit einsums."""
+    m, et = _facet_mesh("tri", 1)
+    elem, fac = fem.facet_select(m.elements().numpy(), m.boundary_mask.numpy(), et)
+    phi, _, fxw = fem.facet_geometry(m.points, m.elements(), elem, fac, et, 2)
+
+    class Robin(tm.ContactAssembler):
+        def element_energy(self, x, u, k, c):
+            return k * c * (u * u).sum() + x[0]
+
+    model = Robin.from_mesh(m, quadrature_order=2)
+    u = torch.sin(3 * m.points)
+    k = torch.linspace(1.0, 2.0, m.elements().shape[0], dtype=torch.float64)
+    e = model.energy(point_data={"u": u}, element_data={"k": {et: k}}, scalar_data={"c": 0.5})
+    dofs = m.elements()[torch.from_numpy(elem)]
+    uq = torch.einsum("sbd,sqb->sqd", u[dofs], phi)
+    xq = torch.einsum("sbd,sqb->sqd", m.points[dofs], phi)
+    dens = k[torch.from_numpy(elem)][:, None] * 0.5 * (uq * uq).sum(-1) + xq[..., 0]
+    assert torch.allclose(e, (dens * fxw).sum(), rtol=1e-13)
+    with pytest.raises(ValueError):
+        model.energy(func=lambda y: y.sum())

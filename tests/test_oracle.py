@@ -518,3 +518,73 @@ def test_j2_update_returns_to_the_yield_surface():
     e1 = fem.j2_energy(P, C, "tetra", 2, u, ep1, al1, **_J2)
     e_el = fem.j2_energy(P, C, "tetra", 2, u, ep1, al1, **dict(_J2, sig0=1e9))
     assert abs(e1.item() - e_el.item()) < 1e-9 * abs(e_el.item())
+
+
+# ------------------------------------------------------- penalty contact ---
+@pytest.mark.parametrize("kind", ["tri", "tet"])
+def test_contact_energy_brute_force(kind):
+    """Rigid translation by -h along y against the plane y = 0: facets between
+    the rows y = 0 and y = h penetrate by h - y (a polynomial on each
+    facet), the rest not at all.  Against per-facet adaptive quadrature of
+    kappa/2 (h - y)_+^2 over the straight facet (scipy quad / dblquad with the
+    facet's own measure), independent of the shape-function tables."""
+    import scipy.integrate as si
+    n = 3 if kind == "tri" else 2
+    if kind == "tri":
+        p, c, bm = unit_square_tri_arrays(n, 0.0, 0)
+        et = "triangle"
+    else:
+        p, c, bm = unit_cube_tet_arrays(n, 0.0, 0)
+        et = "tetra"
+    P, C, BM = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(bm)
+    h, kappa = 1.0 / n, 7.0
+    x = P.clone()
+    x[:, 1] -= h
+    nrm = [0.0, 1.0] if kind == "tri" else [0.0, 1.0, 0.0]
+    got = fem.contact_energy(P, x, C, BM, et, 2, kappa, nrm, 0.0).item()
+    elem, fac = fem.facet_select(c, bm, et)
+    loc = fem.facet_local_nodes(et)
+    want = 0.0
+    for e, f in zip(elem, fac):
+        v = p[c[e, loc[f]]]
+        pen = lambda y: 0.5 * kappa * max(h - y, 0.0) ** 2
+        if kind == "tri":
+            L = np.linalg.norm(v[1] - v[0])
+            want += L * si.quad(lambda t: pen((1 - t) * v[0, 1] + t * v[1, 1]), 0, 1, epsabs=1e-14)[0]
+        else:
+            A = np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0]))
+            g = lambda t, s: pen(v[0, 1] + s * (v[1, 1] - v[0, 1]) + t * (v[2, 1] - v[0, 1]))
+            want += A * si.dblquad(g, 0, 1, 0, lambda s: 1 - s, epsabs=1e-14)[0]
+    assert want > 0
+    assert abs(got - want) < 1e-12 * want, (got, want)
+    # lifted clear of the plane: no energy
+    assert fem.contact_energy(P, P + 1.0, C, BM, et, 2, kappa, nrm, 0.0).item() == 0.0
+
+
+def test_contact_finite_differences():
+    """Gradient = central difference of the energy, Hessian = central
+    difference of the gradient, for a tilted plane cutting through facets."""
+    p, c, bm = quadratic_arrays(*unit_square_tri_arrays(2, 0.1, 3))
+    P, C, BM = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(bm)
+    nrm = [0.3 / math.hypot(0.3, 1.0), 1.0 / math.hypot(0.3, 1.0)]
+    g = torch.Generator().manual_seed(2)
+    x = P + 0.01 * torch.randn(P.shape, generator=g)
+    x[:, 1] -= 0.2
+    res, hes, elem, fac = fem.contact_facet_derivatives(P, x, C, BM, "triangle6", 3, 5.0, nrm, 0.05)
+    assert res.abs().max() > 0
+    # a facet in contact, and one of its nodes with a nonzero gradient
+    fi = int(torch.argmax(res.abs().amax((1, 2))))
+    b = int(torch.argmax(res[fi].abs().amax(1)))
+    node = C[elem[fi], b]
+    hstep = 1e-6
+    E = lambda xx: fem.contact_energy(P, xx, C, BM, "triangle6", 3, 5.0, nrm, 0.05)
+    grad_full = torch.zeros_like(P).index_add(0, C[torch.from_numpy(elem)].reshape(-1), res.reshape(-1, 2))
+    for a in range(2):
+        dx = torch.zeros_like(x)
+        dx[node, a] = hstep
+        fd = (E(x + dx) - E(x - dx)) / (2 * hstep)
+        assert abs(fd.item() - grad_full[node, a].item()) < 1e-7 * grad_full.abs().max().item()
+        rp = fem.contact_facet_derivatives(P, x + dx, C, BM, "triangle6", 3, 5.0, nrm, 0.05)[0]
+        rm = fem.contact_facet_derivatives(P, x - dx, C, BM, "triangle6", 3, 5.0, nrm, 0.05)[0]
+        fd_h = (rp - rm)[fi] / (2 * hstep)
+        assert torch.allclose(fd_h, hes[fi, :, b, :, a], atol=1e-6 * hes[fi].abs().max().item())

@@ -50,7 +50,13 @@ enum tm_cell {
   TM_TRI3 = 0,  /* "triangle"  B=3  D=2 */
   TM_TET4 = 1,  /* "tetra"     B=4  D=3 */
   TM_TRI6 = 2,  /* "triangle6" B=6  D=2 */
-  TM_TET10 = 3  /* "tetra10"   B=10 D=3 */
+  TM_TET10 = 3, /* "tetra10"   B=10 D=3 */
+  /* tensor-product cells on [0,1]^D, lexicographic vertices, then edge,
+   * face and cell nodes (element.py:1801-1935, 2090-2222) */
+  TM_QUAD4 = 4, /* "quad"         B=4  D=2 */
+  TM_QUAD9 = 5, /* "quad9"        B=9  D=2 */
+  TM_HEX8 = 6,  /* "hexahedron"   B=8  D=3 */
+  TM_HEX27 = 7  /* "hexahedron27" B=27 D=3 */
 };
 
 enum tm_form {
@@ -147,7 +153,9 @@ typedef struct tm_block_rows_view {
 
 /* Selected boundary facets (FacetAssembler): facet i is local facet local[i]
  * of element elem[i] (triangle: edges (1,2), (0,2), (0,1); tetrahedron: faces
- * (1,2,3), (0,2,3), (0,1,3), (0,1,2)), in the reference's selection order. */
+ * (1,2,3), (0,2,3), (0,1,3), (0,1,2); quadrilateral: edges (0,1), (0,2), (1,3),
+ * (2,3); hexahedron: faces (0,1,2,3), (0,1,4,5), (0,2,4,6), (1,3,5,7),
+ * (2,3,6,7), (4,5,6,7)), in the reference's selection order. */
 typedef struct tm_facet_view {
   int64_t n_sel;
   const int32_t* elem;       /* [n_sel] */

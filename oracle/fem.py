@@ -22,9 +22,18 @@ import torch
 from tensormesh_amd.element import TET_QUAD, TRI_QUAD  # data tables only
 
 FAMILY = {"triangle": ("tri", 2, 1), "triangle6": ("tri", 2, 2),
-          "tetra": ("tet", 3, 1), "tetra10": ("tet", 3, 2)}
+          "tetra": ("tet", 3, 1), "tetra10": ("tet", 3, 2),
+          "quad": ("quad", 2, 1), "quad9": ("quad", 2, 2),
+          "hexahedron": ("hex", 3, 1), "hexahedron27": ("hex", 3, 2)}
 TRI_EDGE = [[1, 2], [0, 2], [0, 1]]                         # element.py:1684
 TET_EDGE = [[2, 3], [1, 3], [1, 2], [0, 3], [0, 2], [0, 1]]  # element.py:1949
+# tensor-product cells (element.py:1811-1814, 2100-2104)
+QUAD_POINTS = [[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]
+QUAD_EDGE = [[0, 1], [0, 2], [1, 3], [2, 3]]
+HEX_POINTS = [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [1.0, 1.0, 0.0],
+              [0.0, 0.0, 1.0], [1.0, 0.0, 1.0], [0.0, 1.0, 1.0], [1.0, 1.0, 1.0]]
+HEX_EDGE = [[0, 1], [0, 2], [0, 4], [1, 3], [1, 5], [2, 3], [2, 6], [3, 7], [4, 5], [4, 6], [5, 7], [6, 7]]
+HEX_FACE = [(0, 1, 2, 3), (0, 1, 4, 5), (0, 2, 4, 6), (1, 3, 5, 7), (2, 3, 6, 7), (4, 5, 6, 7)]
 
 
 # ---------------------------------------------------------------- pattern ---
@@ -51,16 +60,49 @@ def pattern(conn: np.ndarray, n_points: int):
 
 
 # ----------------------------------------------------- reference element ---
+def line_rule(order: int):
+    """lin_quadrature (quadrature.py:13-66): n-point Gauss-Legendre, weights
+    halved and points mapped to [0, 1]."""
+    w, p = GAUSS_LEGENDRE[order]
+    return torch.tensor(w, dtype=torch.float64) / 2, (torch.tensor(p, dtype=torch.float64) + 1) / 2
+
+
 def quadrature(etype: str, order: int):
-    """Quadrature table (tensormesh/element/quadrature.py:91-160)."""
+    """Quadrature table (tensormesh/element/quadrature.py:91-160); for
+    quadrilaterals and hexahedra the tensor product of the ``order``-point line
+    rule built as quad_quadrature / hex_quadrature do (quadrature.py:163-239):
+    ``torch.meshgrid(..., indexing='xy')``, flattened."""
     fam = FAMILY[etype][0]
+    if fam == "quad":
+        w, p = line_rule(order)
+        wx, wy = torch.meshgrid(w, w, indexing="xy")
+        px, py = torch.meshgrid(p, p, indexing="xy")
+        return wx.flatten() * wy.flatten(), torch.stack([px.flatten(), py.flatten()], -1)
+    if fam == "hex":
+        w, p = line_rule(order)
+        wx, wy, wz = torch.meshgrid(w, w, w, indexing="xy")
+        px, py, pz = torch.meshgrid(p, p, p, indexing="xy")
+        return (wx.flatten() * wy.flatten() * wz.flatten(),
+                torch.stack([px.flatten(), py.flatten(), pz.flatten()], -1))
     w, p = (TRI_QUAD if fam == "tri" else TET_QUAD)[order]
     return torch.tensor(w, dtype=torch.float64), torch.tensor(p, dtype=torch.float64)
 
 
 def lagrange_nodes(etype: str) -> np.ndarray:
-    """Basis nodes: vertices then edge midpoints (basis.py:639-696,750-859)."""
+    """Basis nodes: vertices, then edge midpoints (basis.py:639-696,750-859);
+    for quadrilaterals / hexahedra also the face centres and the cell centre
+    (quad_basis / hex_basis, basis.py:697-748, 861-905)."""
     fam, dim, order = FAMILY[etype]
+    if fam in ("quad", "hex"):
+        verts = np.array(QUAD_POINTS if fam == "quad" else HEX_POINTS)
+        if order == 1:
+            return verts
+        edges = QUAD_EDGE if fam == "quad" else HEX_EDGE
+        faces = [(0, 1, 2, 3)] if fam == "quad" else HEX_FACE
+        out = [verts] + [0.5 * (verts[a] + verts[b]) for a, b in edges] + [verts[list(f)].mean(0) for f in faces]
+        if fam == "hex":
+            out.append(np.full(3, 0.5))
+        return np.vstack(out)
     verts = np.vstack([np.zeros(dim), np.eye(dim)])
     if order == 1:
         return verts
@@ -68,8 +110,10 @@ def lagrange_nodes(etype: str) -> np.ndarray:
     return np.vstack([verts] + [0.5 * (verts[a] + verts[b]) for a, b in edges])
 
 
-def _exponents(dim: int, order: int):
-    return [e for e in itertools.product(range(order + 1), repeat=dim) if sum(e) <= order]
+def _exponents(dim: int, order: int, tensor: bool = False):
+    """Monomial exponents: total degree <= order (simplices), or each
+    exponent <= order (Polynomial.tens_exp, tensor-product cells)."""
+    return [e for e in itertools.product(range(order + 1), repeat=dim) if tensor or sum(e) <= order]
 
 
 def vandermonde_coefficients(etype: str) -> Tuple[np.ndarray, list]:
@@ -80,7 +124,7 @@ def vandermonde_coefficients(etype: str) -> Tuple[np.ndarray, list]:
     """
     fam, dim, order = FAMILY[etype]
     nodes = lagrange_nodes(etype)
-    exps = _exponents(dim, order)
+    exps = _exponents(dim, order, fam in ("quad", "hex"))
     V = np.array([[np.prod(node ** np.array(e)) for e in exps] for node in nodes])
     return np.linalg.inv(V).T, exps
 
@@ -283,7 +327,8 @@ GAUSS_LEGENDRE = {
         [-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526]),
 }
 # facet vertices per cell (element.py:1684-1685 edges of the triangle, :1950 faces of the tetrahedron)
-FACET_VERTS = {"tri": [(1, 2), (0, 2), (0, 1)], "tet": [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]}
+FACET_VERTS = {"tri": [(1, 2), (0, 2), (0, 1)], "tet": [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)],
+               "quad": [tuple(e) for e in QUAD_EDGE], "hex": HEX_FACE}
 
 
 def facet_quadrature(etype: str, order: int):
@@ -292,20 +337,25 @@ def facet_quadrature(etype: str, order: int):
     (quadrature.py:13-66, n = order); triangle facets the triangle rule
     (quadrature.py:362-376)."""
     fam = FAMILY[etype][0]
-    if fam == "tri":
-        w, p = GAUSS_LEGENDRE[order]
-        return (torch.tensor(w, dtype=torch.float64) / 2,
-                (torch.tensor(p, dtype=torch.float64)[:, None] + 1) / 2)
+    if fam in ("tri", "quad"):
+        w, p = line_rule(order)                    # facet_quadrature_2d (quadrature.py:321-364)
+        return w, p[:, None]
+    if fam == "hex":
+        return quadrature("quad", order)           # hex_facet_quadrature (quadrature.py:410-450)
     return quadrature("triangle", order)
 
 
 def facet_local_nodes(etype: str):
-    """Local nodes of each facet (vertices, then P2 edge midpoints on it)."""
+    """Local nodes of each facet (vertices, then the order-2 edge midpoints on
+    it, then a hexahedron face's centre; basis.py:1127-1287)."""
     fam, dim, order = FAMILY[etype]
-    edges = TRI_EDGE if fam == "tri" else TET_EDGE
+    edges = {"tri": TRI_EDGE, "tet": TET_EDGE, "quad": QUAD_EDGE, "hex": HEX_EDGE}[fam]
+    n_vert = {"tri": 3, "tet": 4, "quad": 4, "hex": 8}[fam]
     out = []
-    for fv in FACET_VERTS[fam]:
-        extra = [dim + 1 + m for m, (a, b) in enumerate(edges) if a in fv and b in fv] if order == 2 else []
+    for fi, fv in enumerate(FACET_VERTS[fam]):
+        extra = [n_vert + m for m, (a, b) in enumerate(edges) if a in fv and b in fv] if order == 2 else []
+        if order == 2 and fam == "hex":
+            extra.append(n_vert + len(HEX_EDGE) + fi)
         out.append(list(fv) + extra)
     return np.array(out)
 
@@ -338,7 +388,7 @@ def facet_geometry(points: torch.Tensor, conn: torch.Tensor, elem: np.ndarray, f
     the facet tangents are F = (d gamma / d r) J and FxW = w |sqrt(det(F F^T))|."""
     fam, dim, _ = FAMILY[etype]
     w, r = facet_quadrature(etype, order)
-    verts = lagrange_nodes(etype)[:dim + 1]
+    verts = lagrange_nodes(etype)[:{"tri": 3, "tet": 4, "quad": 4, "hex": 8}[fam]]
     tabs = []
     for fv in FACET_VERTS[fam]:
         a = verts[fv[0]]

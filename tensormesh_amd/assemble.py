@@ -1,8 +1,8 @@
 """ElementAssembler / NodeAssembler with the reference's API.
 
 Mirrors tensormesh/assemble/element_assembler.py:35-879 and
-tensormesh/assemble/node_assembler.py:33-741 for simplex meshes
-(triangle, tetra, triangle6, tetra10):
+tensormesh/assemble/node_assembler.py:33-741 for meshes of one cell type
+(triangle, tetra, triangle6, tetra10, quad, quad9, hexahedron, hexahedron27):
 
 * ``from_mesh(mesh, quadrature_order=2, project='reduce', *args, **kwargs)``
   builds the CSR pattern and the slot permutation once (libtmhip host
@@ -86,7 +86,7 @@ class _AssemblerBase(nn.Module):
             raise NotImplementedError("tensormesh_amd assembles one element type per mesh")
         etype = next(iter(elements))
         if etype not in ELEMENT_SHAPE:
-            raise NotImplementedError(f"element type {etype} is not supported (triangle, tetra, triangle6, tetra10)")
+            raise NotImplementedError(f"element type {etype} is not supported ({', '.join(ELEMENT_SHAPE)})")
         self.element_types = [etype]
         self.elements = {etype: elements[etype].long()}
         self.dimension = ELEMENT_SHAPE[etype][0]
@@ -334,7 +334,9 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
                             None, float(young), float(poisson),
                             nat.TM_FLAG_MFMA if (getattr(self, "use_mfma", False) and dtype == torch.float32) else 0)
-        use_rows = getattr(self, "nh_path", "slots") == "rows"
+        # the block-row reduction is built for simplex cells; other cells use the slot path
+        use_rows = getattr(self, "nh_path", "slots") == "rows" and self.etype in ("triangle", "tetra", "triangle6",
+                                                                                  "tetra10")
         brv = self._block_rows(pat, conn, dev, D) if use_rows else None
         if brv is not None:
             tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None

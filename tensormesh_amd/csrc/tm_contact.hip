@@ -140,9 +140,10 @@ int launch_contact(const tm_mesh_view* m, const tm_facet_view* fv, const double*
                    const void* x, double* e_out, void* rslot, void* tslot, hipStream_t s) {
   const int64_t grid = (fv->n_sel + kBlock - 1) / kBlock;
   if (grid == 0) return TM_OK;
+  const int nF = tmh::cell_facets(m->cell);
   contact_penalty_kernel<T, D, B><<<grid, kBlock, 0, s>>>(m->conn, static_cast<const T*>(m->points),
                                                          static_cast<const T*>(x), fv->elem, fv->local, fv->n_sel,
-                                                         tab, Qf, D + 1, pl, e_out, static_cast<T*>(rslot),
+                                                         tab, Qf, nF, pl, e_out, static_cast<T*>(rslot),
                                                          static_cast<T*>(tslot));
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
@@ -156,10 +157,10 @@ template <typename T>
 int dispatch_contact(const tm_mesh_view* m, const tm_facet_view* fv, const double* tab, int Qf, const Plane& pl,
                      const void* x, double* e_out, void* rslot, void* tslot, hipStream_t s) {
   switch (m->cell) {
-    case TM_TRI3: return launch_contact<T, 2, 3>(m, fv, tab, Qf, pl, x, e_out, rslot, tslot, s);
-    case TM_TRI6: return launch_contact<T, 2, 6>(m, fv, tab, Qf, pl, x, e_out, rslot, tslot, s);
-    case TM_TET4: return launch_contact<T, 3, 4>(m, fv, tab, Qf, pl, x, e_out, rslot, tslot, s);
-    case TM_TET10: return launch_contact<T, 3, 10>(m, fv, tab, Qf, pl, x, e_out, rslot, tslot, s);
+#define TM_CASE(C, D, B) \
+  case C: return launch_contact<T, D, B>(m, fv, tab, Qf, pl, x, e_out, rslot, tslot, s);
+    TM_ALL_CELLS(TM_CASE)
+#undef TM_CASE
     default: return tmh::fail(TM_ERR_UNSUPPORTED, "cell type");
   }
 }

@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, int D, int B>
 int launch_facet(const tm_mesh_view* m, const tm_facet_view* fv, const double* tab, int Qf, int H, const void* data,
                  const double* cval, void* slot, hipStream_t s) {
-  constexpr int nF = D + 1;
+  const int nF = tmh::cell_facets(m->cell);
   const int64_t grid = (fv->n_sel + kBlock - 1) / kBlock;
   if (grid == 0) return TM_OK;
   facet_load_kernel<T, D, B><<<grid, kBlock, 0, s>>>(m->conn, static_cast<const T*>(m->points), fv->elem,
@@ -126,10 +126,10 @@ template <typename T>
 int dispatch_facet(const tm_mesh_view* m, const tm_facet_view* fv, const double* tab, int Qf, int H,
                    const void* data, const double* cval, void* slot, hipStream_t s) {
   switch (m->cell) {
-    case TM_TRI3: return launch_facet<T, 2, 3>(m, fv, tab, Qf, H, data, cval, slot, s);
-    case TM_TRI6: return launch_facet<T, 2, 6>(m, fv, tab, Qf, H, data, cval, slot, s);
-    case TM_TET4: return launch_facet<T, 3, 4>(m, fv, tab, Qf, H, data, cval, slot, s);
-    case TM_TET10: return launch_facet<T, 3, 10>(m, fv, tab, Qf, H, data, cval, slot, s);
+#define TM_CASE(C, D, B) \
+  case C: return launch_facet<T, D, B>(m, fv, tab, Qf, H, data, cval, slot, s);
+    TM_ALL_CELLS(TM_CASE)
+#undef TM_CASE
     default: return tmh::fail(TM_ERR_UNSUPPORTED, "cell type");
   }
 }

@@ -18,20 +18,36 @@ inline int fail(int status, const char* what) {
   return status;
 }
 
+// Every cell type as X(enum, D, B), for switch statements over a cell code.
+#define TM_SIMPLEX_CELLS(X) X(TM_TRI3, 2, 3) X(TM_TET4, 3, 4) X(TM_TRI6, 2, 6) X(TM_TET10, 3, 10)
+#define TM_TENSOR_CELLS(X) X(TM_QUAD4, 2, 4) X(TM_QUAD9, 2, 9) X(TM_HEX8, 3, 8) X(TM_HEX27, 3, 27)
+#define TM_ALL_CELLS(X) TM_SIMPLEX_CELLS(X) TM_TENSOR_CELLS(X)
+
 inline int cell_basis(int cell) {
   switch (cell) {
-    case TM_TRI3: return 3;
-    case TM_TET4: return 4;
-    case TM_TRI6: return 6;
-    case TM_TET10: return 10;
+#define TM_CB(C, D, B) case C: return B;
+    TM_ALL_CELLS(TM_CB)
+#undef TM_CB
     default: return -1;
   }
 }
 
 inline int cell_dim(int cell) {
   switch (cell) {
-    case TM_TRI3: case TM_TRI6: return 2;
-    case TM_TET4: case TM_TET10: return 3;
+#define TM_CD(C, D, B) case C: return D;
+    TM_ALL_CELLS(TM_CD)
+#undef TM_CD
+    default: return -1;
+  }
+}
+
+// Facets per cell: triangle 3 edges, tetrahedron 4 faces, quadrilateral 4
+// edges, hexahedron 6 faces.
+inline int cell_facets(int cell) {
+  switch (cell) {
+    case TM_TRI3: case TM_TRI6: return 3;
+    case TM_TET4: case TM_TET10: case TM_QUAD4: case TM_QUAD9: return 4;
+    case TM_HEX8: case TM_HEX27: return 6;
     default: return -1;
   }
 }

@@ -62,6 +62,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   __shared__ int s_v[kWavesPerBlock][B];
   __shared__ T s_out[kWavesPerBlock][B * kRowVals];
 
+  static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
   const int lane = threadIdx.x % kWave;
   const int wv = threadIdx.x / kWave;
   const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
@@ -72,10 +73,12 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   const double* dphi = s_tab + Q + Q * B;
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
   __syncthreads();
-  if (active && lane < B * D) {
-    const int b = lane / D, d = lane % D;
-    s_x[wv][lane] = (double)pts[(int64_t)s_v[wv][b] * D + d];
-    s_u[wv][lane] = (double)disp[(int64_t)s_v[wv][b] * D + d];
+  if (active) {
+    for (int t = lane; t < B * D; t += kWave) {
+      const int b = t / D, d = t % D;
+      s_x[wv][t] = (double)pts[(int64_t)s_v[wv][b] * D + d];
+      s_u[wv][t] = (double)disp[(int64_t)s_v[wv][b] * D + d];
+    }
   }
   __syncthreads();
   if (active && lane < Q) {
@@ -313,6 +316,11 @@ int dispatch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k,
     case TM_TET4: return launch_j2<T, 3, 4>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     case TM_TRI6: return launch_j2<T, 2, 6>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     case TM_TET10: return launch_j2<T, 3, 10>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_QUAD4: return launch_j2<T, 2, 4>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_QUAD9: return launch_j2<T, 2, 9>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_HEX8: return launch_j2<T, 3, 8>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_HEX27:
+      return tmh::fail(TM_ERR_UNSUPPORTED, "hexahedron27: the 81 x 81 element tangent does not fit the kernel's LDS");
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }

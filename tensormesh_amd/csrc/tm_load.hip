@@ -93,10 +93,10 @@ int launch(const tm_mesh_view* m, const tm_form_desc* f, const double* cval, voi
 template <typename T>
 int dispatch(const tm_mesh_view* m, const tm_form_desc* f, const double* cval, void* slot_ws, hipStream_t s) {
   switch (m->cell) {
-    case TM_TRI3: return launch<T, 2, 3>(m, f, cval, slot_ws, s);
-    case TM_TET4: return launch<T, 3, 4>(m, f, cval, slot_ws, s);
-    case TM_TRI6: return launch<T, 2, 6>(m, f, cval, slot_ws, s);
-    case TM_TET10: return launch<T, 3, 10>(m, f, cval, slot_ws, s);
+#define TM_CASE(C, D, B) \
+  case C: return launch<T, D, B>(m, f, cval, slot_ws, s);
+    TM_ALL_CELLS(TM_CASE)
+#undef TM_CASE
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }

@@ -30,22 +30,24 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 4;
-constexpr int kMaxQ = 16;
-constexpr int kMaxTab = kMaxQ + kMaxQ * 10 + kMaxQ * 3 * 10;
+constexpr int kMaxQ = 16;      // Neo-Hookean kernel
+constexpr int kMaxQWave = 32;  // element-matrix kernel (a 3 x 3 x 3 hexahedron rule has 27 points)
 
-template <typename T, int D, int B, int H, int FORM>
+// MQ: largest number of quadrature points the LDS arrays hold.
+template <typename T, int D, int B, int H, int FORM, int MQ>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     wave_element_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                         const T* __restrict__ coeff, const double* __restrict__ tab, int Q,
                         double lam, double mu, int64_t n_elem, T* __restrict__ slot) {
-  __shared__ double s_tab[kMaxTab];
+  __shared__ double s_tab[MQ * (1 + B + D * B)];
   __shared__ double s_x[kWavesPerBlock][B * D];
-  __shared__ double s_inv[kWavesPerBlock][kMaxQ][D * D];
-  __shared__ double s_jxw[kWavesPerBlock][kMaxQ];
-  __shared__ double s_kq[kWavesPerBlock][kMaxQ];
-  __shared__ double s_g[kWavesPerBlock][kMaxQ * B * D];
+  __shared__ double s_inv[kWavesPerBlock][MQ][D * D];
+  __shared__ double s_jxw[kWavesPerBlock][MQ];
+  __shared__ double s_kq[kWavesPerBlock][MQ];
+  __shared__ double s_g[kWavesPerBlock][MQ * B * D];
   __shared__ int s_v[kWavesPerBlock][B];
 
+  static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
   const int lane = threadIdx.x % kWave;
   const int wv = threadIdx.x / kWave;
   const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
@@ -61,9 +63,11 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     if (lane < B) s_v[wv][lane] = conn[e * B + lane];
   }
   __syncthreads();
-  if (active && lane < B * D) {
-    const int b = lane / D, d = lane % D;
-    s_x[wv][lane] = (double)pts[(int64_t)s_v[wv][b] * D + d];
+  if (active) {
+    for (int t = lane; t < B * D; t += kWave) {
+      const int b = t / D, d = t % D;
+      s_x[wv][t] = (double)pts[(int64_t)s_v[wv][b] * D + d];
+    }
   }
   __syncthreads();
   if (active && lane < Q) {
@@ -217,6 +221,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   __shared__ int s_v[kWavesPerBlock][B];
   __shared__ T s_out[kWavesPerBlock][B * kRowVals];
 
+  static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
   const int lane = threadIdx.x % kWave;
   const int wv = threadIdx.x / kWave;
   const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
@@ -227,10 +232,12 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   const double* dphi = s_tab + Q + Q * B;
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
   __syncthreads();
-  if (active && lane < B * D) {
-    const int b = lane / D, d = lane % D;
-    s_x[wv][lane] = (double)pts[(int64_t)s_v[wv][b] * D + d];
-    s_u[wv][lane] = (double)disp[(int64_t)s_v[wv][b] * D + d];
+  if (active) {
+    for (int t = lane; t < B * D; t += kWave) {
+      const int b = t / D, d = t % D;
+      s_x[wv][t] = (double)pts[(int64_t)s_v[wv][b] * D + d];
+      s_u[wv][t] = (double)disp[(int64_t)s_v[wv][b] * D + d];
+    }
   }
   __syncthreads();
   if (active && lane < Q) {
@@ -507,16 +514,18 @@ int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* dis
     case TM_TET4: return launch_hyper<T, 3, 4>(m, f, disp, tslot, rslot, inc_inv, s);
     case TM_TRI6: return launch_hyper<T, 2, 6>(m, f, disp, tslot, rslot, inc_inv, s);
     case TM_TET10: return launch_hyper<T, 3, 10>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_QUAD4: return launch_hyper<T, 2, 4>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_QUAD9: return launch_hyper<T, 2, 9>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_HEX8: return launch_hyper<T, 3, 8>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_HEX27:
+      return tmh::fail(TM_ERR_UNSUPPORTED, "hexahedron27: the 81 x 81 element tangent does not fit the kernel's LDS");
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }
 
-template <typename T, int D, int B>
-int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream,
-                           int* hh_out) {
+template <typename T, int D, int B, int MQ>
+int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream, int* hh_out) {
   const int Q = f->n_quad;
-  if (Q <= 0 || Q > kMaxQ) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
-  if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   const T* pts = static_cast<const T*>(m->points);
   const T* coeff = static_cast<const T*>(f->coeff);
   T* slot = static_cast<T*>(slot_ws);
@@ -525,30 +534,34 @@ int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* s
   const dim3 block(kWave * kWavesPerBlock);
   double lam = 0.0, mu = 0.0;
   *hh_out = 1;
+#define TM_WAVE(H, FORM)                                                                                       \
+  wave_element_kernel<T, D, B, H, FORM, MQ><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, \
+                                                                        mu, m->n_elem, slot)
   switch (f->form) {
     case TM_FORM_LAPLACE:
-      wave_element_kernel<T, D, B, 1, TM_FORM_LAPLACE><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, slot);
+      TM_WAVE(1, TM_FORM_LAPLACE);
       break;
     case TM_FORM_DIFFUSION_ELEM:
       if (!coeff) return tmh::fail(TM_ERR_ARG, "diffusion form needs a coefficient");
-      wave_element_kernel<T, D, B, 1, TM_FORM_DIFFUSION_ELEM><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, slot);
+      TM_WAVE(1, TM_FORM_DIFFUSION_ELEM);
       break;
     case TM_FORM_DIFFUSION_NODE:
       if (!coeff) return tmh::fail(TM_ERR_ARG, "diffusion form needs a coefficient");
-      wave_element_kernel<T, D, B, 1, TM_FORM_DIFFUSION_NODE><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, slot);
+      TM_WAVE(1, TM_FORM_DIFFUSION_NODE);
       break;
     case TM_FORM_MASS:
-      wave_element_kernel<T, D, B, 1, TM_FORM_MASS><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, slot);
+      TM_WAVE(1, TM_FORM_MASS);
       break;
     case TM_FORM_LINEAR_ELASTICITY:
       mu = f->young / (2.0 * (1.0 + f->poisson));
       lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
       *hh_out = D * D;
-      wave_element_kernel<T, D, B, D, TM_FORM_LINEAR_ELASTICITY><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, slot);
+      TM_WAVE(D, TM_FORM_LINEAR_ELASTICITY);
       break;
     default:
       return tmh::fail(TM_ERR_UNSUPPORTED, "unknown form");
   }
+#undef TM_WAVE
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     tmh::set_error("element kernel launch: %s", hipGetErrorString(err));
@@ -557,13 +570,24 @@ int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* s
   return TM_OK;
 }
 
+template <typename T, int D, int B>
+int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream,
+                           int* hh_out) {
+  const int Q = f->n_quad;
+  if (Q <= 0 || Q > kMaxQWave) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
+  if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
+  if (Q <= 4) return launch_wave<T, D, B, 4>(m, f, slot_ws, stream, hh_out);
+  if (Q <= 16) return launch_wave<T, D, B, 16>(m, f, slot_ws, stream, hh_out);
+  return launch_wave<T, D, B, kMaxQWave>(m, f, slot_ws, stream, hh_out);
+}
+
 template <typename T>
 int dispatch_cell(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream, int* hh) {
   switch (m->cell) {
-    case TM_TRI3: return launch_scalar_or_block<T, 2, 3>(m, f, slot_ws, stream, hh);
-    case TM_TET4: return launch_scalar_or_block<T, 3, 4>(m, f, slot_ws, stream, hh);
-    case TM_TRI6: return launch_scalar_or_block<T, 2, 6>(m, f, slot_ws, stream, hh);
-    case TM_TET10: return launch_scalar_or_block<T, 3, 10>(m, f, slot_ws, stream, hh);
+#define TM_CASE(C, D, B) \
+  case C: return launch_scalar_or_block<T, D, B>(m, f, slot_ws, stream, hh);
+    TM_ALL_CELLS(TM_CASE)
+#undef TM_CASE
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }

@@ -259,7 +259,7 @@ extern "C" int tm_pattern_device_count(const int32_t* conn, int64_t n_elem, int3
   TM_TRY_BEGIN
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
   if (!conn || !inc_ptr || !inc || !row_ptr || !work) return tmh::fail(TM_ERR_ARG, "NULL argument");
-  if (n_elem < 0 || n_basis <= 0 || n_basis > 10 || n_points <= 0) return tmh::fail(TM_ERR_ARG, "bad sizes");
+  if (n_elem < 0 || n_basis <= 0 || n_basis > 27 || n_points <= 0) return tmh::fail(TM_ERR_ARG, "bad sizes");
   if (n_elem * n_basis * n_basis >= tmh::kMaxIndex || n_points >= tmh::kMaxIndex)
     return tmh::fail(TM_ERR_SIZE, "pattern exceeds int32 indexing");
   const int64_t total = n_elem * n_basis;
@@ -288,7 +288,7 @@ extern "C" int tm_pattern_device_fill(const int32_t* conn, int64_t n_elem, int32
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
   if (!conn || !inc_ptr || !inc || !row_ptr || !col_idx || !seg_ptr || !seg_slot || !slot2nnz)
     return tmh::fail(TM_ERR_ARG, "NULL argument");
-  if (n_basis <= 0 || n_basis > 10 || n_points <= 0) return tmh::fail(TM_ERR_ARG, "bad sizes");
+  if (n_basis <= 0 || n_basis > 27 || n_points <= 0) return tmh::fail(TM_ERR_ARG, "bad sizes");
   row_fill_kernel<<<n_points, kWave, 0, s>>>(conn, n_basis, inc_ptr, inc, row_ptr, n_points, col_idx, seg_ptr,
                                              seg_slot, slot2nnz, n_basis <= 4 ? colpos : nullptr,
                                              n_basis <= 4 ? entry_mask : nullptr);

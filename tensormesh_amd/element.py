@@ -101,23 +101,110 @@ _GL = {
 }
 
 ELEMENT_SHAPE = {
-    # element type: (dim, order, n_basis, simplex family)
+    # element type: (dim, order, n_basis, family)
     "triangle": (2, 1, 3, "tri"),
     "triangle6": (2, 2, 6, "tri"),
     "tetra": (3, 1, 4, "tet"),
     "tetra10": (3, 2, 10, "tet"),
+    "quad": (2, 1, 4, "quad"),
+    "quad9": (2, 2, 9, "quad"),
+    "hexahedron": (3, 1, 8, "hex"),
+    "hexahedron27": (3, 2, 27, "hex"),
 }
+TENSOR_FAMILIES = ("quad", "hex")
+
+
+def _tensor_rule(dim: int, order: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Tensor product of the ``order``-point Gauss-Legendre rule on [0, 1] in
+    the point order of quad_quadrature / hex_quadrature (quadrature.py:163-239,
+    torch.meshgrid(..., indexing='xy')): quadrilateral x fastest, then y;
+    hexahedron z fastest, then x, then y."""
+    w1, p1 = line_quadrature(order)
+    p1 = p1[:, 0]
+    n = p1.size
+    if dim == 2:
+        iy, ix = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        iy, ix = iy.ravel(), ix.ravel()
+        return w1[ix] * w1[iy], np.stack([p1[ix], p1[iy]], 1)
+    iy, ix, iz = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    iy, ix, iz = iy.ravel(), ix.ravel(), iz.ravel()
+    return w1[ix] * w1[iy] * w1[iz], np.stack([p1[ix], p1[iy], p1[iz]], 1)
 
 
 def quadrature(element_type: str, order: int) -> Tuple[np.ndarray, np.ndarray]:
-    """(weights[Q], points[Q, D]) as float64 numpy arrays."""
-    fam = ELEMENT_SHAPE[element_type][3]
+    """(weights[Q], points[Q, D]) as float64 numpy arrays.  For quadrilaterals
+    and hexahedra ``order`` is the number of Gauss points per axis (1-7)."""
+    dim, _, _, fam = ELEMENT_SHAPE[element_type]
+    if fam in TENSOR_FAMILIES:
+        if order not in _GL:
+            raise NotImplementedError(f"quadrature_order={order} for {element_type}: supported orders are 1-7")
+        return _tensor_rule(dim, order)
     table = TRI_QUAD if fam == "tri" else TET_QUAD
     if order not in table:
         raise NotImplementedError(
             f"quadrature_order={order} for {element_type}: supported orders are {sorted(table)}")
     w, p = table[order]
     return np.asarray(w, dtype=np.float64), np.asarray(p, dtype=np.float64)
+
+
+# Tensor-product cells on [0, 1]^D (element.py:1811-1814, 2100-2104): vertices in
+# lexicographic order (x fastest), edges and faces as the reference lists them.
+_QUAD_EDGES = ((0, 1), (0, 2), (1, 3), (2, 3))
+_HEX_EDGES = ((0, 1), (0, 2), (0, 4), (1, 3), (1, 5), (2, 3), (2, 6), (3, 7), (4, 5), (4, 6), (5, 7), (6, 7))
+_HEX_FACES = ((0, 1, 2, 3), (0, 1, 4, 5), (0, 2, 4, 6), (1, 3, 5, 7), (2, 3, 6, 7), (4, 5, 6, 7))
+
+
+@functools.lru_cache(maxsize=None)
+def tensor_nodes(element_type: str) -> np.ndarray:
+    """Lagrange nodes [B, D] of a tensor-product cell in the reference's basis
+    order (basis.py:697-748, 861-905): vertices, edge nodes (edge order), face
+    nodes (face order), cell node; for order 2 these are midpoints/centres."""
+    dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    verts = np.array([[(v >> d) & 1 for d in range(dim)] for v in range(2 ** dim)], np.float64)
+    if order == 1:
+        return verts
+    edges = _QUAD_EDGES if fam == "quad" else _HEX_EDGES
+    nodes = [verts] + [0.5 * (verts[a] + verts[b])[None] for a, b in edges]
+    faces = ((0, 1, 2, 3),) if fam == "quad" else _HEX_FACES
+    nodes += [verts[list(f)].mean(0)[None] for f in faces]
+    if fam == "hex":
+        nodes.append(np.full((1, 3), 0.5))
+    return np.concatenate(nodes, 0)
+
+
+def _lagrange_1d(t: np.ndarray, order: int):
+    """Values and derivatives [len(t), order+1] of the 1-D Lagrange polynomials
+    on the nodes k / order."""
+    nodes = np.arange(order + 1) / order
+    val = np.ones((t.size, order + 1))
+    der = np.zeros((t.size, order + 1))
+    for k in range(order + 1):
+        others = [m for m in range(order + 1) if m != k]
+        for m in others:
+            val[:, k] *= (t - nodes[m]) / (nodes[k] - nodes[m])
+        for m in others:
+            term = np.ones(t.size) / (nodes[k] - nodes[m])
+            for m2 in others:
+                if m2 != m:
+                    term = term * (t - nodes[m2]) / (nodes[k] - nodes[m2])
+            der[:, k] += term
+    return val, der
+
+
+def _tensor_shape(element_type: str, points: np.ndarray):
+    """phi[Q, B], dphi[Q, D, B] of a tensor-product cell: node n at grid index
+    (k_0, .., k_{D-1}) has phi_n = prod_d L_{k_d}(xi_d)."""
+    dim, order, nb, _ = ELEMENT_SHAPE[element_type]
+    idx = np.rint(tensor_nodes(element_type) * order).astype(np.int64)      # [B, D]
+    vals, ders = zip(*[_lagrange_1d(points[:, d], order) for d in range(dim)])
+    phi = np.ones((points.shape[0], nb))
+    for d in range(dim):
+        phi *= vals[d][:, idx[:, d]]
+    dphi = np.ones((points.shape[0], dim, nb))
+    for g in range(dim):
+        for d in range(dim):
+            dphi[:, g] *= (ders[d] if d == g else vals[d])[:, idx[:, d]]
+    return phi, dphi
 
 
 def _barycentric(points: np.ndarray) -> np.ndarray:
@@ -127,6 +214,8 @@ def _barycentric(points: np.ndarray) -> np.ndarray:
 def shape_values(element_type: str, points: np.ndarray) -> np.ndarray:
     """phi[Q, B] at reference points."""
     dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    if fam in TENSOR_FAMILIES:
+        return _tensor_shape(element_type, points)[0]
     lam = _barycentric(points)
     if order == 1:
         return lam.copy()
@@ -140,6 +229,8 @@ def shape_ref_grads(element_type: str, points: np.ndarray) -> np.ndarray:
     """dphi/dxi [Q, D, B] at reference points (layout of the reference's
     ``Polynomials.map`` output, tensormesh/element/element.py:960)."""
     dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    if fam in TENSOR_FAMILIES:
+        return _tensor_shape(element_type, points)[1]
     q = points.shape[0]
     lam = _barycentric(points)
     # d lambda_k / d xi_i : lambda_0 = 1 - sum(xi), lambda_k = xi_{k-1}
@@ -199,20 +290,29 @@ def line_quadrature(order: int) -> Tuple[np.ndarray, np.ndarray]:
 
 
 _REF_VERTS = {"tri": np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]),
-              "tet": np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])}
-_FACET_VERTS = {"tri": ((1, 2), (0, 2), (0, 1)), "tet": ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))}
-_EDGES = {"tri": ((1, 2), (0, 2), (0, 1)), "tet": ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))}
+              "tet": np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]),
+              "quad": np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]),
+              "hex": np.array([[(v >> d) & 1 for d in range(3)] for v in range(8)], np.float64)}
+# facets: triangle / quadrilateral edges, tetrahedron / hexahedron faces
+_FACET_VERTS = {"tri": ((1, 2), (0, 2), (0, 1)), "tet": ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)),
+                "quad": _QUAD_EDGES, "hex": _HEX_FACES}
+_EDGES = {"tri": ((1, 2), (0, 2), (0, 1)), "tet": ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1)),
+          "quad": _QUAD_EDGES, "hex": _HEX_EDGES}
 
 
 def facet_nodes(element_type: str) -> np.ndarray:
     """[n_facet, n_node_per_facet] local nodes of each facet: its vertices,
-    then (P2) the midpoint nodes of the element edges lying on it."""
+    then (order 2) the midpoint nodes of the element edges lying on it and, for
+    hexahedron faces, the face's centre node (basis.py:1127-1287)."""
     dim, order, nb, fam = ELEMENT_SHAPE[element_type]
+    n_vert = _REF_VERTS[fam].shape[0]
     out = []
-    for fv in _FACET_VERTS[fam]:
+    for fi, fv in enumerate(_FACET_VERTS[fam]):
         nodes = list(fv)
         if order == 2:
-            nodes += [dim + 1 + m for m, (a, b) in enumerate(_EDGES[fam]) if a in fv and b in fv]
+            nodes += [n_vert + m for m, (a, b) in enumerate(_EDGES[fam]) if a in fv and b in fv]
+            if fam == "hex":
+                nodes.append(n_vert + len(_HEX_EDGES) + fi)
         out.append(nodes)
     return np.asarray(out, np.int64)
 
@@ -223,10 +323,12 @@ def facet_tables(element_type: str, qorder: int):
     (w[Qf], phi[nF, Qf, B], dphi[nF, Qf, D, B], dgamma[nF, D-1, D]) with
     dgamma[f][k] = d gamma_f / d r_k (the facet's reference tangents)."""
     dim, order, nb, fam = ELEMENT_SHAPE[element_type]
-    if fam == "tri":
-        w, r = line_quadrature(qorder)
+    if fam in ("tri", "quad"):
+        w, r = line_quadrature(qorder)                    # facet_quadrature_2d
+    elif fam == "tet":
+        w, r = quadrature("triangle", qorder)             # tet_facet_quadrature
     else:
-        w, r = quadrature("triangle", qorder)
+        w, r = quadrature("quad", qorder)                 # hex_facet_quadrature
     P = _REF_VERTS[fam]
     phi, dphi, dgam = [], [], []
     for fv in _FACET_VERTS[fam]:

@@ -43,6 +43,10 @@ ELEMENT_INFO = {
     "tetra": (3, 1, 4),
     "triangle6": (2, 2, 6),
     "tetra10": (3, 2, 10),
+    "quad": (2, 1, 4),
+    "quad9": (2, 2, 9),
+    "hexahedron": (3, 1, 8),
+    "hexahedron27": (3, 2, 27),
 }
 
 
@@ -211,6 +215,61 @@ def quadratic_arrays(points: np.ndarray, cells: np.ndarray, boundary: np.ndarray
     return new_points, new_cells, new_boundary
 
 
+def _tensor_grid_arrays(dim: int, n: int, order: int, jitter: float, seed: Optional[int]):
+    """Structured n^dim grid of quadrilaterals / hexahedra on [0, 1]^dim.
+
+    The element vertices are jittered (interior ones only); order-2 nodes sit
+    on the (2n+1)^dim grid at the averages of the surrounding vertices, so
+    edges stay straight.  Node ids are x-fastest on that grid; each cell lists
+    its nodes in the reference order (vertices lexicographic, then edge, face
+    and cell nodes)."""
+    from .element import tensor_nodes
+    h = 1.0 / n
+    m = n + 1
+    grids = np.meshgrid(*[np.arange(m)] * dim, indexing="ij")[::-1]        # x-fastest numbering
+    idx = [g.ravel() for g in grids]
+    coarse = np.stack([i * h for i in idx], 1).astype(np.float64)
+    cb = np.zeros(coarse.shape[0], bool)
+    for i in idx:
+        cb |= (i == 0) | (i == n)
+    coarse = _jitter(coarse, ~cb, jitter * h, seed).reshape([m] * dim + [dim])   # [z][y][x] / [y][x]
+    p = order
+    f = p * n + 1
+    fgrids = np.meshgrid(*[np.arange(f)] * dim, indexing="ij")[::-1]
+    fidx = [g.ravel() for g in fgrids]                                       # [x, y(, z)] fine indices
+    pts = np.zeros((fidx[0].size, dim))
+    corners = 0
+    for bits in itertools.product((0, 1), repeat=dim):
+        cidx = [(fi + (p - 1) * b) // p for fi, b in zip(fidx, bits)]       # floor / ceil of fi / p
+        pts += coarse[tuple(cidx[::-1])]
+        corners += 1
+    pts /= corners
+    boundary = np.zeros(pts.shape[0], bool)
+    for fi in fidx:
+        boundary |= (fi == 0) | (fi == f - 1)
+    etype = {(2, 1): "quad", (2, 2): "quad9", (3, 1): "hexahedron", (3, 2): "hexahedron27"}[(dim, order)]
+    off = np.rint(tensor_nodes(etype) * p).astype(np.int64)                 # [B, dim]
+    cgrids = np.meshgrid(*[np.arange(n)] * dim, indexing="ij")[::-1]
+    cbase = [g.ravel() * p for g in cgrids]
+    stride = [f ** d for d in range(dim)]
+    cells = np.zeros((cbase[0].size, off.shape[0]), np.int64)
+    for d in range(dim):
+        cells += (cbase[d][:, None] + off[None, :, d]) * stride[d]
+    return pts, cells, boundary, etype
+
+
+def unit_square_quad_arrays(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order: int = 1):
+    """numpy (points, cells, boundary) of an n x n quadrilateral grid (quad / quad9)."""
+    pts, cells, bnd, _ = _tensor_grid_arrays(2, n, order, jitter, seed)
+    return pts, cells, bnd
+
+
+def unit_cube_hex_arrays(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order: int = 1):
+    """numpy (points, cells, boundary) of an n^3 hexahedral grid (hexahedron / hexahedron27)."""
+    pts, cells, bnd, _ = _tensor_grid_arrays(3, n, order, jitter, seed)
+    return pts, cells, bnd
+
+
 def _to_mesh(pts, cells, boundary, etype, dtype, device) -> Mesh:
     return Mesh(torch.from_numpy(pts).to(dtype=dtype, device=device),
                 {etype: torch.from_numpy(cells).to(device)},
@@ -234,6 +293,18 @@ def unit_cube_tet(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order: i
     if order == 2:
         pts, cells, bnd = quadratic_arrays(pts, cells, bnd)
         etype = "tetra10"
+    return _to_mesh(pts, cells, bnd, etype, dtype, device)
+
+
+def unit_square_quad(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order: int = 1,
+                     dtype: torch.dtype = torch.float64, device="cpu") -> Mesh:
+    pts, cells, bnd, etype = _tensor_grid_arrays(2, n, order, jitter, seed)
+    return _to_mesh(pts, cells, bnd, etype, dtype, device)
+
+
+def unit_cube_hex(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order: int = 1,
+                  dtype: torch.dtype = torch.float64, device="cpu") -> Mesh:
+    pts, cells, bnd, etype = _tensor_grid_arrays(3, n, order, jitter, seed)
     return _to_mesh(pts, cells, bnd, etype, dtype, device)
 
 

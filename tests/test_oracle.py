@@ -588,3 +588,115 @@ def test_contact_finite_differences():
         rm = fem.contact_facet_derivatives(P, x - dx, C, BM, "triangle6", 3, 5.0, nrm, 0.05)[0]
         fd_h = (rp - rm)[fi] / (2 * hstep)
         assert torch.allclose(fd_h, hes[fi, :, b, :, a], atol=1e-6 * hes[fi].abs().max().item())
+
+
+# ------------------------------------------- quadrilaterals and hexahedra ---
+_TENSOR = ["quad", "quad9", "hexahedron", "hexahedron27"]
+
+
+@pytest.mark.parametrize("etype", ["quad", "hexahedron"])
+def test_tensor_rule_exact_on_monomials(etype):
+    """The n-point-per-axis rule integrates x^a y^b (z^c) exactly on [0, 1]^D
+    for every exponent up to 2n - 1."""
+    dim = 2 if etype == "quad" else 3
+    for n in (1, 2, 3, 4):
+        w, p = fem.quadrature(etype, n)
+        assert w.numel() == n ** dim and abs(w.sum().item() - 1.0) < 1e-14
+        for e in itertools.product(range(2 * n), repeat=dim):
+            got = (w * torch.prod(p ** torch.tensor(e, dtype=torch.float64), 1)).sum().item()
+            want = 1.0 / np.prod([k + 1 for k in e])
+            assert abs(got - want) < 1e-13, (n, e)
+
+
+@pytest.mark.parametrize("etype", _TENSOR)
+def test_tensor_basis_kronecker_and_partition(etype):
+    nodes = fem.lagrange_nodes(etype)
+    assert np.allclose(fem.shape_values(etype, nodes), np.eye(nodes.shape[0]), atol=1e-12)
+    pts = np.random.default_rng(0).random((7, nodes.shape[1]))
+    assert np.allclose(fem.shape_values(etype, pts).sum(1), 1.0, atol=1e-12)
+    assert np.allclose(fem.shape_ref_grads(etype, pts).sum(2), 0.0, atol=1e-11)
+    # the product code's closed-form tensor Lagrange basis agrees
+    from tensormesh_amd import element as el
+    assert np.allclose(el.shape_values(etype, pts), fem.shape_values(etype, pts), atol=1e-12)
+    assert np.allclose(el.shape_ref_grads(etype, pts), fem.shape_ref_grads(etype, pts), atol=1e-11)
+    assert np.array_equal(el.tensor_nodes(etype), nodes)
+
+
+@pytest.mark.parametrize("etype", ["quad", "hexahedron"])
+def test_unit_cell_q1_matrices_closed_form(etype):
+    """Q1 stiffness and mass of the unit cell as Kronecker products of the
+    1-D element matrices k = [[1,-1],[-1,1]], m = [[1/3,1/6],[1/6,1/3]]
+    (x fastest in the lexicographic node order)."""
+    dim = 2 if etype == "quad" else 3
+    P = torch.tensor(fem.lagrange_nodes(etype))
+    C = torch.arange(2 ** dim)[None]
+    k = np.array([[1.0, -1.0], [-1.0, 1.0]])
+    m = np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+    if dim == 2:
+        K = np.kron(m, k) + np.kron(k, m)
+        M = np.kron(m, m)
+    else:
+        K = np.kron(np.kron(m, m), k) + np.kron(np.kron(m, k), m) + np.kron(np.kron(k, m), m)
+        M = np.kron(np.kron(m, m), m)
+    got_k = fem.element_integrals(P, C, etype, 2, "laplace")[0].numpy()
+    got_m = fem.element_integrals(P, C, etype, 2, "mass")[0].numpy()
+    assert np.allclose(got_k, K, atol=1e-14) and np.allclose(got_m, M, atol=1e-15)
+
+
+@pytest.mark.parametrize("etype,qo", [("quad", 2), ("quad9", 3), ("hexahedron", 2), ("hexahedron27", 3)])
+def test_tensor_patch_test_and_rigid_modes(etype, qo):
+    """On jittered grids: the assembled Laplace matrix annihilates a linear
+    field in every interior row (the rule is exact for adj(J)^T grad phi),
+    and the element elasticity matrices annihilate rigid-body motions."""
+    from tensormesh_amd.mesh import unit_cube_hex_arrays, unit_square_quad_arrays
+    order = 2 if etype in ("quad9", "hexahedron27") else 1
+    if etype.startswith("quad"):
+        p, c, bm = unit_square_quad_arrays(3, 0.2, 5, order=order)
+    else:
+        p, c, bm = unit_cube_hex_arrays(2, 0.2, 5, order=order)
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    Ke = fem.element_integrals(P, C, etype, qo, "laplace")
+    row, col, eids = fem.pattern(c, p.shape[0])
+    vals = fem._index_add(Ke.reshape(-1), torch.from_numpy(eids), row.size)
+    K = _dense(vals, torch.from_numpy(row), torch.from_numpy(col), (p.shape[0], p.shape[0]))
+    lin = P @ torch.arange(1.0, P.shape[1] + 1, dtype=torch.float64) + 0.5
+    r = K @ lin
+    assert r[~torch.from_numpy(bm)].abs().max() < 1e-12
+    Ee = fem.element_integrals(P, C, etype, qo, "linear_elasticity", E=1.0, nu=0.3)   # [E,B,B,D,D]
+    D, B = P.shape[1], C.shape[1]
+    KE = Ee.permute(0, 1, 3, 2, 4).reshape(C.shape[0], B * D, B * D)
+    X = P[C]                                                                  # [E,B,D]
+    modes = [torch.zeros_like(X) + torch.eye(D, dtype=torch.float64)[a] for a in range(D)]
+    if D == 2:
+        modes.append(torch.stack([-X[..., 1], X[..., 0]], -1))
+    else:
+        for a, b in ((0, 1), (1, 2), (0, 2)):
+            rot = torch.zeros_like(X)
+            rot[..., a], rot[..., b] = -X[..., b], X[..., a]
+            modes.append(rot)
+    for mo in modes:
+        assert (KE @ mo.reshape(C.shape[0], -1, 1)).abs().max() < 1e-12
+
+
+@pytest.mark.parametrize("etype", _TENSOR)
+def test_tensor_facet_loads_closed_form(etype):
+    """On grids with straight boundaries (only interior nodes jittered) the
+    selected facets are exactly the boundary: c * v sums to the perimeter /
+    surface area, and f * v with f = x + 2y (+ 3z) to the boundary integral of f
+    (6 on the unit square, 3 + 6 + 9 = 18 on the unit cube)."""
+    from tensormesh_amd.mesh import unit_cube_hex_arrays, unit_square_quad_arrays
+    order = 2 if etype in ("quad9", "hexahedron27") else 1
+    if etype.startswith("quad"):
+        p, c, bm = unit_square_quad_arrays(3, 0.2, 2, order=order)
+    else:
+        p, c, bm = unit_cube_hex_arrays(2, 0.2, 2, order=order)
+    P, C, BM = torch.from_numpy(p), torch.from_numpy(c), torch.from_numpy(bm)
+    dim = P.shape[1]
+    one = fem.assemble_facet_linear(P, C, BM, etype, 2, const=torch.tensor(1.0))
+    assert abs(one.sum().item() - (4.0 if dim == 2 else 6.0)) < 1e-13
+    f = P @ torch.arange(1.0, dim + 1, dtype=torch.float64)
+    lin = fem.assemble_facet_linear(P, C, BM, etype, 2, f_nodal=f)
+    assert abs(lin.sum().item() - (6.0 if dim == 2 else 18.0)) < 1e-12
+    elem, fac = fem.facet_select(c, bm, etype)
+    n_cells = 3 if dim == 2 else 2
+    assert len(elem) == (4 * n_cells if dim == 2 else 6 * n_cells * n_cells)

@@ -56,7 +56,11 @@ enum tm_cell {
   TM_QUAD4 = 4, /* "quad"         B=4  D=2 */
   TM_QUAD9 = 5, /* "quad9"        B=9  D=2 */
   TM_HEX8 = 6,  /* "hexahedron"   B=8  D=3 */
-  TM_HEX27 = 7  /* "hexahedron27" B=27 D=3 */
+  TM_HEX27 = 7, /* "hexahedron27" B=27 D=3 */
+  /* cubic simplices: vertices, two nodes per edge (at 1/3 and 2/3 from the
+   * edge's first vertex), one centroid node per face (basis.py:639-859) */
+  TM_TRI10 = 8, /* "triangle10"   B=10 D=2 */
+  TM_TET20 = 9  /* "tetra20"      B=20 D=3 */
 };
 
 enum tm_form {

@@ -23,6 +23,7 @@ from tensormesh_amd.element import TET_QUAD, TRI_QUAD  # data tables only
 
 FAMILY = {"triangle": ("tri", 2, 1), "triangle6": ("tri", 2, 2),
           "tetra": ("tet", 3, 1), "tetra10": ("tet", 3, 2),
+          "triangle10": ("tri", 2, 3), "tetra20": ("tet", 3, 3),
           "quad": ("quad", 2, 1), "quad9": ("quad", 2, 2),
           "hexahedron": ("hex", 3, 1), "hexahedron27": ("hex", 3, 2)}
 TRI_EDGE = [[1, 2], [0, 2], [0, 1]]                         # element.py:1684
@@ -107,7 +108,16 @@ def lagrange_nodes(etype: str) -> np.ndarray:
     if order == 1:
         return verts
     edges = TRI_EDGE if fam == "tri" else TET_EDGE
-    return np.vstack([verts] + [0.5 * (verts[a] + verts[b]) for a, b in edges])
+    if order == 2:
+        return np.vstack([verts] + [0.5 * (verts[a] + verts[b]) for a, b in edges])
+    # order 3: lin_linspace along each edge (k/3, k = 1, 2 from the edge's first
+    # vertex), then tri_linspace of each face without its boundary (the centroid)
+    faces = [(0, 1, 2)] if fam == "tri" else [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    out = [verts]
+    for a, b in edges:
+        out += [verts[a] + t * (verts[b] - verts[a]) for t in (1 / 3, 2 / 3)]
+    out += [verts[list(f)].mean(0) for f in faces]
+    return np.vstack(out)
 
 
 def _exponents(dim: int, order: int, tensor: bool = False):
@@ -353,9 +363,16 @@ def facet_local_nodes(etype: str):
     n_vert = {"tri": 3, "tet": 4, "quad": 4, "hex": 8}[fam]
     out = []
     for fi, fv in enumerate(FACET_VERTS[fam]):
-        extra = [n_vert + m for m, (a, b) in enumerate(edges) if a in fv and b in fv] if order == 2 else []
-        if order == 2 and fam == "hex":
-            extra.append(n_vert + len(HEX_EDGE) + fi)
+        on = [m for m, (a, b) in enumerate(edges) if a in fv and b in fv]
+        extra = []
+        if order == 2:
+            extra = [n_vert + m for m in on]
+            if fam == "hex":
+                extra.append(n_vert + len(HEX_EDGE) + fi)
+        elif order == 3:
+            extra = [n_vert + 2 * m + k for m in on for k in (0, 1)]
+            if fam == "tet":
+                extra.append(n_vert + 2 * len(edges) + fi)
         out.append(list(fv) + extra)
     return np.array(out)
 

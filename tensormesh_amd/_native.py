@@ -20,7 +20,7 @@ ABI_VERSION = 1
 # enum values (include/tensormesh_amd.h)
 TM_F32, TM_F64 = 0, 1
 CELL = {"triangle": 0, "tetra": 1, "triangle6": 2, "tetra10": 3,
-        "quad": 4, "quad9": 5, "hexahedron": 6, "hexahedron27": 7}
+        "quad": 4, "quad9": 5, "hexahedron": 6, "hexahedron27": 7, "triangle10": 8, "tetra20": 9}
 FORM = {"laplace": 0, "diffusion_elem": 1, "diffusion_node": 2, "mass": 3, "linear_elasticity": 4}
 
 c_i32p = ctypes.POINTER(ctypes.c_int32)

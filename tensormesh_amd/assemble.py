@@ -2,7 +2,8 @@
 
 Mirrors tensormesh/assemble/element_assembler.py:35-879 and
 tensormesh/assemble/node_assembler.py:33-741 for meshes of one cell type
-(triangle, tetra, triangle6, tetra10, quad, quad9, hexahedron, hexahedron27):
+(triangle, tetra, triangle6, tetra10, triangle10, tetra20, quad, quad9,
+hexahedron, hexahedron27):
 
 * ``from_mesh(mesh, quadrature_order=2, project='reduce', *args, **kwargs)``
   builds the CSR pattern and the slot permutation once (libtmhip host

@@ -19,7 +19,8 @@ inline int fail(int status, const char* what) {
 }
 
 // Every cell type as X(enum, D, B), for switch statements over a cell code.
-#define TM_SIMPLEX_CELLS(X) X(TM_TRI3, 2, 3) X(TM_TET4, 3, 4) X(TM_TRI6, 2, 6) X(TM_TET10, 3, 10)
+#define TM_SIMPLEX_CELLS(X) \
+  X(TM_TRI3, 2, 3) X(TM_TET4, 3, 4) X(TM_TRI6, 2, 6) X(TM_TET10, 3, 10) X(TM_TRI10, 2, 10) X(TM_TET20, 3, 20)
 #define TM_TENSOR_CELLS(X) X(TM_QUAD4, 2, 4) X(TM_QUAD9, 2, 9) X(TM_HEX8, 3, 8) X(TM_HEX27, 3, 27)
 #define TM_ALL_CELLS(X) TM_SIMPLEX_CELLS(X) TM_TENSOR_CELLS(X)
 
@@ -45,8 +46,8 @@ inline int cell_dim(int cell) {
 // edges, hexahedron 6 faces.
 inline int cell_facets(int cell) {
   switch (cell) {
-    case TM_TRI3: case TM_TRI6: return 3;
-    case TM_TET4: case TM_TET10: case TM_QUAD4: case TM_QUAD9: return 4;
+    case TM_TRI3: case TM_TRI6: case TM_TRI10: return 3;
+    case TM_TET4: case TM_TET10: case TM_TET20: case TM_QUAD4: case TM_QUAD9: return 4;
     case TM_HEX8: case TM_HEX27: return 6;
     default: return -1;
   }

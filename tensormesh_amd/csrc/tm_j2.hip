@@ -319,6 +319,9 @@ int dispatch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k,
     case TM_QUAD4: return launch_j2<T, 2, 4>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     case TM_QUAD9: return launch_j2<T, 2, 9>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     case TM_HEX8: return launch_j2<T, 3, 8>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_TRI10: return launch_j2<T, 2, 10>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_TET20:
+      return tmh::fail(TM_ERR_UNSUPPORTED, "tetra20: the 60 x 60 element tangent does not fit the kernel's LDS");
     case TM_HEX27:
       return tmh::fail(TM_ERR_UNSUPPORTED, "hexahedron27: the 81 x 81 element tangent does not fit the kernel's LDS");
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");

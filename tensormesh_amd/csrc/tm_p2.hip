@@ -517,6 +517,9 @@ int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* dis
     case TM_QUAD4: return launch_hyper<T, 2, 4>(m, f, disp, tslot, rslot, inc_inv, s);
     case TM_QUAD9: return launch_hyper<T, 2, 9>(m, f, disp, tslot, rslot, inc_inv, s);
     case TM_HEX8: return launch_hyper<T, 3, 8>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_TRI10: return launch_hyper<T, 2, 10>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_TET20:
+      return tmh::fail(TM_ERR_UNSUPPORTED, "tetra20: the 60 x 60 element tangent does not fit the kernel's LDS");
     case TM_HEX27:
       return tmh::fail(TM_ERR_UNSUPPORTED, "hexahedron27: the 81 x 81 element tangent does not fit the kernel's LDS");
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");

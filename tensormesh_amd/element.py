@@ -50,6 +50,20 @@ TRI_QUAD = {
         [[0.445948490915965, 0.10810301816807], [0.0915762135097699, 0.816847572980459],
          [0.10810301816807, 0.445948490915965], [0.816847572980459, 0.091576213509771],
          [0.445948490915965, 0.445948490915965], [0.091576213509771, 0.0915762135097699]]),
+    5: ([0.1125, 0.066197076394253, 0.0629695902724135, 0.066197076394253, 0.0629695902724135,
+         0.066197076394253, 0.0629695902724135],
+        [[0.333333333333333, 0.333333333333333], [0.470142064105115, 0.05971587178977],
+         [0.101286507323457, 0.797426985353087], [0.05971587178977, 0.470142064105115],
+         [0.797426985353087, 0.101286507323456], [0.470142064105115, 0.470142064105115],
+         [0.101286507323456, 0.101286507323457]]),
+    6: ([0.0583931378631895, 0.0254224531851035, 0.0583931378631895, 0.0254224531851035, 0.0583931378631895,
+         0.0254224531851035] + [0.041425537809187] * 6,
+        [[0.249286745170911, 0.501426509658179], [0.0630890144915021, 0.873821971016996],
+         [0.501426509658179, 0.24928674517091], [0.873821971016996, 0.063089014491502],
+         [0.24928674517091, 0.249286745170911], [0.063089014491502, 0.0630890144915021],
+         [0.636502499121399, 0.053145049844817], [0.310352451033784, 0.053145049844817],
+         [0.053145049844817, 0.310352451033784], [0.053145049844817, 0.636502499121399],
+         [0.310352451033784, 0.636502499121399], [0.636502499121399, 0.310352451033784]]),
 }
 
 TET_QUAD = {
@@ -76,6 +90,31 @@ TET_QUAD = {
          [0.3994035761667992, 0.1005964238332008, 0.1005964238332008],
          [0.1005964238332008, 0.3994035761667992, 0.1005964238332008],
          [0.1005964238332008, 0.1005964238332008, 0.3994035761667992]]),
+    5: ([0.003174603174603167] * 6 + [0.014764970790496783] * 4 + [0.022139791114265117] * 4,
+        [[0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0], [0.5, 0.0, 0.0], [0.0, 0.5, 0.0], [0.0, 0.0, 0.5],
+         [0.6984197043243866, 0.1005267652252045, 0.1005267652252045],
+         [0.1005267652252045, 0.1005267652252045, 0.1005267652252045],
+         [0.1005267652252045, 0.1005267652252045, 0.6984197043243866],
+         [0.1005267652252045, 0.6984197043243866, 0.1005267652252045],
+         [0.0568813795204234, 0.3143728734931922, 0.3143728734931922],
+         [0.3143728734931922, 0.3143728734931922, 0.3143728734931922],
+         [0.3143728734931922, 0.3143728734931922, 0.0568813795204234],
+         [0.3143728734931922, 0.0568813795204234, 0.3143728734931922]]),
+    6: ([0.030283678097089182] + [0.006026785714285717] * 4 + [0.011645249086028967] * 4
+        + [0.010949141561386449] * 6,
+        [[0.25, 0.25, 0.25], [0.0, 0.3333333333333333, 0.3333333333333333],
+         [0.3333333333333333, 0.3333333333333333, 0.3333333333333333],
+         [0.3333333333333333, 0.3333333333333333, 0.0], [0.3333333333333333, 0.0, 0.3333333333333333],
+         [0.7272727272727273, 0.0909090909090909, 0.0909090909090909],
+         [0.0909090909090909, 0.0909090909090909, 0.0909090909090909],
+         [0.0909090909090909, 0.0909090909090909, 0.7272727272727273],
+         [0.0909090909090909, 0.7272727272727273, 0.0909090909090909],
+         [0.4334498464263357, 0.0665501535736643, 0.0665501535736643],
+         [0.0665501535736643, 0.4334498464263357, 0.0665501535736643],
+         [0.0665501535736643, 0.0665501535736643, 0.4334498464263357],
+         [0.0665501535736643, 0.4334498464263357, 0.4334498464263357],
+         [0.4334498464263357, 0.0665501535736643, 0.4334498464263357],
+         [0.4334498464263357, 0.4334498464263357, 0.0665501535736643]]),
 }
 
 # n-point Gauss-Legendre on [-1, 1] (quadrature.py:36-54), mapped to [0, 1] with
@@ -106,6 +145,8 @@ ELEMENT_SHAPE = {
     "triangle6": (2, 2, 6, "tri"),
     "tetra": (3, 1, 4, "tet"),
     "tetra10": (3, 2, 10, "tet"),
+    "triangle10": (2, 3, 10, "tri"),
+    "tetra20": (3, 3, 20, "tet"),
     "quad": (2, 1, 4, "quad"),
     "quad9": (2, 2, 9, "quad"),
     "hexahedron": (3, 1, 8, "hex"),
@@ -219,6 +260,8 @@ def shape_values(element_type: str, points: np.ndarray) -> np.ndarray:
     lam = _barycentric(points)
     if order == 1:
         return lam.copy()
+    if order == 3:
+        return _p3_shape(fam, lam, None)[0]
     edges = ((1, 2), (0, 2), (0, 1)) if fam == "tri" else ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))
     vert = lam * (2.0 * lam - 1.0)
     mid = np.stack([4.0 * lam[:, a] * lam[:, b] for a, b in edges], 1)
@@ -242,12 +285,46 @@ def shape_ref_grads(element_type: str, points: np.ndarray) -> np.ndarray:
     if order == 1:
         out[:] = dlam[None]
         return out
+    if order == 3:
+        return _p3_shape(fam, lam, dlam)[1]
     edges = ((1, 2), (0, 2), (0, 1)) if fam == "tri" else ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))
     for k in range(dim + 1):
         out[:, :, k] = (4.0 * lam[:, k:k + 1] - 1.0) * dlam[None, :, k]
     for m, (a, b) in enumerate(edges):
         out[:, :, dim + 1 + m] = 4.0 * (lam[:, b:b + 1] * dlam[None, :, a] + lam[:, a:a + 1] * dlam[None, :, b])
     return out
+
+
+_SIMPLEX_FACES = {"tri": ((0, 1, 2),), "tet": ((1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2))}
+
+
+def _p3_shape(fam: str, lam: np.ndarray, dlam):
+    """Cubic Lagrange basis in barycentric coordinates, in the reference's node
+    order (tri_basis / tet_basis, basis.py:639-859): vertices
+    l_i (3 l_i - 1)(3 l_i - 2) / 2; two nodes per edge (a, b) at 1/3 and 2/3
+    from a, 9/2 l_a l_b (3 l_a - 1) and 9/2 l_a l_b (3 l_b - 1); one node per
+    face (i, j, k) at its centroid, 27 l_i l_j l_k.  Returns (phi [Q, B],
+    dphi [Q, D, B] when dlam [D, D+1] is given)."""
+    edges = ((1, 2), (0, 2), (0, 1)) if fam == "tri" else ((2, 3), (1, 3), (1, 2), (0, 3), (0, 2), (0, 1))
+    terms = []   # each basis function as a product of (coefficient, [(k, a, b)]) factors a*l_k + b
+    for i in range(lam.shape[1]):
+        terms.append((0.5, [(i, 1.0, 0.0), (i, 3.0, -1.0), (i, 3.0, -2.0)]))
+    for a, b in edges:
+        terms.append((4.5, [(a, 1.0, 0.0), (b, 1.0, 0.0), (a, 3.0, -1.0)]))
+        terms.append((4.5, [(a, 1.0, 0.0), (b, 1.0, 0.0), (b, 3.0, -1.0)]))
+    for i, j, k in _SIMPLEX_FACES[fam]:
+        terms.append((27.0, [(i, 1.0, 0.0), (j, 1.0, 0.0), (k, 1.0, 0.0)]))
+    q = lam.shape[0]
+    phi = np.zeros((q, len(terms)))
+    dphi = None if dlam is None else np.zeros((q, dlam.shape[0], len(terms)))
+    for n, (c, fac) in enumerate(terms):
+        vals = [a * lam[:, k] + b for k, a, b in fac]
+        phi[:, n] = c * np.prod(vals, axis=0)
+        if dlam is not None:
+            for m, (k, a, b) in enumerate(fac):
+                others = np.prod([v for t, v in enumerate(vals) if t != m], axis=0)
+                dphi[:, :, n] += c * a * others[:, None] * dlam[None, :, k]
+    return phi, dphi
 
 
 @functools.lru_cache(maxsize=None)
@@ -313,6 +390,12 @@ def facet_nodes(element_type: str) -> np.ndarray:
             nodes += [n_vert + m for m, (a, b) in enumerate(_EDGES[fam]) if a in fv and b in fv]
             if fam == "hex":
                 nodes.append(n_vert + len(_HEX_EDGES) + fi)
+        elif order == 3:
+            for m, (a, b) in enumerate(_EDGES[fam]):
+                if a in fv and b in fv:
+                    nodes += [n_vert + 2 * m, n_vert + 2 * m + 1]
+            if fam == "tet":
+                nodes.append(n_vert + 2 * len(_EDGES[fam]) + fi)
         out.append(nodes)
     return np.asarray(out, np.int64)
 

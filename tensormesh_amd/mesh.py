@@ -43,6 +43,8 @@ ELEMENT_INFO = {
     "tetra": (3, 1, 4),
     "triangle6": (2, 2, 6),
     "tetra10": (3, 2, 10),
+    "triangle10": (2, 3, 10),
+    "tetra20": (3, 3, 20),
     "quad": (2, 1, 4),
     "quad9": (2, 2, 9),
     "hexahedron": (3, 1, 8),
@@ -215,6 +217,58 @@ def quadratic_arrays(points: np.ndarray, cells: np.ndarray, boundary: np.ndarray
     return new_points, new_cells, new_boundary
 
 
+def cubic_arrays(points: np.ndarray, cells: np.ndarray, boundary: np.ndarray):
+    """Promote P1 simplices to P3 (triangle10 / tetra20).
+
+    Each unique edge (lo, hi) gets two nodes, at 1/3 and 2/3 from lo, numbered
+    in order of first appearance; a cell lists them from its own local edge's
+    first vertex (tri_basis / tet_basis node order, basis.py:639-859).  Each
+    face gets one centroid node: per cell for triangles, per unique face for
+    tetrahedra.  A new node is a boundary node iff all vertices of its edge /
+    face are; triangle centroids are interior."""
+    b = cells.shape[1]
+    loc = np.asarray(TRI_EDGES if b == 3 else TET_EDGES, dtype=np.int64)
+    n = points.shape[0]
+    e0, e1 = cells[:, loc[:, 0]], cells[:, loc[:, 1]]
+    lo, hi = np.minimum(e0, e1).ravel(), np.maximum(e0, e1).ravel()
+    uniq, first_idx, inv = np.unique(lo * n + hi, return_index=True, return_inverse=True)
+    order = np.argsort(first_idx, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    ulo, uhi = uniq[order] // n, uniq[order] % n
+    n_e = ulo.size
+    third = np.empty((2 * n_e, points.shape[1]))
+    third[0::2] = (2.0 * points[ulo] + points[uhi]) / 3.0       # 1/3 from lo
+    third[1::2] = (points[ulo] + 2.0 * points[uhi]) / 3.0       # 2/3 from lo
+    near_lo = n + 2 * rank[inv]
+    forward = (e0.ravel() == lo)                                  # local edge runs lo -> hi
+    first = np.where(forward, near_lo, near_lo + 1).reshape(cells.shape[0], -1)
+    second = np.where(forward, near_lo + 1, near_lo).reshape(cells.shape[0], -1)
+    edge_nodes = np.stack([first, second], 2).reshape(cells.shape[0], -1)
+    ebnd = np.repeat(boundary[ulo] & boundary[uhi], 2)
+    base = n + 2 * n_e
+    if b == 3:
+        face_pts = points[cells].mean(1)
+        face_nodes = (base + np.arange(cells.shape[0]))[:, None]
+        fbnd = np.zeros(cells.shape[0], bool)
+    else:
+        floc = np.array([(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)], np.int64)
+        fverts = np.sort(cells[:, floc], axis=2).reshape(-1, 3)
+        key = (fverts[:, 0] * n + fverts[:, 1]) * n + fverts[:, 2]
+        fu, ffirst, finv = np.unique(key, return_index=True, return_inverse=True)
+        forder = np.argsort(ffirst, kind="stable")
+        frank = np.empty_like(forder)
+        frank[forder] = np.arange(forder.size)
+        uf = fverts[ffirst[forder]]
+        face_pts = points[uf].mean(1)
+        face_nodes = (base + frank[finv]).reshape(cells.shape[0], 4)
+        fbnd = boundary[uf].all(1)
+    new_points = np.concatenate([points, third, face_pts], 0)
+    new_boundary = np.concatenate([boundary, ebnd, fbnd])
+    new_cells = np.concatenate([cells, edge_nodes, face_nodes], 1)
+    return new_points, new_cells, new_boundary
+
+
 def _tensor_grid_arrays(dim: int, n: int, order: int, jitter: float, seed: Optional[int]):
     """Structured n^dim grid of quadrilaterals / hexahedra on [0, 1]^dim.
 
@@ -283,6 +337,9 @@ def unit_square_tri(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order:
     if order == 2:
         pts, cells, bnd = quadratic_arrays(pts, cells, bnd)
         etype = "triangle6"
+    elif order == 3:
+        pts, cells, bnd = cubic_arrays(pts, cells, bnd)
+        etype = "triangle10"
     return _to_mesh(pts, cells, bnd, etype, dtype, device)
 
 
@@ -293,6 +350,9 @@ def unit_cube_tet(n: int, jitter: float = 0.0, seed: Optional[int] = 0, order: i
     if order == 2:
         pts, cells, bnd = quadratic_arrays(pts, cells, bnd)
         etype = "tetra10"
+    elif order == 3:
+        pts, cells, bnd = cubic_arrays(pts, cells, bnd)
+        etype = "tetra20"
     return _to_mesh(pts, cells, bnd, etype, dtype, device)
 
 

@@ -1,5 +1,6 @@
 """Quadrilateral and hexahedral cells (quad, quad9, hexahedron, hexahedron27)
-on the HIP kernels against the CPU oracle.
+and cubic simplices (triangle10, tetra20) on the HIP kernels against the CPU
+oracle.
 
 Tolerances (float64): values within rtol 1e-12 plus a floor of 1e-14 max|ref|
 for cancelling entries; Neo-Hookean / J2 / contact as in their own tests
@@ -15,11 +16,12 @@ import torch
 import tensormesh_amd as tm
 from oracle import fem
 from tensormesh_amd import _native as nat
-from tensormesh_amd.mesh import unit_cube_hex, unit_square_quad
+from tensormesh_amd.mesh import unit_cube_hex, unit_cube_tet, unit_square_quad, unit_square_tri
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
 TENSOR = ["quad", "quad9", "hexahedron", "hexahedron27"]
+CUBIC = ["triangle10", "tetra20"]
 
 
 def say(msg):
@@ -28,6 +30,9 @@ def say(msg):
 
 
 def _mesh(etype, n=None, seed=1):
+    if etype in CUBIC:
+        return (unit_square_tri(n or 3, 0.15, seed, order=3) if etype == "triangle10"
+                else unit_cube_tet(n or 2, 0.1, seed, order=3))
     order = 2 if etype in ("quad9", "hexahedron27") else 1
     if etype.startswith("quad"):
         return unit_square_quad(n or 4, 0.2, seed, order=order)
@@ -46,8 +51,7 @@ def _assembled(Ke, C, n_points):
     return fem._index_add(Ke.reshape((-1,) + tuple(Ke.shape[3:])), torch.from_numpy(eids), row.size), row, col
 
 
-@pytest.mark.parametrize("etype", TENSOR)
-@pytest.mark.parametrize("qo", [2, 3])
+@pytest.mark.parametrize("etype,qo", [(e, q) for e in TENSOR for q in (2, 3)] + [(e, q) for e in CUBIC for q in (4, 6)])
 def test_tensor_bilinear_forms_against_oracle(etype, qo):
     say(f"case tensor bilinear {etype} qorder {qo}")
     m = _mesh(etype)
@@ -76,7 +80,7 @@ def test_tensor_bilinear_forms_against_oracle(etype, qo):
     assert ok, f"elasticity: {err:.2e}"
 
 
-@pytest.mark.parametrize("etype", TENSOR)
+@pytest.mark.parametrize("etype", TENSOR + CUBIC)
 def test_tensor_loads_and_facet_loads_against_oracle(etype):
     say(f"case tensor loads {etype}")
     m = _mesh(etype)
@@ -102,10 +106,10 @@ def test_tensor_loads_and_facet_loads_against_oracle(etype):
     assert ok, f"facet load: {err:.2e}"
 
 
-@pytest.mark.parametrize("etype", ["quad", "quad9", "hexahedron"])
+@pytest.mark.parametrize("etype", ["quad", "quad9", "hexahedron", "triangle10"])
 def test_tensor_neohookean_against_oracle(etype):
     say(f"case tensor neo-hookean {etype}")
-    m = _mesh(etype, n=3 if etype.startswith("quad") else 2)
+    m = _mesh(etype, n=3 if etype.startswith(("quad", "tri")) else 2)
     P = m.points
     g = torch.Generator().manual_seed(5)
     u = 0.05 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, -1], 0.5 * P[:, 0] ** 2][:P.shape[1]], 1)
@@ -125,10 +129,11 @@ def test_tensor_neohookean_against_oracle(etype):
     assert ok, f"tangent: {err:.2e}"
 
 
-def test_hexahedron27_neohookean_is_rejected():
-    m = _mesh("hexahedron27", n=1).to(DEV)
+@pytest.mark.parametrize("etype", ["hexahedron27", "tetra20"])
+def test_large_cells_neohookean_is_rejected(etype):
+    m = _mesh(etype, n=1).to(DEV)
     model = tm.NeoHookeanModel.from_mesh(m, quadrature_order=2)
-    with pytest.raises(nat.NativeError, match="hexahedron27"):
+    with pytest.raises(nat.NativeError, match=etype):
         model.residual_and_tangent(torch.zeros_like(m.points))
 
 
@@ -159,10 +164,10 @@ def test_hexahedron_j2_against_oracle():
     assert ok, f"tangent: {err:.2e}"
 
 
-@pytest.mark.parametrize("etype", ["quad9", "hexahedron", "hexahedron27"])
+@pytest.mark.parametrize("etype", ["quad9", "hexahedron", "hexahedron27", "triangle10", "tetra20"])
 def test_tensor_contact_against_oracle(etype):
     say(f"case tensor contact {etype}")
-    m = _mesh(etype, n=3 if etype.startswith("quad") else 2)
+    m = _mesh(etype, n=3 if etype.startswith(("quad", "tri")) else 2)
     D = m.points.shape[1]
     n = [0.2, 1.0] if D == 2 else [0.1, 1.0, -0.2]
     s = math.sqrt(sum(x * x for x in n))

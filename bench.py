@@ -425,6 +425,73 @@ def plasticity_bench(n=48, reps=10):
             "grad_max_rel_diff": float((g_nat - g_ref).abs().max() / g_ref.abs().max())}
 
 
+def _timed(call, reps):
+    out = call()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        out = call()
+    ev[1].record()
+    torch.cuda.synchronize()
+    return out, ev[0].elapsed_time(ev[1]) / reps
+
+
+def contact_bench(n=64, reps=20):
+    """Penalty contact (PenaltyContact, kappa = 1e3, plane y = 0.02) on the
+    boundary of the jittered 64^3 x 6 Kuhn tetrahedral cube pushed 0.05 below
+    the plane, float64, qorder 2: energy + backward (the reference's workflow)
+    natively and by the reference algorithm (vmapped torch density +
+    autograd) on the same GPU, and the native gradient + Hessian."""
+    import tensormesh_amd as tm
+    from tensormesh_amd.mesh import Mesh, unit_cube_tet_arrays
+    p, c, bm = unit_cube_tet_arrays(n, 0.15, 0)
+    m = Mesh(torch.from_numpy(p), {"tetra": torch.from_numpy(c)}, torch.from_numpy(bm)).to("cuda")
+    model = tm.PenaltyContact.from_mesh(m, quadrature_order=2, kappa=1e3, obstacle_y=0.02)
+    x = m.points.clone()
+    x[:, 1] -= 0.05
+
+    def energy_and_grad():
+        xx = x.clone().requires_grad_(True)
+        e = model.energy(points=xx)
+        e.backward()
+        return e.detach(), xx.grad
+
+    (e_nat, g_nat), ms = _timed(energy_and_grad, reps)
+    _, ms_rt = _timed(lambda: model.residual_and_tangent(x), reps)
+    model.reference_path = True
+    (e_ref, g_ref), ms_ref = _timed(energy_and_grad, 5)
+    model.reference_path = False
+    F = int(model._topo.elem.numel())
+    return {"workload": f"penalty contact on the boundary of {n}^3x6 tets: {F} facets, f64, qorder 2",
+            "ms_energy_and_backward": ms, "ms_gradient_and_hessian": ms_rt,
+            "reference_algorithm_ms_energy_and_backward": ms_ref, "speedup_energy_and_backward": ms_ref / ms,
+            "energy_rel_diff": abs(e_nat.item() - e_ref.item()) / abs(e_ref.item()),
+            "grad_max_rel_diff": float((g_nat - g_ref).abs().max() / g_ref.abs().max())}
+
+
+def hex_bench(n=64, reps=10):
+    """Hexahedral assembly on the jittered 64^3 hexahedron grid (262,144 cells,
+    274,625 nodes), float64, 2 x 2 x 2 points: Laplace stiffness and linear
+    elasticity (3 x 3 blocks) natively and by the reference algorithm (the
+    vmapped torch integrand and index_add projection) on the same GPU."""
+    import tensormesh_amd as tm
+    from tensormesh_amd.mesh import unit_cube_hex
+    m = unit_cube_hex(n, 0.15, 0).to("cuda")
+    out = {"workload": f"hexahedron {n}^3 = {m.elements().shape[0]} cells, {m.n_points} nodes, qorder 2 (8 points), f64"}
+    for name, cls, kw in (("laplace", tm.LaplaceElementAssembler, {}),
+                          ("elasticity", tm.LinearElasticityElementAssembler, {"E": 1.0, "nu": 0.3})):
+        A = cls.from_mesh(m, quadrature_order=2, **kw)
+        K, ms = _timed(lambda: A(), reps)
+        A.reference_path = True
+        K_ref, ms_ref = _timed(lambda: A(), 2)
+        A.reference_path = False
+        out[name] = {"ms_per_call": ms, "cells_per_s": m.elements().shape[0] / (ms * 1e-3),
+                     "reference_algorithm_ms_per_call": ms_ref, "speedup": ms_ref / ms,
+                     "max_rel_diff": float((K.values - K_ref.values).abs().max() / K_ref.values.abs().max())}
+    return out
+
+
 def batched_bench(n_meshes=256, seed=0):
     """256 jittered 2-D P1 meshes, side N ~ U{16..128} (seeded), own log-normal
     kappa and a sinusoidal source each; one assembly launch for all stiffness
@@ -552,6 +619,9 @@ def main():
                     help="only launch each assembly kernel a few times (for rocprofv3 counter runs)")
     ap.add_argument("--profile-hyper", action="store_true",
                     help="only launch the Neo-Hookean assembly a few times (for rocprofv3 runs)")
+    ap.add_argument("--legs", default="",
+                    help="comma-separated side legs to run alone, one JSON line each "
+                         "(plasticity, contact, hexahedra, facet, neo_hookean, batched)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -566,6 +636,12 @@ def main():
         return
     if args.profile_hyper:
         profile_hyper()
+        return
+    if args.legs:
+        legs = {"plasticity": plasticity_bench, "contact": contact_bench, "hexahedra": hex_bench,
+                "facet": facet_bench, "neo_hookean": hyperelastic_bench, "batched": batched_bench}
+        for name in args.legs.split(","):
+            print(json.dumps({name: legs[name]()}), flush=True)
         return
     dtype = torch.float64 if args.dtype == "f64" else torch.float32
     mesh, kappa, A, setup_s, host = build_workload(dtype)
@@ -675,6 +751,8 @@ def main():
         out["batched"] = batched_bench()
         out["facet"] = facet_bench()
         out["plasticity"] = plasticity_bench()
+        out["contact"] = contact_bench()
+        out["hexahedra"] = hex_bench()
     if not args.no_cpu_baseline:
         cb = cpu_baseline(*host)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -144,7 +144,8 @@ class ElementAssembler(_AssemblerBase):
     #: "two_stage" = element kernel + segmented reduction (tm_assemble_bilinear),
     #: "two_stage_rows" = element kernel + row-centric reduction (tm_reduce_p1_rows),
     #: "incidence" = element kernel writing in incidence order + contiguous row
-    #: reduction (tm_assemble_p1_incidence).
+    #: reduction (tm_assemble_p1_incidence), "auto" = the fastest of the four on
+    #: this device and pattern, timed once and cached (tensormesh_amd.tune).
     p1_path: str = "rows"
     #: Build the pattern on the GPU (tm_pattern_device_*) when the mesh is on the device.
     device_pattern: bool = True
@@ -284,26 +285,17 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
                             tab.ctypes.data, tab_dev.data_ptr(),
                             None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
-        if H == 1 and self.p1_path in ("rows", "two_stage_rows", "incidence") and self.etype in ("triangle", "tetra"):
-            rv = pat.rows_view(conn)
-            if rv is not None:
-                values = torch.empty(pat.nnz, dtype=dtype, device=dev)
-                if self.p1_path == "rows":
-                    nat.check(L.tm_assemble_p1_rows(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
-                                                    values.data_ptr(), nat.stream_of(dev)), "tm_assemble_p1_rows")
-                elif self.p1_path == "incidence":
-                    slot = torch.empty(E * B * B, dtype=dtype, device=dev)
-                    nat.check(L.tm_assemble_p1_incidence(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
-                                                         pat.inc_inv().data_ptr(), slot.data_ptr(),
-                                                         values.data_ptr(), nat.stream_of(dev)),
-                              "tm_assemble_p1_incidence")
-                else:
-                    slot = torch.empty(E * B * B, dtype=dtype, device=dev)
-                    st = nat.stream_of(dev)
-                    nat.check(L.tm_element_matrices(ctypes.byref(mesh), ctypes.byref(form), slot.data_ptr(), st),
-                              "tm_element_matrices")
-                    nat.check(L.tm_reduce_p1_rows(nat.dtype_code(dtype), nat.CELL[self.etype], ctypes.byref(rv),
-                                                  slot.data_ptr(), 0, values.data_ptr(), st), "tm_reduce_p1_rows")
+        path = self.p1_path
+        p1_scalar = H == 1 and self.etype in ("triangle", "tetra")
+        if p1_scalar and path == "auto":
+            from .tune import device_tag, tuner
+            key = f"{device_tag(dev)}|p1_path|{self.etype}|{match.form}|{dtype}|{pat.hash}"
+            cands = {p: (lambda p=p: self._p1_launch(p, mesh, form, pat, conn, dtype, dev))
+                     for p in ("rows", "two_stage", "two_stage_rows", "incidence")}
+            path = tuner().choose(key, cands, device=dev)
+        if p1_scalar and path in ("rows", "two_stage_rows", "incidence"):
+            values = self._p1_launch(path, mesh, form, pat, conn, dtype, dev)
+            if values is not None:
                 return values
         slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
         values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
@@ -313,8 +305,40 @@ class ElementAssembler(_AssemblerBase):
                   "tm_assemble_bilinear")
         return values if H == 1 else values.reshape(pat.nnz, H, H)
 
+    def _p1_launch(self, path: str, mesh, form, pat, conn, dtype, dev) -> Optional[torch.Tensor]:
+        """CSR values of a P1 scalar form on one of the four paths; None when the
+        row paths do not apply to this pattern (repeated vertices, long rows)."""
+        L = nat.lib()
+        E, B = conn.shape
+        st = nat.stream_of(dev)
+        values = torch.empty(pat.nnz, dtype=dtype, device=dev)
+        if path == "two_stage":
+            slot = torch.empty(E * B * B, dtype=dtype, device=dev)
+            pv = pat.view(dtype, 1)
+            nat.check(L.tm_assemble_bilinear(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(pv),
+                                             slot.data_ptr(), values.data_ptr(), st), "tm_assemble_bilinear")
+            return values
+        rv = pat.rows_view(conn)
+        if rv is None:
+            return None
+        if path == "rows":
+            nat.check(L.tm_assemble_p1_rows(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
+                                            values.data_ptr(), st), "tm_assemble_p1_rows")
+        elif path == "incidence":
+            slot = torch.empty(E * B * B, dtype=dtype, device=dev)
+            nat.check(L.tm_assemble_p1_incidence(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(rv),
+                                                 pat.inc_inv().data_ptr(), slot.data_ptr(), values.data_ptr(), st),
+                      "tm_assemble_p1_incidence")
+        else:
+            slot = torch.empty(E * B * B, dtype=dtype, device=dev)
+            nat.check(L.tm_element_matrices(ctypes.byref(mesh), ctypes.byref(form), slot.data_ptr(), st),
+                      "tm_element_matrices")
+            nat.check(L.tm_reduce_p1_rows(nat.dtype_code(dtype), nat.CELL[self.etype], ctypes.byref(rv),
+                                          slot.data_ptr(), 0, values.data_ptr(), st), "tm_reduce_p1_rows")
+        return values
+
     def _native_hyperelastic(self, u: torch.Tensor, points: Optional[torch.Tensor], young: float, poisson: float,
-                             want_residual: bool = True, want_tangent: bool = True):
+                             want_residual: bool = True, want_tangent: bool = True, path: Optional[str] = None):
         """Neo-Hookean residual [N*D] and tangent (block SparseMatrix) at
         displacement u [N, D] on the HIP kernels (tm_assemble_neohookean)."""
         points = self._points if points is None else points
@@ -336,8 +360,16 @@ class ElementAssembler(_AssemblerBase):
                             None, float(young), float(poisson),
                             nat.TM_FLAG_MFMA if (getattr(self, "use_mfma", False) and dtype == torch.float32) else 0)
         # the block-row reduction is built for simplex cells; other cells use the slot path
-        use_rows = getattr(self, "nh_path", "slots") == "rows" and self.etype in ("triangle", "tetra", "triangle6",
-                                                                                  "tetra10")
+        simplex = self.etype in ("triangle", "tetra", "triangle6", "tetra10")
+        path = path or getattr(self, "nh_path", "slots")
+        if path == "auto" and simplex and want_tangent:
+            from .tune import device_tag, tuner
+            key = (f"{device_tag(dev)}|nh_path|{self.etype}|{dtype}|{int(form.flags)}|{want_residual}|"
+                   f"{pat.hash}")
+            cands = {p: (lambda p=p: self._native_hyperelastic(u, points, young, poisson, want_residual, True, p))
+                     for p in ("slots", "rows")}
+            path = tuner().choose(key, cands, device=dev)
+        use_rows = path == "rows" and simplex
         brv = self._block_rows(pat, conn, dev, D) if use_rows else None
         if brv is not None:
             tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None

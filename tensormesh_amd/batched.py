@@ -55,6 +55,7 @@ class BatchedDiffusion:
         self.union = Mesh(torch.cat(pts), {etype: torch.cat(cells)}, torch.cat(bnd))
         self.n_sys = len(meshes)
         self.K = DiffusionElementAssembler.from_mesh(self.union, quadrature_order=quadrature_order)
+        self.K.p1_path = "auto"   # the P1 assembly path is timed once per union pattern (tensormesh_amd.tune)
         self.F = SourceNodeAssembler.from_mesh(self.union, quadrature_order=quadrature_order)
         self.cond = Condenser(self.union.boundary_mask)
         inner = (~self.union.boundary_mask).to(torch.int64).cpu()

@@ -71,7 +71,9 @@ class NeoHookeanModel(ElementAssembler):
     ``nh_path = "slots"`` (default) writes every element's block rows in
     element order and sums them with the segment reduction
     (tm_assemble_neohookean); ``"rows"`` writes them in node-incidence order
-    and sums them by block rows (tm_assemble_neohookean_rows).
+    and sums them by block rows (tm_assemble_neohookean_rows); ``"auto"``
+    times both once per device and pattern and keeps the faster
+    (tensormesh_amd.tune).
     ``use_mfma = True`` (float32) contracts the tangent on the matrix cores
     (v_mfma_f32_32x32x2f32) instead of the closed-form VALU loop."""
 

@@ -110,6 +110,7 @@ class BatchedPCG:
         self.scalars = torch.zeros(self.n_sys * self.stride, dtype=torch.float64, device=self.dev)
         self.rtol = rtol
         self._graphs = {}
+        self._last_iters = 0
         self.refresh_values(None)
 
     def refresh_values(self, values: Optional[torch.Tensor]):
@@ -151,9 +152,17 @@ class BatchedPCG:
                   "tm_pcg_iterate")
         self._parity = (self._parity + n) & 1
 
-    def solve(self, b: torch.Tensor, x0: Optional[torch.Tensor] = None, max_iter: int = 20000, chunk: int = 32,
-              graph: bool = True):
-        """Returns (x, iterations per system [n_sys], relative residual per system)."""
+    def solve(self, b: torch.Tensor, x0: Optional[torch.Tensor] = None, max_iter: int = 20000,
+              chunk: Optional[int] = None, graph: bool = True):
+        """Returns (x, iterations per system [n_sys], relative residual per system).
+
+        ``chunk``: iterations per graph replay between convergence polls; None
+        picks it from the previous solve's iteration count of this solver
+        (tensormesh_amd.tune.pcg_chunk).  Converged systems stop on the device,
+        so the chunk does not change the result."""
+        if chunk is None:
+            from .tune import pcg_chunk
+            chunk = pcg_chunk(self._last_iters)
         view = self._view()
         if x0 is None:
             self.x.zero_()
@@ -192,6 +201,7 @@ class BatchedPCG:
         sc = self.scalars.view(self.n_sys, self.stride)
         iters = sc[:, 7].to(torch.int64)
         rel = torch.sqrt(sc[:, 4] / torch.clamp(sc[:, 5], min=1e-300))
+        self._last_iters = int(iters.max().item()) if self.n_sys else 0
         return self.x.clone(), iters, rel
 
 

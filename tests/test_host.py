@@ -442,3 +442,28 @@ def test_cubic_cells_torch_path_matches_oracle(etype):
     row, col, eids = fem.pattern(m.elements().numpy(), m.n_points)
     ref = fem._index_add(Ke.reshape(-1), torch.from_numpy(eids), row.size)
     assert torch.allclose(K.values, ref, rtol=1e-12, atol=1e-13 * ref.abs().max().item())
+
+
+# ------------------------------------------------------------------ tuner ---
+def test_tuner_cache_roundtrip_and_disabled_default(tmp_path, monkeypatch):
+    from tensormesh_amd import tune
+    path = tmp_path / "tune.json"
+    t = tune.Tuner(path)
+    t.put("dev|op|abc", {"choice": "rows", "ms": {"rows": 0.1}})
+    t2 = tune.Tuner(path)
+    assert t2.get("dev|op|abc")["choice"] == "rows"
+    monkeypatch.setenv("TENSORMESH_AMD_TUNE", "0")
+    t3 = tune.Tuner(tmp_path / "other.json")
+    calls = []
+    assert t3.choose("k", {"a": lambda: calls.append("a"), "b": lambda: calls.append("b")}) == "a"
+    assert calls == []          # disabled: nothing is timed
+    assert t3.choose("k", {"only": lambda: calls.append("x")}) == "only"
+
+
+def test_pcg_chunk_rule():
+    from tensormesh_amd.tune import pcg_chunk
+    assert pcg_chunk(None) == 32 and pcg_chunk(0) == 32
+    assert pcg_chunk(100) == 8 and pcg_chunk(2697) == 128 and pcg_chunk(700) == 32
+    for n in (1, 50, 300, 1000, 5000, 10 ** 6):
+        c = pcg_chunk(n)
+        assert c in (8, 16, 32, 64, 128) and c % 2 == 0

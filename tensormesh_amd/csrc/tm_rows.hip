@@ -24,8 +24,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kCapInc = 512;   // incidences staged per workgroup (matches the host plan cap)
-constexpr int kMaxRows = 128;  // rows staged per workgroup (the host plan splits longer runs)
-constexpr int kMaxEnt = 2048;  // CSR entries staged per workgroup
+constexpr int kMaxRows = 64;   // rows staged per workgroup (the host plan splits longer runs)
+constexpr int kMaxEnt = 1024;  // CSR entries staged per workgroup
 
 struct RowConsts {
   double W;        // sum_q w_q

@@ -62,7 +62,7 @@ class Pattern:
     # blocks of the row-centric P1 kernels (tm_rows.hip): at most ROWS_CAP
     # incidences and ROWS_MAX rows per workgroup
     ROWS_CAP = 512
-    ROWS_MAX = 128
+    ROWS_MAX = 64
 
     def colpos(self, conn: torch.Tensor) -> Optional[torch.Tensor]:
         """Packed per-incidence column positions for the fused P1 path, or

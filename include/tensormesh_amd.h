@@ -398,7 +398,8 @@ typedef struct tm_contact_plane {
  *                        fpat->seg_slot list, per pattern entry, the facet
  *                        slots f*B*B + b*B + c that land on it (ascending);
  *                        entries with none come out 0.  tslot_ws [n_sel*B*B*D*D]
- * facet_tab_dev / n_quad as tm_assemble_facet_linear. */
+ * facet_tab_dev / n_quad as tm_assemble_facet_linear.  With no selected facet
+ * (n_sel = 0) the requested outputs are set to zero. */
 int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view* facets, const double* facet_tab_dev,
                        int32_t n_quad, const tm_contact_plane* plane, const void* x, void* energy_ws,
                        void* energy, const tm_incidence_view* inc, void* rslot_ws, void* grad,
@@ -470,7 +471,8 @@ int tm_condense(int32_t dtype, int64_t n_inner, int64_t n_kept, const void* valu
  * cell basis at the facet points mapped into the reference cell, and the
  * reference tangents of each facet).  inc: the facet-node incidence lists of
  * the selected facets' cell nodes (tm_incidence_build on elem's rows, entry
- * facet*B + b), slot_ws [n_sel*B*n_comp], out [n_points*n_comp].  n_comp <= 4. */
+ * facet*B + b), slot_ws [n_sel*B*n_comp], out [n_points*n_comp].  n_comp <= 4.
+ * With no selected facet (n_sel = 0) out is set to zero. */
 int tm_assemble_facet_linear(const tm_mesh_view* mesh, const tm_facet_view* facets, const double* facet_tab_dev,
                              int32_t n_quad, int32_t n_comp, const void* data, const double* cval_dev,
                              const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream);

@@ -280,6 +280,9 @@ class ElementAssembler(_AssemblerBase):
         if coeff is not None:
             nat.require_device(coeff, match.coeff_key)
             coeff = coeff.detach().to(dtype).contiguous()
+        if E == 0 or pat.nnz == 0:
+            values = torch.zeros(pat.nnz * H * H, dtype=dtype, device=dev)
+            return values if H == 1 else values.reshape(pat.nnz, H, H)
         tab, tab_dev = self._tables(dev)
         mesh = self._mesh_view(points)
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
@@ -684,6 +687,8 @@ class NodeAssembler(_AssemblerBase):
         points = points.detach().contiguous()
         conn = self._conn32(dev)
         E, B = conn.shape
+        if E == 0:
+            return torch.zeros(points.shape[0], dtype=points.dtype, device=dev)
         if f is not None:
             nat.require_device(f, "point data")
             f = f.detach().to(points.dtype).contiguous()

@@ -186,6 +186,14 @@ extern "C" int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view*
   if (grad && (!inc || !rslot_ws)) return tmh::fail(TM_ERR_ARG, "gradient needs incidence and a workspace");
   if (hess && (!fpat || !tslot_ws)) return tmh::fail(TM_ERR_ARG, "Hessian needs facet-slot segments and a workspace");
   if (!(plane->kappa >= 0.0)) return tmh::fail(TM_ERR_ARG, "penalty must be non-negative");
+  if (facets->n_sel == 0) {
+    // no facet selected: zero energy, gradient and Hessian
+    const int64_t s_val = mesh->dtype == TM_F64 ? 8 : 4;
+    int st = energy ? tmh::zero_fill(energy, s_val, stream_) : TM_OK;
+    if (!st && grad) st = tmh::zero_fill(grad, mesh->n_points * D * s_val, stream_);
+    if (!st && hess) st = tmh::zero_fill(hess, fpat->nnz * D * D * s_val, stream_);
+    return st;
+  }
   Plane pl{plane->kappa, {plane->normal[0], plane->normal[1], plane->normal[2]}, plane->offset};
   double* e_out = energy ? static_cast<double*>(energy_ws) : nullptr;
   void* rs = grad ? rslot_ws : nullptr;

@@ -150,7 +150,9 @@ extern "C" int tm_assemble_facet_linear(const tm_mesh_view* mesh, const tm_facet
   const int64_t B = tmh::cell_basis(mesh->cell);
   if (B < 0) return tmh::fail(TM_ERR_ARG, "unknown cell");
   if (facets->n_sel < 0 || facets->n_sel * B * n_comp >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "too large");
-  if (facets->n_sel > 0 && (!facets->elem || !facets->local || !slot_ws || !mesh->conn || !mesh->points))
+  if (facets->n_sel == 0)  // no facet selected: a zero load
+    return tmh::zero_fill(out, mesh->n_points * n_comp * (mesh->dtype == TM_F64 ? 8 : 4), stream_);
+  if (!facets->elem || !facets->local || !slot_ws || !mesh->conn || !mesh->points)
     return tmh::fail(TM_ERR_ARG, "NULL buffer");
   int st = mesh->dtype == TM_F64
                ? dispatch_facet<double>(mesh, facets, facet_tab_dev, n_quad, n_comp, data, cval_dev, slot_ws, s)

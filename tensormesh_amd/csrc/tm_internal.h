@@ -58,6 +58,8 @@ constexpr int64_t kMaxIndex = 2147483647LL;
 // Fixed-order sum of v[0, n) into out[0] (mesh dtype); part: kSumParts doubles.
 constexpr int kSumParts = 1024;
 int sum_fixed(const double* v, int64_t n, double* part, int32_t dtype, void* out, tm_stream_t stream);
+// Zeroes `bytes` (a multiple of 4) of device memory with a kernel.
+int zero_fill(void* p, int64_t bytes, tm_stream_t stream);
 
 }  // namespace tmh
 

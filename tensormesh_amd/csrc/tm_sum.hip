@@ -2,6 +2,8 @@
 // kSumParts sums a contiguous chunk (strided per thread, then an LDS tree)
 // and one block sums the partials, so the result does not depend on
 // scheduling.  Used for the scalar energies (J2 plasticity, penalty contact).
+// Also a zero-fill kernel for outputs that receive no contribution (used in
+// place of a memset, so the calls stay safe to capture in a graph).
 #include <hip/hip_runtime.h>
 
 #include "tm_internal.h"
@@ -41,9 +43,28 @@ __global__ void __launch_bounds__(kSumBlock) sum_final_kernel(const double* __re
   if (threadIdx.x == 0) out[0] = (T)tot;
 }
 
+__global__ void __launch_bounds__(kSumBlock) zero_kernel(uint32_t* __restrict__ p, int64_t n_words) {
+  const int64_t i = (int64_t)blockIdx.x * kSumBlock + threadIdx.x;
+  if (i < n_words) p[i] = 0u;
+}
+
 }  // namespace
 
 namespace tmh {
+
+int zero_fill(void* p, int64_t bytes, tm_stream_t stream) {
+  if (bytes <= 0) return TM_OK;
+  if (!p || bytes % 4 != 0) return fail(TM_ERR_ARG, "zero fill needs a buffer of whole 4-byte words");
+  const int64_t words = bytes / 4;
+  zero_kernel<<<(unsigned)((words + kSumBlock - 1) / kSumBlock), kSumBlock, 0,
+                reinterpret_cast<hipStream_t>(stream)>>>(static_cast<uint32_t*>(p), words);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    set_error("zero fill launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+}
 
 int sum_fixed(const double* v, int64_t n, double* part, int32_t dtype, void* out, tm_stream_t stream) {
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -172,6 +172,8 @@ class Pattern:
         dev = conn.device
         c = conn.to(torch.int32).contiguous()
         E, B = c.shape
+        if E == 0:
+            return None   # no elements: the host builder gives the empty pattern
         st = nat.stream_of(dev)
         i32 = lambda n: torch.empty(n, dtype=torch.int32, device=dev)
         inc_ptr, inc, row_ptr, work = i32(n_points + 1), i32(E * B), i32(n_points + 1), i32(n_points + 1)

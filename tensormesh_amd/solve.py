@@ -18,6 +18,7 @@ tensormesh/sparse/solve/torch_solve.py:54-128 with the adjoint
 from __future__ import annotations
 
 import ctypes
+import warnings
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -234,7 +235,9 @@ def cg(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, rtol: float = 1e-1
     if A.device.type == "cuda" and A.row_ptr is not None and b.dim() == 1:
         solver = _solver_for(A, rtol)
         x, it, rel = solver.solve(b.detach(), x0=None if x0 is None else x0.detach(), max_iter=max_iter)
-        return x, int(it[0].item()), float(rel[0].item())
+        it, rel = int(it[0].item()), float(rel[0].item())
+        _warn_unconverged(it, rel, rtol, atol, b)
+        return x, it, rel
     vals = A.values.detach()
     row, col = A.row_indices, A.col_indices
     n = A.shape[0]
@@ -268,7 +271,16 @@ def cg(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, rtol: float = 1e-1
         p = z + (rz_new / rz) * p
         rz = rz_new
     rnorm = torch.linalg.vector_norm(b - _spmv_raw(vals, row, col, n, x)).item()
+    _warn_unconverged(it, rnorm / max(bnorm, 1e-300), rtol, atol, b)
     return x, it, rnorm / max(bnorm, 1e-300)
+
+
+def _warn_unconverged(it: int, rel: float, rtol: float, atol: float, b: torch.Tensor) -> None:
+    """Warn as the reference's CG does when the iteration budget runs out
+    (torch_solve.py:125-126)."""
+    bnorm = torch.linalg.vector_norm(b.detach()).item()
+    if not (rel <= rtol or rel * bnorm <= atol):
+        warnings.warn(f"cg did not converge after {it} iterations. with relative residual {rel:.3e}")
 
 
 class _Solve(torch.autograd.Function):

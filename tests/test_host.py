@@ -467,3 +467,21 @@ def test_pcg_chunk_rule():
     for n in (1, 50, 300, 1000, 5000, 10 ** 6):
         c = pcg_chunk(n)
         assert c in (8, 16, 32, 64, 128) and c % 2 == 0
+
+
+def test_cg_warns_when_the_budget_runs_out():
+    """Host CG on a small SPD system: converged solves are silent, a starved
+    iteration budget warns (torch_solve.py:125-126)."""
+    import warnings as _w
+    from tensormesh_amd.solve import cg
+    m = unit_square_tri(6, 0.1, 2)
+    A = tm.LaplaceElementAssembler.from_mesh(m)
+    A.reference_path = True
+    K = A()
+    Ki, fi = tm.Condenser(m.boundary_mask)(K, torch.ones(m.n_points, dtype=torch.float64))
+    with _w.catch_warnings():
+        _w.simplefilter("error")
+        x, it, rel = cg(Ki, fi, rtol=1e-10, check_every=1)
+    assert rel <= 1e-10
+    with pytest.warns(UserWarning, match="did not converge"):
+        cg(Ki, fi, rtol=1e-12, max_iter=2, check_every=1)

@@ -49,6 +49,28 @@ facet loads                 Gauss-Legendre exactness on monomials; facet
                             face x = 0: area 1, integral of y = 1/2
 condense / recover          tests/operator/test_condenser.py 4x4 system
 adjoint gradients           central finite differences; dense solve autograd
+j2 (psi / derivatives /     yield stress -> inf: energy and Hessian equal
+update)                     linear elasticity (2-D plane strain and 3-D);
+                            simple shear closed form below and past yield;
+                            residual / tangent == central FD past yield; the
+                            radial return lands on the updated yield surface,
+                            keeps eps_p trace-free, alpha grows by
+                            sqrt(2/3)|d eps_p|, elastic points unchanged
+contact (penalty)           per-facet adaptive quadrature (scipy quad /
+                            dblquad) on straight facets; zero when clear of
+                            the plane; gradient / Hessian == central FD
+quadrilaterals, hexahedra   tensor rule exact on x^a y^b z^c, a,b,c <= 2n-1;
+                            Kronecker / partition of unity; Q1 unit-cell
+                            stiffness and mass as Kronecker products of the
+                            1-D matrices; patch test on jittered grids;
+                            rigid-body modes; boundary integrals 4 / 6 and 6 /
+                            18 (f = 1 / f = x + 2y (+ 3z))
+cubic simplices             Kronecker / partition of unity; patch test and
+                            rigid modes on jittered P3 meshes; degree-6 rule
+                            integrates the P3 mass to the cell volume; the
+                            reference's rules reach the degrees recorded in
+                            tests (tetrahedron "order 5" / "order 6" reach
+                            degrees 4 / 5)
 ==========================  =================================================
 
 Every function above has at least one such pin; none is "parity unpinned".

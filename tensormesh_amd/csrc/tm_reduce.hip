@@ -95,7 +95,28 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
     for (; t < n; t += kBlock) lds[t] = slot[seg_slot[cb + t]];
   } else {
-    for (int32_t u = threadIdx.x; u < n * HH; u += kBlock) {
+    // consecutive threads read consecutive values of a slot; kGather
+    // independent (index, value) gathers per iteration keep the memory
+    // pipeline full
+    constexpr int kGather = 4;
+    const int32_t nv = n * HH;
+    int32_t u = threadIdx.x;
+    for (; u + (kGather - 1) * kBlock < nv; u += kGather * kBlock) {
+      int32_t t[kGather], h[kGather], sl[kGather];
+#pragma unroll
+      for (int k = 0; k < kGather; ++k) {
+        t[k] = (u + k * kBlock) / HH;
+        h[k] = u + k * kBlock - t[k] * HH;
+      }
+#pragma unroll
+      for (int k = 0; k < kGather; ++k) sl[k] = seg_slot[cb + t[k]];
+      T v[kGather];
+#pragma unroll
+      for (int k = 0; k < kGather; ++k) v[k] = slot[(int64_t)sl[k] * HH + h[k]];
+#pragma unroll
+      for (int k = 0; k < kGather; ++k) lds[u + k * kBlock] = v[k];
+    }
+    for (; u < nv; u += kBlock) {
       const int32_t t = u / HH, h = u - t * HH;
       lds[u] = slot[(int64_t)seg_slot[cb + t] * HH + h];
     }

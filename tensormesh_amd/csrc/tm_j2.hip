@@ -45,36 +45,42 @@ struct J2Consts {
 // EPW elements per wavefront, each on L = 64 / EPW lanes (EPW = 4 packs the
 // small cells, whose stages use at most 16 lanes).
 // TAN: the launch assembles the tangent (its LDS staging exists only then).
-template <typename T, int D, int B, int MQ, int EPW, bool TAN>
-__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+// BIG (tetra20, hexahedron27): one wavefront per workgroup, rule tables read
+// through the caches instead of LDS, tangent blocks stored from registers.
+template <typename T, int D, int B, int MQ, int EPW, bool TAN, bool BIG>
+__global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     j2_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                    const double* __restrict__ tab, int Q, J2Consts k, int64_t n_elem, T* __restrict__ eps_p,
                    T* __restrict__ alpha, int update, double* __restrict__ e_out, T* __restrict__ rslot,
                    T* __restrict__ tslot) {
   constexpr int kRowVals = B * D * D;
-  __shared__ double s_tab[MQ * (1 + B + D * B)];
-  __shared__ double s_x[kWavesPerBlock * EPW][B * D];
-  __shared__ double s_u[kWavesPerBlock * EPW][B * D];
-  __shared__ double s_inv[kWavesPerBlock * EPW][MQ][D * D];
-  __shared__ double s_jxw[kWavesPerBlock * EPW][MQ];
-  __shared__ double s_g[kWavesPerBlock * EPW][MQ * B * D];
-  __shared__ double s_h[kWavesPerBlock * EPW][TAN ? MQ * B * D : 1];
-  __shared__ double s_P[kWavesPerBlock * EPW][MQ][D * D];
-  __shared__ double s_m[kWavesPerBlock * EPW][MQ][D * D];
-  __shared__ double s_sc[kWavesPerBlock * EPW][MQ][4];  // K - c1/3, c1/2, c2, psi jxw
-  __shared__ int s_v[kWavesPerBlock * EPW][B];
-  __shared__ T s_out[kWavesPerBlock * EPW][TAN ? B * kRowVals : 1];
+  constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
+  static_assert(!BIG || EPW == 1, "one element per wavefront");
+  __shared__ double s_tab[BIG ? 1 : MQ * (1 + B + D * B)];
+  __shared__ double s_x[kWpb * EPW][B * D];
+  __shared__ double s_u[kWpb * EPW][B * D];
+  __shared__ double s_inv[kWpb * EPW][MQ][D * D];
+  __shared__ double s_jxw[kWpb * EPW][MQ];
+  __shared__ double s_g[kWpb * EPW][MQ * B * D];
+  __shared__ double s_h[kWpb * EPW][TAN ? MQ * B * D : 1];
+  __shared__ double s_P[kWpb * EPW][MQ][D * D];
+  __shared__ double s_m[kWpb * EPW][MQ][D * D];
+  __shared__ double s_sc[kWpb * EPW][MQ][4];  // K - c1/3, c1/2, c2, psi jxw
+  __shared__ int s_v[kWpb * EPW][B];
+  __shared__ T s_out[kWpb * EPW][TAN && !BIG ? B * kRowVals : 1];
 
   constexpr int L = kWave / EPW;
   static_assert(B <= L && MQ <= L, "an element's lanes index its nodes and its quadrature points");
   const int lane = threadIdx.x % L;   // lane within the element's group
   const int wv = threadIdx.x / L;     // element slot within the workgroup
-  const int64_t e = (int64_t)blockIdx.x * (kWavesPerBlock * EPW) + wv;
+  const int64_t e = (int64_t)blockIdx.x * (kWpb * EPW) + wv;
   const bool active = e < n_elem;
   const int tab_len = Q + Q * B + Q * D * B;
-  for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
-  const double* w = s_tab;
-  const double* dphi = s_tab + Q + Q * B;
+  if constexpr (!BIG)
+    for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
+  const double* tb = BIG ? tab : s_tab;
+  const double* w = tb;
+  const double* dphi = tb + Q + Q * B;
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
   __syncthreads();
   if (active) {
@@ -267,24 +273,26 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
             acc[a * D + b] += lw * vgi[a] * vgj[b] + mw * vgi[b] * vgj[a] - cw * vhi[a] * vhj[b] +
                               (a == b ? mw * dot : 0.0);
       }
-      T* out = &s_out[wv][(i * B + j) * DD];
+      T* out = BIG ? tslot + (e * B + i) * kRowVals + j * DD : &s_out[wv][(i * B + j) * DD];
 #pragma unroll
       for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
     }
   }
-  __syncthreads();
-  if (TAN && !update && active) {
-    // whole slot rows (B blocks of D x D) with vector stores: 16 bytes when a
-    // row's size allows, else 8 (slot buffers are 256-byte aligned)
-    constexpr int kRowBytes = kRowVals * (int)sizeof(T);
-    constexpr int kVec = kRowBytes % 16 == 0 ? 16 : 8;
-    static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
-    constexpr int kPerRow = kRowBytes / kVec;
-    typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
-    for (int v = lane; v < B * kPerRow; v += L) {
-      const int i = v / kPerRow, o = v % kPerRow;
-      const vec_t val = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
-      reinterpret_cast<vec_t*>(tslot + (e * B + i) * kRowVals)[o] = val;
+  if constexpr (TAN && !BIG) {
+    __syncthreads();
+    if (!update && active) {
+      // whole slot rows (B blocks of D x D) with vector stores: 16 bytes when
+      // a row's size allows, else 8 (slot buffers are 256-byte aligned)
+      constexpr int kRowBytes = kRowVals * (int)sizeof(T);
+      constexpr int kVec = kRowBytes % 16 == 0 ? 16 : 8;
+      static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
+      constexpr int kPerRow = kRowBytes / kVec;
+      typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
+      for (int v = lane; v < B * kPerRow; v += L) {
+        const int i = v / kPerRow, o = v % kPerRow;
+        const vec_t val = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
+        reinterpret_cast<vec_t*>(tslot + (e * B + i) * kRowVals)[o] = val;
+      }
     }
   }
 }
@@ -292,19 +300,23 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 template <typename T, int D, int B>
 int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, const void* disp,
               const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s) {
+  // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
+  // per workgroup, the BIG variant, with rules of up to 32 points
+  constexpr bool kBig = B * D > 32;
+  constexpr int kWpb = kBig ? 1 : kWavesPerBlock;
   const int Q = f->n_quad;
-  if (Q <= 0 || Q > 16) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
+  if (Q <= 0 || Q > (kBig ? 32 : 16)) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   // four small elements (B <= 4) per wavefront when the rule has at most 4 points
   const int epw = (B <= 4 && Q <= 4) ? 4 : 1;
-  const int64_t per_block = (int64_t)kWavesPerBlock * epw;
+  const int64_t per_block = (int64_t)kWpb * epw;
   const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
-  const dim3 g((unsigned)grid), blk(kWave * kWavesPerBlock);
-#define TM_J2_TAN(MQ, EPW, TAN)                                                                         \
-  j2_wave_kernel<T, D, B, MQ, EPW, TAN><<<g, blk, 0, s>>>(m->conn, (const T*)m->points, (const T*)disp,  \
-                                                          f->tab_dev, Q, k, m->n_elem, (T*)st->eps_p,     \
-                                                          (T*)st->alpha, update, e_out, (T*)rslot, (T*)tslot)
+  const dim3 g((unsigned)grid), blk(kWave * kWpb);
+#define TM_J2_TAN(MQ, EPW, TAN)                                                                        \
+  j2_wave_kernel<T, D, B, MQ, EPW, TAN, kBig><<<g, blk, 0, s>>>(                                       \
+      m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, k, m->n_elem, (T*)st->eps_p,       \
+      (T*)st->alpha, update, e_out, (T*)rslot, (T*)tslot)
 #define TM_J2(MQ, EPW)            \
   do {                            \
     if (tslot && !update)         \
@@ -312,7 +324,9 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
     else                          \
       TM_J2_TAN(MQ, EPW, false);  \
   } while (0)
-  if constexpr (B <= 4) {
+  if constexpr (kBig) {
+    if (Q <= 16) TM_J2(16, 1); else TM_J2(32, 1);
+  } else if constexpr (B <= 4) {
     if (epw == 4) TM_J2(4, 4); else if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
   } else {
     if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
@@ -339,10 +353,8 @@ int dispatch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k,
     case TM_QUAD9: return launch_j2<T, 2, 9>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     case TM_HEX8: return launch_j2<T, 3, 8>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     case TM_TRI10: return launch_j2<T, 2, 10>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_TET20:
-      return tmh::fail(TM_ERR_UNSUPPORTED, "tetra20: the 60 x 60 element tangent does not fit the kernel's LDS");
-    case TM_HEX27:
-      return tmh::fail(TM_ERR_UNSUPPORTED, "hexahedron27: the 81 x 81 element tangent does not fit the kernel's LDS");
+    case TM_TET20: return launch_j2<T, 3, 20>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+    case TM_HEX27: return launch_j2<T, 3, 27>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }

@@ -201,35 +201,43 @@ typedef float tm_floatx16 __attribute__((ext_vector_type(16)));
 // of each element is staged in LDS and written as whole slot rows (B blocks of
 // D x D values each) with vector stores; row (e, i) goes to e*B + i, or with
 // inc_inv to its position inc_inv[e*B + i] in the node-sorted incidence list.
-template <typename T, int D, int B, bool MFMA, int MQ>
-__global__ void __launch_bounds__(kWave * kWavesPerBlock)
+// BIG (tetra20, hexahedron27: element tangents of 60 x 60 and 81 x 81): one
+// wavefront per workgroup, the rule tables read through the caches instead of
+// LDS, and each lane's D x D blocks stored from registers, so that a wave
+// needs LDS only for its gradients (2 Q B D doubles).
+template <typename T, int D, int B, bool MFMA, int MQ, bool BIG>
+__global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem,
                       T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv) {
   constexpr int kMaxQ = MQ;
   constexpr int kRowVals = B * D * D;  // values of one slot row
-  __shared__ double s_tab[MQ * (1 + B + D * B)];
-  __shared__ double s_x[kWavesPerBlock][B * D];
-  __shared__ double s_u[kWavesPerBlock][B * D];
-  __shared__ double s_inv[kWavesPerBlock][kMaxQ][D * D];
-  __shared__ double s_jxw[kWavesPerBlock][kMaxQ];
-  __shared__ double s_g[kWavesPerBlock][kMaxQ * B * D];
-  __shared__ double s_h[kWavesPerBlock][kMaxQ * B * D];
-  __shared__ double s_P[kWavesPerBlock][kMaxQ][D * D];
-  __shared__ double s_c2[kWavesPerBlock][kMaxQ];
-  __shared__ double s_Fi[kWavesPerBlock][kMaxQ][D * D];
-  __shared__ int s_v[kWavesPerBlock][B];
-  __shared__ T s_out[kWavesPerBlock][B * kRowVals];
+  constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
+  static_assert(!(BIG && MFMA), "the MFMA tile holds at most 32 x 32");
+  __shared__ double s_tab[BIG ? 1 : MQ * (1 + B + D * B)];
+  __shared__ double s_x[kWpb][B * D];
+  __shared__ double s_u[kWpb][B * D];
+  __shared__ double s_inv[kWpb][kMaxQ][D * D];
+  __shared__ double s_jxw[kWpb][kMaxQ];
+  __shared__ double s_g[kWpb][kMaxQ * B * D];
+  __shared__ double s_h[kWpb][kMaxQ * B * D];
+  __shared__ double s_P[kWpb][kMaxQ][D * D];
+  __shared__ double s_c2[kWpb][kMaxQ];
+  __shared__ double s_Fi[kWpb][kMaxQ][D * D];
+  __shared__ int s_v[kWpb][B];
+  __shared__ T s_out[kWpb][BIG ? 1 : B * kRowVals];
 
   static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
   const int lane = threadIdx.x % kWave;
   const int wv = threadIdx.x / kWave;
-  const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  const int64_t e = (int64_t)blockIdx.x * kWpb + wv;
   const bool active = e < n_elem;
   const int tab_len = Q + Q * B + Q * D * B;
-  for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
-  const double* w = s_tab;
-  const double* dphi = s_tab + Q + Q * B;
+  if constexpr (!BIG)
+    for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
+  const double* tb = BIG ? tab : s_tab;
+  const double* w = tb;
+  const double* dphi = tb + Q + Q * B;
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
   __syncthreads();
   if (active) {
@@ -417,7 +425,13 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     constexpr int DD = D * D;
     for (int ij = lane; ij < B * B; ij += kWave) {
       const int i = ij / B, j = ij % B;
-      T* out = &s_out[wv][(i * B + j) * DD];
+      T* out;
+      if constexpr (BIG) {
+        const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
+        out = tslot + srow * kRowVals + j * DD;
+      } else {
+        out = &s_out[wv][(i * B + j) * DD];
+      }
       double acc[DD];
 #pragma unroll
       for (int k = 0; k < DD; ++k) acc[k] = 0.0;
@@ -450,20 +464,22 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
     }
   }
-  __syncthreads();
-  if (tslot && active) {
-    // whole slot rows with vector stores: 16 bytes when a row's size allows,
-    // else 8 (slot buffers are 256-byte aligned)
-    constexpr int kRowBytes = kRowVals * (int)sizeof(T);
-    constexpr int kVec = kRowBytes % 16 == 0 ? 16 : 8;
-    static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
-    constexpr int kPerRow = kRowBytes / kVec;
-    typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
-    for (int v = lane; v < B * kPerRow; v += kWave) {
-      const int i = v / kPerRow, o = v % kPerRow;
-      const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
-      const vec_t x = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
-      reinterpret_cast<vec_t*>(tslot + srow * kRowVals)[o] = x;
+  if constexpr (!BIG) {
+    __syncthreads();
+    if (tslot && active) {
+      // whole slot rows with vector stores: 16 bytes when a row's size allows,
+      // else 8 (slot buffers are 256-byte aligned)
+      constexpr int kRowBytes = kRowVals * (int)sizeof(T);
+      constexpr int kVec = kRowBytes % 16 == 0 ? 16 : 8;
+      static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
+      constexpr int kPerRow = kRowBytes / kVec;
+      typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
+      for (int v = lane; v < B * kPerRow; v += kWave) {
+        const int i = v / kPerRow, o = v % kPerRow;
+        const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
+        const vec_t x = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
+        reinterpret_cast<vec_t*>(tslot + srow * kRowVals)[o] = x;
+      }
     }
   }
 }
@@ -471,10 +487,15 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 template <typename T, int D, int B>
 int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
                  const int32_t* inc_inv, hipStream_t stream) {
+  // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
+  // per workgroup, the BIG variant
+  constexpr bool kBig = B * D > 32;
+  constexpr int kWpb = kBig ? 1 : kWavesPerBlock;
   const int Q = f->n_quad;
-  if (Q <= 0 || Q > kMaxQ) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
+  if (Q <= 0 || Q > (kBig ? kMaxQWave : kMaxQ))
+    return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
-  const int64_t grid = (m->n_elem + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t grid = (m->n_elem + kWpb - 1) / kWpb;
   if (grid == 0) return TM_OK;
   const double mu = f->young / (2.0 * (1.0 + f->poisson));
   const double lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
@@ -482,12 +503,14 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   if (mfma && !(std::is_same<T, float>::value && B * D <= 32))
     return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
   if (mfma && (Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
-  const dim3 g((unsigned)grid), blk(kWave * kWavesPerBlock);
-#define TM_HYPER(MF, MQ)                                                                                   \
-  hyper_wave_kernel<T, D, B, MF, MQ><<<g, blk, 0, stream>>>(m->conn, (const T*)m->points, (const T*)disp, \
-                                                            f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot,  \
-                                                            (T*)rslot, inc_inv)
-  if constexpr (std::is_same<T, float>::value && B * D <= 32) {
+  const dim3 g((unsigned)grid), blk(kWave * kWpb);
+#define TM_HYPER(MF, MQ)                                                                            \
+  hyper_wave_kernel<T, D, B, MF, MQ, kBig><<<g, blk, 0, stream>>>(                                  \
+      m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, \
+      (T*)rslot, inc_inv)
+  if constexpr (kBig) {
+    if (Q <= 16) TM_HYPER(false, 16); else TM_HYPER(false, 32);
+  } else if constexpr (std::is_same<T, float>::value && B * D <= 32) {
     if (mfma) {
       if (Q <= 4) TM_HYPER(true, 4); else TM_HYPER(true, 16);
     } else {
@@ -518,10 +541,8 @@ int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* dis
     case TM_QUAD9: return launch_hyper<T, 2, 9>(m, f, disp, tslot, rslot, inc_inv, s);
     case TM_HEX8: return launch_hyper<T, 3, 8>(m, f, disp, tslot, rslot, inc_inv, s);
     case TM_TRI10: return launch_hyper<T, 2, 10>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_TET20:
-      return tmh::fail(TM_ERR_UNSUPPORTED, "tetra20: the 60 x 60 element tangent does not fit the kernel's LDS");
-    case TM_HEX27:
-      return tmh::fail(TM_ERR_UNSUPPORTED, "hexahedron27: the 81 x 81 element tangent does not fit the kernel's LDS");
+    case TM_TET20: return launch_hyper<T, 3, 20>(m, f, disp, tslot, rslot, inc_inv, s);
+    case TM_HEX27: return launch_hyper<T, 3, 27>(m, f, disp, tslot, rslot, inc_inv, s);
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }

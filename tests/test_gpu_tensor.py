@@ -15,7 +15,6 @@ import torch
 
 import tensormesh_amd as tm
 from oracle import fem
-from tensormesh_amd import _native as nat
 from tensormesh_amd.mesh import unit_cube_hex, unit_cube_tet, unit_square_quad, unit_square_tri
 
 pytestmark = pytest.mark.gpu
@@ -106,21 +105,25 @@ def test_tensor_loads_and_facet_loads_against_oracle(etype):
     assert ok, f"facet load: {err:.2e}"
 
 
-@pytest.mark.parametrize("etype", ["quad", "quad9", "hexahedron", "triangle10"])
-def test_tensor_neohookean_against_oracle(etype):
-    say(f"case tensor neo-hookean {etype}")
+# tetra20 and hexahedron27 (60 and 81 element dofs) run the one-element-per-
+# workgroup kernels; hexahedron27 at order 3 (27 points) and tetra20 at order 6
+# (15 points) fill their larger quadrature arrays
+@pytest.mark.parametrize("etype,qo", [("quad", 2), ("quad9", 2), ("hexahedron", 2), ("triangle10", 2),
+                                      ("hexahedron27", 2), ("hexahedron27", 3), ("tetra20", 2), ("tetra20", 6)])
+def test_tensor_neohookean_against_oracle(etype, qo):
+    say(f"case tensor neo-hookean {etype} qorder {qo}")
     m = _mesh(etype, n=3 if etype.startswith(("quad", "tri")) else 2)
     P = m.points
     g = torch.Generator().manual_seed(5)
     u = 0.05 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, -1], 0.5 * P[:, 0] ** 2][:P.shape[1]], 1)
     u = u + 0.002 * torch.randn(P.shape, generator=g, dtype=torch.float64)
-    res_e, tan_e = fem.neohookean_element_tangent(P, m.elements(), etype, 2, u, E=1.0, nu=0.3)
+    res_e, tan_e = fem.neohookean_element_tangent(P, m.elements(), etype, qo, u, E=1.0, nu=0.3)
     D = P.shape[1]
     res = torch.zeros(m.n_points, D, dtype=torch.float64).index_add(0, m.elements().reshape(-1),
                                                                      res_e.reshape(-1, D)).reshape(-1)
     tan, row, col = _assembled(tan_e, m.elements(), m.n_points)
     v, r, c = fem.block_coo(tan, torch.from_numpy(row), torch.from_numpy(col))
-    model = tm.NeoHookeanModel.from_mesh(m.to(DEV), quadrature_order=2, E=1.0, nu=0.3)
+    model = tm.NeoHookeanModel.from_mesh(m.to(DEV), quadrature_order=qo, E=1.0, nu=0.3)
     R, K = model.residual_and_tangent(u.to(DEV))
     ok, err = _within(R, res, 1e-11, 1e-13)
     assert ok, f"residual: {err:.2e}"
@@ -129,31 +132,28 @@ def test_tensor_neohookean_against_oracle(etype):
     assert ok, f"tangent: {err:.2e}"
 
 
-@pytest.mark.parametrize("etype", ["hexahedron27", "tetra20"])
-def test_large_cells_neohookean_is_rejected(etype):
-    m = _mesh(etype, n=1).to(DEV)
-    model = tm.NeoHookeanModel.from_mesh(m, quadrature_order=2)
-    with pytest.raises(nat.NativeError, match=etype):
-        model.residual_and_tangent(torch.zeros_like(m.points))
+# displacement amplitudes that leave about a third to two fifths of the points yielding
+J2_AMP = {"hexahedron": 3e-4, "hexahedron27": 1e-4, "tetra20": 5e-5}
 
 
-def test_hexahedron_j2_against_oracle():
-    say("case tensor J2 hexahedron")
+@pytest.mark.parametrize("etype,qo", [("hexahedron", 2), ("hexahedron27", 3), ("tetra20", 4)])
+def test_tensor_j2_against_oracle(etype, qo):
+    say(f"case tensor J2 {etype} qorder {qo}")
     prm = dict(E=200.0, nu=0.3, sig0=0.25, H=1.0)
-    m = _mesh("hexahedron", n=2, seed=3)
+    m = _mesh(etype, n=2, seed=3)
     C = m.elements()
-    Q = fem.quadrature("hexahedron", 2)[0].shape[0]
+    Q = fem.quadrature(etype, qo)[0].shape[0]
     g = torch.Generator().manual_seed(7)
     ep = 1e-4 * torch.randn(C.shape[0], Q, 3, 3, generator=g, dtype=torch.float64)
     ep = 0.5 * (ep + ep.transpose(-1, -2))
     al = 1e-4 * torch.rand(C.shape[0], Q, generator=g, dtype=torch.float64)
-    u = 6e-4 * torch.randn(m.points.shape, generator=g, dtype=torch.float64)
-    e_ref = fem.j2_energy(m.points, C, "hexahedron", 2, u, ep, al, **prm)
-    res_e, tan_e = fem.j2_element_derivatives(m.points, C, "hexahedron", 2, u, ep, al, **prm)
+    u = J2_AMP[etype] * torch.randn(m.points.shape, generator=g, dtype=torch.float64)
+    e_ref = fem.j2_energy(m.points, C, etype, qo, u, ep, al, **prm)
+    res_e, tan_e = fem.j2_element_derivatives(m.points, C, etype, qo, u, ep, al, **prm)
     res = torch.zeros(m.n_points, 3, dtype=torch.float64).index_add(0, C.reshape(-1), res_e.reshape(-1, 3))
     tan, row, col = _assembled(tan_e, C, m.n_points)
     v, r, c = fem.block_coo(tan, torch.from_numpy(row), torch.from_numpy(col))
-    model = tm.J2Plasticity.from_mesh(m.to(DEV), quadrature_order=2, **prm)
+    model = tm.J2Plasticity.from_mesh(m.to(DEV), quadrature_order=qo, **prm)
     hist = {"eps_p_n": ep.to(DEV), "alpha_n": al.to(DEV)}
     e = model.energy(point_data={"displacement": u.to(DEV)}, element_data=hist)
     assert abs(e.item() - e_ref.item()) <= 1e-12 * abs(e_ref.item())

@@ -347,12 +347,15 @@ def grad_step_bench(n=512, steps=2, rtol=1e-10):
     return out
 
 
-def hyperelastic_bench(n=32, reps=5):
+def hyperelastic_bench(n=32, reps=20):
     """Neo-Hookean residual + consistent tangent per Newton step on the unit
     cube split into n^3 x 6 Kuhn tets promoted to tetra10 (32^3 -> 196,608
     elements, 274,625 nodes, 823,875 dofs), E = 1, nu = 0.3, 4-point rule,
     at the displacement u = 0.1 x (the prescribed face value spread linearly).
-    Times float64 and float32 assembly (device time via HIP events)."""
+    Times float64 and float32 assembly (device time via HIP events): the
+    default slot path, the same with symmetric and with full element blocks
+    forced (``*_symmetric_blocks``, ``*_full_blocks``), the block-row path
+    (``*_rows``) and the float32 MFMA tangent."""
     import tensormesh_amd as tm
     from tensormesh_amd.mesh import unit_cube_tet
     m = unit_cube_tet(n, 0.0, 0, order=2).to("cuda")
@@ -361,11 +364,19 @@ def hyperelastic_bench(n=32, reps=5):
     t0 = time.perf_counter()
     model = tm.NeoHookeanModel.from_mesh(m, quadrature_order=2, E=1.0, nu=0.3)
     out["pattern_setup_s"] = time.perf_counter() - t0
-    for dt, mfma, path in ((torch.float64, False, "slots"), (torch.float64, False, "rows"),
-                           (torch.float32, False, "slots"), (torch.float32, False, "rows"),
-                           (torch.float32, True, "slots")):
+    _warm(lambda: model.residual_and_tangent(torch.zeros_like(m.points)))
+    for key, dt, mfma, path, sym in (("f64", torch.float64, False, "slots", None),
+                                     ("f64_symmetric_blocks", torch.float64, False, "slots", True),
+                                     ("f64_full_blocks", torch.float64, False, "slots", False),
+                                     ("f64_rows", torch.float64, False, "rows", None),
+                                     ("f32", torch.float32, False, "slots", None),
+                                     ("f32_symmetric_blocks", torch.float32, False, "slots", True),
+                                     ("f32_full_blocks", torch.float32, False, "slots", False),
+                                     ("f32_rows", torch.float32, False, "rows", None),
+                                     ("f32_mfma", torch.float32, True, "slots", None)):
         model.use_mfma = mfma
         model.nh_path = path
+        model.symmetric_slots = sym
         pts = m.points.to(dt)
         u = torch.zeros_like(pts)
         u[:, 0] = 0.1 * pts[:, 0]
@@ -385,7 +396,6 @@ def hyperelastic_bench(n=32, reps=5):
         byts = (4 * 10 * E + s * 3 * 2 * m.n_points + s * E * 930) + (4 * 100 * E + 4 * (nnzb + 1) + s * E * 900
                                                                        + s * nnzb * 9) \
             + (4 * 10 * E + 4 * (m.n_points + 1) + s * E * 30 + s * 3 * m.n_points)
-        key = ("f64" if dt == torch.float64 else ("f32_mfma" if mfma else "f32")) + ("_rows" if path == "rows" else "")
         out[key] = {
             "ms_per_newton_assembly": ms, "elements_per_s": E / (ms * 1e-3), "block_entries": nnzb,
             "scalar_nnz": nnzb * 9, "algorithmic_bytes": byts, "GBps": byts / (ms * 1e-3) / 1e9,
@@ -472,9 +482,15 @@ def plasticity_bench(n=48, reps=10):
         e.backward()
         return e.detach(), x.grad
 
+    _warm(lambda: model.residual_and_tangent(u))
     e_nat, ms_e = timed(lambda: model.energy(point_data={"displacement": u}, element_data=hist))
     (e1, g_nat), ms_eg = timed(energy_and_grad)
     _, ms_rt = timed(lambda: model.residual_and_tangent(u))
+    alt = {}
+    for name, sym in (("symmetric_blocks", True), ("full_blocks", False)):
+        model.symmetric_slots = sym
+        alt[f"ms_residual_and_tangent_{name}"] = timed(lambda: model.residual_and_tangent(u))[1]
+    model.symmetric_slots = None
     model.reference_path = True
     (e_ref, g_ref), ms_ref = timed(energy_and_grad, 3)
     model.reference_path = False
@@ -485,11 +501,20 @@ def plasticity_bench(n=48, reps=10):
     E = m.elements().shape[0]
     return {"workload": f"J2 plasticity P1 tets {n}^3x6 = {E} elements, {m.n_points} nodes, qorder 2 (4 points), f64",
             "ms_energy": ms_e, "ms_energy_and_backward": ms_eg, "ms_residual_and_tangent": ms_rt,
-            "ms_update_state": ms_up, "reference_algorithm_ms_energy_and_backward": ms_ref,
+            **alt, "ms_update_state": ms_up, "reference_algorithm_ms_energy_and_backward": ms_ref,
             "speedup_energy_and_backward": ms_ref / ms_eg,
             "elements_per_s_residual_and_tangent": E / (ms_rt * 1e-3),
             "energy_rel_diff": abs(e_nat.item() - e_ref.item()) / abs(e_ref.item()),
             "grad_max_rel_diff": float((g_nat - g_ref).abs().max() / g_ref.abs().max())}
+
+
+def _warm(call, seconds=0.2):
+    """Repeat ``call`` untimed for about ``seconds`` so that the first timed
+    variant of a leg does not pay the device's clock ramp."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        call()
+        torch.cuda.synchronize()
 
 
 def _timed(call, reps):
@@ -549,11 +574,18 @@ def hex_bench(n=64, reps=10):
     for name, cls, kw in (("laplace", tm.LaplaceElementAssembler, {}),
                           ("elasticity", tm.LinearElasticityElementAssembler, {"E": 1.0, "nu": 0.3})):
         A = cls.from_mesh(m, quadrature_order=2, **kw)
+        _warm(lambda: A())
         K, ms = _timed(lambda: A(), reps)
+        alt = {}
+        for alt_name, sym in (("symmetric_blocks", True), ("full_blocks", False)):
+            A.symmetric_slots = sym
+            alt[f"ms_per_call_{alt_name}"] = _timed(lambda: A(), reps)[1]
+        A.symmetric_slots = None
         A.reference_path = True
         K_ref, ms_ref = _timed(lambda: A(), 2)
         A.reference_path = False
-        out[name] = {"ms_per_call": ms, "cells_per_s": m.elements().shape[0] / (ms * 1e-3),
+        out[name] = {"ms_per_call": ms, **alt,
+                     "cells_per_s": m.elements().shape[0] / (ms * 1e-3),
                      "reference_algorithm_ms_per_call": ms_ref, "speedup": ms_ref / ms,
                      "max_rel_diff": float((K.values - K_ref.values).abs().max() / K_ref.values.abs().max())}
     return out

@@ -102,6 +102,12 @@ typedef struct tm_form_desc {
 
 /* tm_form_desc.flags */
 #define TM_FLAG_MFMA 1       /* float32 block tangents: contract on v_mfma_f32_32x32x2f32 */
+#define TM_FLAG_UPPER 2      /* symmetric forms on the wave, Neo-Hookean and J2 kernels:
+                                compute and store only the element blocks (i, j) with
+                                i <= j; the pattern's seg_slot must then be the list of
+                                tm_symmetric_segments.  The reference stores every block
+                                (element_assembler.py:811-830); the assembled values are
+                                the same.  Ignored by the P1 kernels, which store all. */
 
 /* Sparsity pattern plus the slot -> nnz permutation in segment form. */
 typedef struct tm_pattern_view {
@@ -316,7 +322,8 @@ int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, c
  * tm_assemble_neohookean: tangent [nnz*D*D] in BSR(D x D) node-pair CSR
  * order, residual [n_points*D]; either may be NULL.  tslot_ws: [n_elem*B*B*D*D],
  * rslot_ws: [n_elem*B*D], in the mesh dtype.  TM_FLAG_MFMA selects the
- * matrix-core tangent (float32).  Replaces the reference's autograd residual /
+ * matrix-core tangent (float32); TM_FLAG_UPPER is rejected (the block rows
+ * are read whole).  Replaces the reference's autograd residual /
  * Hessian of NeoHookeanModel.energy (builtin.py:219-333,
  * element_assembler.py:447-659) with the ReduceProjector sum
  * (projector.py:95-123). */
@@ -436,6 +443,17 @@ int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void
 int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
                         const int32_t* seg_slot, int64_t n_blocks, const int32_t* blk_ptr,
                         const void* slot, void* out, tm_stream_t stream);
+
+/* Segment list for symmetric element blocks (TM_FLAG_UPPER), built once per
+ * pattern: sym_slot[t] = seg_slot[t] where that slot s = e*B*B + i*B + j has
+ * i <= j, else (e*B*B + j*B + i) | 0x80000000 -- the transposed upper block,
+ * which tm_segmented_reduce reads as value b*D + a for value a*D + b of a
+ * D x D entry (hh = 4 or 9; bit 31 only selects the slot for other hh).
+ * Segment order, and so the order of every sum, is unchanged.  No
+ * counterpart in the reference, which stores all B*B blocks
+ * (projector.py:95-123 sums them).  n_slots = n_elem * n_basis^2. */
+int tm_symmetric_segments(int32_t n_basis, int64_t n_slots, const int32_t* seg_slot, int32_t* sym_slot,
+                          tm_stream_t stream);
 
 /* ---------------------------------------------------------------------
  * Linear-form assembly (load vector), two stages on one stream.

@@ -42,6 +42,7 @@ class FormDesc(ctypes.Structure):
 
 
 TM_FLAG_MFMA = 1
+TM_FLAG_UPPER = 2   # symmetric forms: element blocks i <= j only, symmetric segment list
 
 
 class PatternView(ctypes.Structure):
@@ -117,6 +118,8 @@ SIGNATURES = {
                                            ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_void_p, ctypes.c_void_p]),
     "tm_reduce_plan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, c_i64p]),
+    "tm_symmetric_segments": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
     "tm_assemble_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                           ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),

@@ -147,6 +147,18 @@ class ElementAssembler(_AssemblerBase):
     #: reduction (tm_assemble_p1_incidence), "auto" = the fastest of the four on
     #: this device and pattern, timed once and cached (tensormesh_amd.tune).
     p1_path: str = "rows"
+    #: True: symmetric built-in forms on the element-matrix, Neo-Hookean and
+    #: J2 kernels compute and store only the element blocks (i, j) with i <= j
+    #: and the reduction reads the others transposed (TM_FLAG_UPPER with the
+    #: pattern's symmetric segment list); the products are formed so that the
+    #: assembled values are bit for bit those of the full blocks.  False
+    #: stores every block, as the reference does.  None (default) picks per
+    #: kernel what measured faster on MI355X: symmetric for the D x D block
+    #: forms (linear elasticity, the Neo-Hookean and J2 tangents), whose
+    #: element kernels are bound by their block stores; full blocks for scalar
+    #: forms, where the transposed gathers cost more than the halved element
+    #: work saves.
+    symmetric_slots: Optional[bool] = None
     #: Build the pattern on the GPU (tm_pattern_device_*) when the mesh is on the device.
     device_pattern: bool = True
     #: True runs every integrand through the restated reference vmap/autograd
@@ -302,7 +314,11 @@ class ElementAssembler(_AssemblerBase):
                 return values
         slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
         values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
-        pv = pat.view(dtype, H * H)
+        # the element-matrix (wave) kernel serves everything but P1 scalar forms
+        sym = not p1_scalar and self._symmetric_blocks(H > 1)
+        if sym:
+            form.flags |= nat.TM_FLAG_UPPER
+        pv = pat.view(dtype, H * H, symmetric=sym)
         nat.check(L.tm_assemble_bilinear(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(pv),
                                          slot.data_ptr(), values.data_ptr(), nat.stream_of(dev)),
                   "tm_assemble_bilinear")
@@ -339,6 +355,10 @@ class ElementAssembler(_AssemblerBase):
             nat.check(L.tm_reduce_p1_rows(nat.dtype_code(dtype), nat.CELL[self.etype], ctypes.byref(rv),
                                           slot.data_ptr(), 0, values.data_ptr(), st), "tm_reduce_p1_rows")
         return values
+
+    def _symmetric_blocks(self, default: bool) -> bool:
+        """Whether a kernel stores symmetric element blocks (symmetric_slots)."""
+        return default if self.symmetric_slots is None else bool(self.symmetric_slots)
 
     def _native_hyperelastic(self, u: torch.Tensor, points: Optional[torch.Tensor], young: float, poisson: float,
                              want_residual: bool = True, want_tangent: bool = True, path: Optional[str] = None):
@@ -386,7 +406,11 @@ class ElementAssembler(_AssemblerBase):
                       "tm_assemble_neohookean_rows")
             K = self._wrap_output(tan.reshape(pat.nnz, D, D), dev) if want_tangent else None
             return res, K
-        pv = pat.view(dtype, D * D)
+        # the MFMA tile forms every block at once (and not bitwise symmetric): full rows
+        sym = want_tangent and not (form.flags & nat.TM_FLAG_MFMA) and self._symmetric_blocks(True)
+        if sym:
+            form.flags |= nat.TM_FLAG_UPPER
+        pv = pat.view(dtype, D * D, symmetric=sym)
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
         tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None
         tan = torch.empty(pat.nnz * D * D, dtype=dtype, device=dev) if want_tangent else None

@@ -278,9 +278,11 @@ class J2Plasticity(ElementAssembler):
         assert tuple(alpha.shape) == (E, nq), f"alpha_n must be [{E}, {nq}], got {tuple(alpha.shape)}"
         pat = self._pattern_on(dev)
         tab, tab_dev = self._tables(dev)
-        form = nat.FormDesc(0, nq, tab.ctypes.data, tab_dev.data_ptr(), None, 0.0, 0.0, 0)
+        sym = want_tangent and self._symmetric_blocks(True)
+        form = nat.FormDesc(0, nq, tab.ctypes.data, tab_dev.data_ptr(), None, 0.0, 0.0,
+                            nat.TM_FLAG_UPPER if sym else 0)
         st = nat.J2State(eps_p.data_ptr(), alpha.data_ptr())
-        pv = pat.view(dtype, D * D)
+        pv = pat.view(dtype, D * D, symmetric=sym)
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
         new = lambda n, want, dt=dtype: torch.empty(n, dtype=dt, device=dev) if want else None
         ews, energy = new(E + nat.J2_ENERGY_PARTS, want_energy, torch.float64), new((), want_energy)

@@ -27,9 +27,11 @@
 // lane q forms J_q, its inverse and jxw_q; lanes over (q, b, d) form the
 // physical gradients; lane q evaluates the material point (and, in update
 // mode, writes the new history); lanes over (i, a) sum the residual row and
-// lanes over (i, j) the D x D tangent blocks, which leave as whole slot rows.
+// lanes over (i, j) the D x D tangent blocks, each stored from its lane's
+// registers (with TM_FLAG_UPPER only the pairs i <= j).
 #include <hip/hip_runtime.h>
 
+#include "tm_device.h"
 #include "tm_internal.h"
 
 namespace {
@@ -44,15 +46,15 @@ struct J2Consts {
 
 // EPW elements per wavefront, each on L = 64 / EPW lanes (EPW = 4 packs the
 // small cells, whose stages use at most 16 lanes).
-// TAN: the launch assembles the tangent (its LDS staging exists only then).
+// TAN: the launch assembles the tangent (the h = m g arrays exist only then).
 // BIG (tetra20, hexahedron27): one wavefront per workgroup, rule tables read
-// through the caches instead of LDS, tangent blocks stored from registers.
+// through the caches instead of LDS.
 template <typename T, int D, int B, int MQ, int EPW, bool TAN, bool BIG>
 __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     j2_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                    const double* __restrict__ tab, int Q, J2Consts k, int64_t n_elem, T* __restrict__ eps_p,
-                   T* __restrict__ alpha, int update, double* __restrict__ e_out, T* __restrict__ rslot,
-                   T* __restrict__ tslot) {
+                   T* __restrict__ alpha, int update, int upper, double* __restrict__ e_out,
+                   T* __restrict__ rslot, T* __restrict__ tslot) {
   constexpr int kRowVals = B * D * D;
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
   static_assert(!BIG || EPW == 1, "one element per wavefront");
@@ -67,7 +69,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __shared__ double s_m[kWpb * EPW][MQ][D * D];
   __shared__ double s_sc[kWpb * EPW][MQ][4];  // K - c1/3, c1/2, c2, psi jxw
   __shared__ int s_v[kWpb * EPW][B];
-  __shared__ T s_out[kWpb * EPW][TAN && !BIG ? B * kRowVals : 1];
+  __shared__ uint16_t s_pair[TAN ? B * B : 1];
 
   constexpr int L = kWave / EPW;
   static_assert(B <= L && MQ <= L, "an element's lanes index its nodes and its quadrature points");
@@ -78,6 +80,8 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   const int tab_len = Q + Q * B + Q * D * B;
   if constexpr (!BIG)
     for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
+  int n_pairs = 0;
+  if constexpr (TAN) n_pairs = tmh::fill_pairs<B>(s_pair, upper != 0);
   const double* tb = BIG ? tab : s_tab;
   const double* w = tb;
   const double* dphi = tb + Q + Q * B;
@@ -243,8 +247,8 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __syncthreads();
   if (TAN && !update && active) {
     constexpr int DD = D * D;
-    for (int ij = lane; ij < B * B; ij += L) {
-      const int i = ij / B, j = ij % B;
+    for (int p = lane; p < n_pairs; p += L) {
+      const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       double acc[DD];
 #pragma unroll
       for (int c = 0; c < DD; ++c) acc[c] = 0.0;
@@ -270,29 +274,12 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
         for (int a = 0; a < D; ++a)
 #pragma unroll
           for (int b = 0; b < D; ++b)
-            acc[a * D + b] += lw * vgi[a] * vgj[b] + mw * vgi[b] * vgj[a] - cw * vhi[a] * vhj[b] +
+            acc[a * D + b] += lw * (vgi[a] * vgj[b]) + mw * (vgi[b] * vgj[a]) - cw * (vhi[a] * vhj[b]) +
                               (a == b ? mw * dot : 0.0);
       }
-      T* out = BIG ? tslot + (e * B + i) * kRowVals + j * DD : &s_out[wv][(i * B + j) * DD];
+      T* out = tslot + (e * B + i) * kRowVals + j * DD;  // 72 contiguous bytes at f64
 #pragma unroll
       for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
-    }
-  }
-  if constexpr (TAN && !BIG) {
-    __syncthreads();
-    if (!update && active) {
-      // whole slot rows (B blocks of D x D) with vector stores: 16 bytes when
-      // a row's size allows, else 8 (slot buffers are 256-byte aligned)
-      constexpr int kRowBytes = kRowVals * (int)sizeof(T);
-      constexpr int kVec = kRowBytes % 16 == 0 ? 16 : 8;
-      static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
-      constexpr int kPerRow = kRowBytes / kVec;
-      typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
-      for (int v = lane; v < B * kPerRow; v += L) {
-        const int i = v / kPerRow, o = v % kPerRow;
-        const vec_t val = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
-        reinterpret_cast<vec_t*>(tslot + (e * B + i) * kRowVals)[o] = val;
-      }
     }
   }
 }
@@ -313,10 +300,11 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
   const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
   const dim3 g((unsigned)grid), blk(kWave * kWpb);
+  const int upper = (f->flags & TM_FLAG_UPPER) != 0;
 #define TM_J2_TAN(MQ, EPW, TAN)                                                                        \
   j2_wave_kernel<T, D, B, MQ, EPW, TAN, kBig><<<g, blk, 0, s>>>(                                       \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, k, m->n_elem, (T*)st->eps_p,       \
-      (T*)st->alpha, update, e_out, (T*)rslot, (T*)tslot)
+      (T*)st->alpha, update, upper, e_out, (T*)rslot, (T*)tslot)
 #define TM_J2(MQ, EPW)            \
   do {                            \
     if (tslot && !update)         \

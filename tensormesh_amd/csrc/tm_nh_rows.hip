@@ -223,6 +223,8 @@ extern "C" int tm_assemble_neohookean_rows(const tm_mesh_view* mesh, const tm_fo
   TM_TRY_BEGIN
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
   if (!mesh || !form || !rows || !disp || !inc_inv) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  if (form->flags & TM_FLAG_UPPER)
+    return tmh::fail(TM_ERR_ARG, "the block-row reduction reads whole element rows: TM_FLAG_UPPER is not accepted");
   if (!tangent && !residual) return TM_OK;
   if ((tangent && !tslot_ws) || (residual && !rslot_ws)) return tmh::fail(TM_ERR_ARG, "slot workspace missing");
   if (rows->n_blocks > 0 && (!rows->seg_ptr || !rows->pairs || !rows->inc_ptr || !rows->blk_info))

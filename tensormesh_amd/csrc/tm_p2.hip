@@ -24,6 +24,7 @@
 
 #include <type_traits>
 
+#include "tm_device.h"
 #include "tm_internal.h"
 
 namespace {
@@ -33,13 +34,16 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kMaxQ = 16;      // Neo-Hookean kernel
 constexpr int kMaxQWave = 32;  // element-matrix kernel (a 3 x 3 x 3 hexahedron rule has 27 points)
 
-// MQ: largest number of quadrature points the LDS arrays hold.
+// MQ: largest number of quadrature points the LDS arrays hold.  upper: only
+// the blocks i <= j (symmetric forms, TM_FLAG_UPPER); products are formed so
+// that block (j, i) would be the exact transpose of block (i, j).
 template <typename T, int D, int B, int H, int FORM, int MQ>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     wave_element_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                         const T* __restrict__ coeff, const double* __restrict__ tab, int Q,
-                        double lam, double mu, int64_t n_elem, T* __restrict__ slot) {
+                        double lam, double mu, int64_t n_elem, int upper, T* __restrict__ slot) {
   __shared__ double s_tab[MQ * (1 + B + D * B)];
+  __shared__ uint16_t s_pair[B * B];
   __shared__ double s_x[kWavesPerBlock][B * D];
   __shared__ double s_inv[kWavesPerBlock][MQ][D * D];
   __shared__ double s_jxw[kWavesPerBlock][MQ];
@@ -55,6 +59,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 
   const int tab_len = Q + Q * B + Q * D * B;
   for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
+  const int n_pairs = tmh::fill_pairs<B>(s_pair, upper != 0);
   const double* w = s_tab;
   const double* phi = s_tab + Q;
   const double* dphi = s_tab + Q + Q * B;
@@ -134,8 +139,8 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   T* out = slot + e * (int64_t)(B * B * HH);
   if constexpr (FORM == TM_FORM_LINEAR_ELASTICITY) {
     // one lane per node pair: the H x H block from one read of g_i, g_j
-    for (int ij = lane; ij < B * B; ij += kWave) {
-      const int i = ij / B, j = ij % B;
+    for (int p = lane; p < n_pairs; p += kWave) {
+      const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       double acc[HH];
 #pragma unroll
       for (int k = 0; k < HH; ++k) acc[k] = 0.0;
@@ -155,16 +160,16 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
         for (int a = 0; a < H; ++a)
 #pragma unroll
           for (int c = 0; c < H; ++c)
-            acc[a * H + c] += (lam * gi[a] * gj[c] + mu * gi[c] * gj[a] + (a == c ? mu * dot : 0.0)) * w;
+            acc[a * H + c] += (lam * (gi[a] * gj[c]) + mu * (gi[c] * gj[a]) + (a == c ? mu * dot : 0.0)) * w;
       }
 #pragma unroll
-      for (int k = 0; k < HH; ++k) out[ij * HH + k] = (T)acc[k];
+      for (int k = 0; k < HH; ++k) out[(i * B + j) * HH + k] = (T)acc[k];
     }
     return;
   }
-  for (int t = lane; t < B * B * HH; t += kWave) {
-    const int ij = t / HH;
-    const int i = ij / B, j = ij % B;
+  static_assert(FORM == TM_FORM_LINEAR_ELASTICITY || H == 1, "scalar forms have one value per pair");
+  for (int p = lane; p < n_pairs; p += kWave) {
+    const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
     double acc = 0.0;
     for (int q = 0; q < Q; ++q) {
       const double* g = &s_g[wv][q * B * D];
@@ -179,7 +184,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       }
       acc += val * s_jxw[wv][q];
     }
-    out[t] = (T)acc;
+    out[i * B + j] = (T)acc;
   }
 }
 
@@ -197,23 +202,28 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 // residual slots [e][i][a] summed into [N][D] by the node incidence reduction.
 typedef float tm_floatx16 __attribute__((ext_vector_type(16)));
 
-// MQ: largest number of quadrature points the LDS arrays hold.  The tangent
-// of each element is staged in LDS and written as whole slot rows (B blocks of
-// D x D values each) with vector stores; row (e, i) goes to e*B + i, or with
-// inc_inv to its position inc_inv[e*B + i] in the node-sorted incidence list.
+// MQ: largest number of quadrature points the LDS arrays hold.  Slot row
+// (e, i) (B blocks of D x D values) goes to e*B + i, or with inc_inv to its
+// position inc_inv[e*B + i] in the node-sorted incidence list.  The VALU
+// tangent leaves from registers: each lane stores its D x D block (72
+// contiguous bytes at f64, lanes j, j+1 of a row side by side), which needs no
+// LDS staging and measured faster than staged 16-byte row stores (82 instead
+// of 130 VGPRs).  The MFMA tile spreads a row over lanes, so that variant is
+// staged in LDS and written as whole slot rows with vector stores.
+// upper (TM_FLAG_UPPER): only the pairs i <= j.
 // BIG (tetra20, hexahedron27: element tangents of 60 x 60 and 81 x 81): one
-// wavefront per workgroup, the rule tables read through the caches instead of
-// LDS, and each lane's D x D blocks stored from registers, so that a wave
-// needs LDS only for its gradients (2 Q B D doubles).
+// wavefront per workgroup and the rule tables read through the caches instead
+// of LDS, so that a wave needs LDS only for its gradients (2 Q B D doubles).
 template <typename T, int D, int B, bool MFMA, int MQ, bool BIG>
 __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
-                      const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem,
+                      const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem, int upper,
                       T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv) {
   constexpr int kMaxQ = MQ;
   constexpr int kRowVals = B * D * D;  // values of one slot row
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
   static_assert(!(BIG && MFMA), "the MFMA tile holds at most 32 x 32");
+  constexpr bool kStage = MFMA;  // the MFMA tile spreads a row over lanes: staged row stores
   __shared__ double s_tab[BIG ? 1 : MQ * (1 + B + D * B)];
   __shared__ double s_x[kWpb][B * D];
   __shared__ double s_u[kWpb][B * D];
@@ -225,7 +235,8 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __shared__ double s_c2[kWpb][kMaxQ];
   __shared__ double s_Fi[kWpb][kMaxQ][D * D];
   __shared__ int s_v[kWpb][B];
-  __shared__ T s_out[kWpb][BIG ? 1 : B * kRowVals];
+  __shared__ T s_out[kWpb][kStage ? B * kRowVals : 1];
+  __shared__ uint16_t s_pair[B * B];
 
   static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
   const int lane = threadIdx.x % kWave;
@@ -235,6 +246,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   const int tab_len = Q + Q * B + Q * D * B;
   if constexpr (!BIG)
     for (int t = threadIdx.x; t < tab_len; t += blockDim.x) s_tab[t] = tab[t];
+  const int n_pairs = tmh::fill_pairs<B>(s_pair, upper != 0);
   const double* tb = BIG ? tab : s_tab;
   const double* w = tb;
   const double* dphi = tb + Q + Q * B;
@@ -423,10 +435,10 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     // one lane per node pair (i, j): the whole D x D block from one read of
     // g_i, g_j, h_i, h_j per quadrature point
     constexpr int DD = D * D;
-    for (int ij = lane; ij < B * B; ij += kWave) {
-      const int i = ij / B, j = ij % B;
+    for (int p = lane; p < n_pairs; p += kWave) {
+      const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       T* out;
-      if constexpr (BIG) {
+      if constexpr (!kStage) {
         const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
         out = tslot + srow * kRowVals + j * DD;
       } else {
@@ -458,13 +470,13 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
         for (int a = 0; a < D; ++a)
 #pragma unroll
           for (int b = 0; b < D; ++b)
-            acc[a * D + b] += lw * vhi[a] * vhj[b] - cw * vhi[b] * vhj[a] + (a == b ? mw : 0.0);
+            acc[a * D + b] += lw * (vhi[a] * vhj[b]) - cw * (vhi[b] * vhj[a]) + (a == b ? mw : 0.0);
       }
 #pragma unroll
       for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
     }
   }
-  if constexpr (!BIG) {
+  if constexpr (kStage) {
     __syncthreads();
     if (tslot && active) {
       // whole slot rows with vector stores: 16 bytes when a row's size allows,
@@ -476,6 +488,8 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
       typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
       for (int v = lane; v < B * kPerRow; v += kWave) {
         const int i = v / kPerRow, o = v % kPerRow;
+        // upper: row i from the vector holding block (i, i) on (earlier blocks unused)
+        if (upper && o < (i * D * D * (int)sizeof(T)) / kVec) continue;
         const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
         const vec_t x = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
         reinterpret_cast<vec_t*>(tslot + srow * kRowVals)[o] = x;
@@ -504,10 +518,11 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
     return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
   if (mfma && (Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
   const dim3 g((unsigned)grid), blk(kWave * kWpb);
+  const int upper = (f->flags & TM_FLAG_UPPER) != 0;
 #define TM_HYPER(MF, MQ)                                                                            \
   hyper_wave_kernel<T, D, B, MF, MQ, kBig><<<g, blk, 0, stream>>>(                                  \
-      m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, (T*)tslot, \
-      (T*)rslot, inc_inv)
+      m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \
+      (T*)tslot, (T*)rslot, inc_inv)
   if constexpr (kBig) {
     if (Q <= 16) TM_HYPER(false, 16); else TM_HYPER(false, 32);
   } else if constexpr (std::is_same<T, float>::value && B * D <= 32) {
@@ -558,9 +573,10 @@ int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hip
   const dim3 block(kWave * kWavesPerBlock);
   double lam = 0.0, mu = 0.0;
   *hh_out = 1;
+  const int upper = (f->flags & TM_FLAG_UPPER) != 0;
 #define TM_WAVE(H, FORM)                                                                                       \
   wave_element_kernel<T, D, B, H, FORM, MQ><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, \
-                                                                        mu, m->n_elem, slot)
+                                                                        mu, m->n_elem, upper, slot)
   switch (f->form) {
     case TM_FORM_LAPLACE:
       TM_WAVE(1, TM_FORM_LAPLACE);

@@ -18,6 +18,11 @@
 // Workgroups are remapped so that each XCD gets a contiguous range of runs:
 // neighbouring runs read the same element slot rows, which then stay in one
 // XCD's L2.
+//
+// Symmetric segment lists (tm_symmetric_segments): a slot id with bit 31 set
+// names the transposed block -- value h = a*D + b of the entry is read from
+// position b*D + a of the slot -- so element kernels of symmetric forms store
+// only the blocks i <= j.
 #include <hip/hip_runtime.h>
 
 #include "tm_internal.h"
@@ -26,6 +31,17 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kLdsBytes = 32768;
+constexpr int32_t kTransposed = INT32_MIN;  // bit 31 of a slot id
+
+// Value index inside a slot for value h of an entry: transposed for D x D
+// blocks (HH = 4, 9) when the slot id carries bit 31; other HH ignore it.
+template <int HH>
+__device__ __forceinline__ int64_t slot_value(int32_t raw, int h) {
+  constexpr int kD = HH == 9 ? 3 : HH == 4 ? 2 : 1;
+  const int64_t s = raw & INT32_MAX;
+  const int hv = (kD > 1 && raw < 0) ? (h % kD) * kD + h / kD : h;
+  return s * HH + hv;
+}
 
 // Bijection from the hardware dispatch order (block b runs on XCD b % 8) to a
 // logical order in which every XCD owns one contiguous range.
@@ -46,9 +62,9 @@ __global__ void __launch_bounds__(kBlock) reduce_walk_kernel(const int32_t* __re
 #pragma unroll
   for (int h = 0; h < HH; ++h) acc[h] = T(0);
   for (int32_t t = seg_ptr[k]; t < seg_ptr[k + 1]; ++t) {
-    const int64_t s = seg_slot[t];
+    const int32_t raw = seg_slot[t];
 #pragma unroll
-    for (int h = 0; h < HH; ++h) acc[h] += slot[s * HH + h];
+    for (int h = 0; h < HH; ++h) acc[h] += slot[slot_value<HH>(raw, h)];
   }
 #pragma unroll
   for (int h = 0; h < HH; ++h) out[k * HH + h] = acc[h];
@@ -72,9 +88,9 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
 #pragma unroll
       for (int h = 0; h < HH; ++h) acc[h] = T(0);
       for (int32_t t = seg_ptr[k]; t < seg_ptr[k + 1]; ++t) {
-        const int64_t s = seg_slot[t];
+        const int32_t raw = seg_slot[t];
 #pragma unroll
-        for (int h = 0; h < HH; ++h) acc[h] += slot[s * HH + h];
+        for (int h = 0; h < HH; ++h) acc[h] += slot[slot_value<HH>(raw, h)];
       }
 #pragma unroll
       for (int h = 0; h < HH; ++h) out[(int64_t)k * HH + h] = acc[h];
@@ -85,15 +101,15 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     int32_t t = threadIdx.x;
     // four independent gathers per iteration keep the memory pipeline full
     for (; t + 3 * kBlock < n; t += 4 * kBlock) {
-      const int32_t s0 = seg_slot[cb + t], s1 = seg_slot[cb + t + kBlock];
-      const int32_t s2 = seg_slot[cb + t + 2 * kBlock], s3 = seg_slot[cb + t + 3 * kBlock];
+      const int32_t s0 = seg_slot[cb + t] & INT32_MAX, s1 = seg_slot[cb + t + kBlock] & INT32_MAX;
+      const int32_t s2 = seg_slot[cb + t + 2 * kBlock] & INT32_MAX, s3 = seg_slot[cb + t + 3 * kBlock] & INT32_MAX;
       const T v0 = slot[s0], v1 = slot[s1], v2 = slot[s2], v3 = slot[s3];
       lds[t] = v0;
       lds[t + kBlock] = v1;
       lds[t + 2 * kBlock] = v2;
       lds[t + 3 * kBlock] = v3;
     }
-    for (; t < n; t += kBlock) lds[t] = slot[seg_slot[cb + t]];
+    for (; t < n; t += kBlock) lds[t] = slot[seg_slot[cb + t] & INT32_MAX];
   } else {
     // consecutive threads read consecutive values of a slot; kGather
     // independent (index, value) gathers per iteration keep the memory
@@ -112,13 +128,13 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
       for (int k = 0; k < kGather; ++k) sl[k] = seg_slot[cb + t[k]];
       T v[kGather];
 #pragma unroll
-      for (int k = 0; k < kGather; ++k) v[k] = slot[(int64_t)sl[k] * HH + h[k]];
+      for (int k = 0; k < kGather; ++k) v[k] = slot[slot_value<HH>(sl[k], h[k])];
 #pragma unroll
       for (int k = 0; k < kGather; ++k) lds[u + k * kBlock] = v[k];
     }
     for (; u < nv; u += kBlock) {
       const int32_t t = u / HH, h = u - t * HH;
-      lds[u] = slot[(int64_t)seg_slot[cb + t] * HH + h];
+      lds[u] = slot[slot_value<HH>(seg_slot[cb + t], h)];
     }
   }
   __syncthreads();
@@ -168,7 +184,38 @@ int launch(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot,
   }
 }
 
+// Slot s = e*B*B + i*B + j with i > j becomes (e*B*B + j*B + i) | bit 31.
+__global__ void __launch_bounds__(kBlock) symmetric_slots_kernel(const int32_t* __restrict__ seg_slot, int64_t n,
+                                                                 int32_t B, int32_t* __restrict__ sym) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int32_t s = seg_slot[t];
+  const int32_t BB = B * B;
+  const int32_t r = s % BB, i = r / B, j = r % B;
+  sym[t] = i <= j ? s : ((s - r + j * B + i) | kTransposed);
+}
+
 }  // namespace
+
+extern "C" int tm_symmetric_segments(int32_t n_basis, int64_t n_slots, const int32_t* seg_slot, int32_t* sym_slot,
+                                     tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (n_basis < 1 || n_basis > 255) return tmh::fail(TM_ERR_ARG, "n_basis out of range");
+  if (n_slots < 0 || n_slots >= tmh::kMaxIndex || n_slots % ((int64_t)n_basis * n_basis) != 0)
+    return tmh::fail(TM_ERR_SIZE, "n_slots must be n_elem * n_basis^2 < 2^31");
+  if (n_slots == 0) return TM_OK;
+  if (!seg_slot || !sym_slot) return tmh::fail(TM_ERR_ARG, "NULL pointer");
+  const int64_t grid = (n_slots + kBlock - 1) / kBlock;
+  symmetric_slots_kernel<<<grid, kBlock, 0, stream>>>(seg_slot, n_slots, n_basis, sym_slot);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("symmetric segments launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+  TM_TRY_END
+}
 
 extern "C" int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* seg_ptr,
                                    const int32_t* seg_slot, int64_t n_blocks, const int32_t* blk_ptr,

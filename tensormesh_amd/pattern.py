@@ -210,7 +210,23 @@ class Pattern:
                        mv(self.seg_ptr), mv(self.seg_slot), mv(self.slot2nnz), mv(self.row), mv(self.col),
                        self.hash, mv(self.inc_ptr), mv(self.inc))
 
-    def view(self, dtype: torch.dtype = torch.float64, hh: int = 1) -> nat.PatternView:
+    def sym_slot(self) -> torch.Tensor:
+        """Segment list for symmetric element blocks (TM_FLAG_UPPER): the slot of
+        every pair i > j replaced by its transposed upper block (bit 31 set),
+        built once on the pattern's device by tm_symmetric_segments."""
+        if getattr(self, "_sym_slot", None) is None:
+            nat.require_device(self.seg_slot, "pattern")
+            sym = torch.empty_like(self.seg_slot)
+            nat.check(nat.lib().tm_symmetric_segments(self.n_basis, self.seg_slot.numel(), self.seg_slot.data_ptr(),
+                                                      sym.data_ptr(), nat.stream_of(self.seg_slot.device)),
+                      "tm_symmetric_segments")
+            self._sym_slot = sym
+        return self._sym_slot
+
+    def view(self, dtype: torch.dtype = torch.float64, hh: int = 1, symmetric: bool = False) -> nat.PatternView:
+        """ctypes view for the reductions; ``symmetric`` selects the segment list
+        of ``sym_slot`` (element kernels launched with TM_FLAG_UPPER)."""
         bp = self.plan(nat.reduce_cap(dtype, hh))
+        seg = self.sym_slot() if symmetric else self.seg_slot
         return nat.PatternView(self.nnz, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
-                               self.seg_ptr.data_ptr(), self.seg_slot.data_ptr(), bp.numel() - 1, bp.data_ptr())
+                               self.seg_ptr.data_ptr(), seg.data_ptr(), bp.numel() - 1, bp.data_ptr())

@@ -1,0 +1,128 @@
+"""Symmetric element blocks (TM_FLAG_UPPER) against full blocks.
+
+With ``symmetric_slots`` (the default) the element-matrix, Neo-Hookean and J2
+kernels store only the blocks (i, j) with i <= j, and the segment reduction
+reads every other block transposed through the pattern's symmetric segment
+list (tm_symmetric_segments).  The kernels form their products so that block
+(j, i) is the exact transpose of block (i, j), and every entry still sums its
+contributions in ascending element order, so the assembled values must equal
+those of the full blocks (``symmetric_slots = False``, as the reference
+stores them) bit for bit.  Agreement of the full blocks with the oracle is
+covered by the parity tests of each form.
+"""
+import sys
+
+import pytest
+import torch
+
+import tensormesh_amd as tm
+from oracle import fem
+from tensormesh_amd.mesh import unit_cube_hex, unit_cube_tet, unit_square_quad, unit_square_tri
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _mesh(etype, seed=1):
+    if etype in ("triangle", "triangle6", "triangle10"):
+        return unit_square_tri(4, 0.15, seed, order={"triangle": 1, "triangle6": 2, "triangle10": 3}[etype])
+    if etype in ("tetra", "tetra10", "tetra20"):
+        return unit_cube_tet(2, 0.1, seed, order={"tetra": 1, "tetra10": 2, "tetra20": 3}[etype])
+    if etype in ("quad", "quad9"):
+        return unit_square_quad(4, 0.2, seed, order=2 if etype == "quad9" else 1)
+    return unit_cube_hex(2, 0.2, seed, order=2 if etype == "hexahedron27" else 1)
+
+
+def _both(make, run):
+    """run(make()) with symmetric and with full element blocks."""
+    out = []
+    for sym in (True, False):
+        A = make()
+        A.symmetric_slots = sym
+        out.append(run(A))
+    return out
+
+
+def _same(a, b, what):
+    a, b = a.detach(), b.detach()
+    assert a.shape == b.shape, what
+    assert torch.equal(a, b), f"{what}: max |diff| {(a - b).abs().max().item():.3e}"
+
+
+@pytest.mark.parametrize("etype", ["triangle6", "tetra10", "triangle10", "tetra20", "quad", "quad9", "hexahedron",
+                                   "hexahedron27", "tetra"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_symmetric_blocks_bilinear_forms(etype, dtype):
+    say(f"case symmetric bilinear {etype} {dtype}")
+    m = _mesh(etype)
+    md = tm.Mesh(m.points.to(dtype), {etype: m.elements()}, m.boundary_mask).to(DEV)
+    E = m.elements().shape[0]
+    kappa = torch.linspace(0.5, 2.0, E, dtype=dtype, device=DEV)
+    qo = 3 if etype in ("triangle10", "tetra20") else 2
+    cases = [("linear_elasticity", lambda: tm.LinearElasticityElementAssembler.from_mesh(md, quadrature_order=qo,
+                                                                                         E=2.0, nu=0.3), {})]
+    if etype != "tetra":   # P1 scalar forms run the P1 kernels, which store every block
+        cases += [("laplace", lambda: tm.LaplaceElementAssembler.from_mesh(md, quadrature_order=qo), {}),
+                  ("diffusion", lambda: tm.DiffusionElementAssembler.from_mesh(md, quadrature_order=qo),
+                   {"element_data": {"kappa": kappa}}),
+                  ("mass", lambda: tm.MassElementAssembler.from_mesh(md, quadrature_order=qo), {})]
+    for name, make, kw in cases:
+        Ks, Kf = _both(make, lambda A: A(**kw))
+        assert torch.equal(Ks.row, Kf.row) and torch.equal(Ks.col, Kf.col)
+        _same(Ks.values, Kf.values, f"{etype} {name}")
+
+
+@pytest.mark.parametrize("etype,qo", [("tetra10", 2), ("triangle6", 2), ("hexahedron", 2), ("tetra20", 2),
+                                      ("hexahedron27", 3)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_symmetric_blocks_neohookean(etype, qo, dtype):
+    say(f"case symmetric neo-hookean {etype} qorder {qo} {dtype}")
+    m = _mesh(etype)
+    P = m.points
+    u = 0.05 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, -1], 0.5 * P[:, 0] ** 2][:P.shape[1]], 1)
+    md = tm.Mesh(P.to(dtype), {etype: m.elements()}, m.boundary_mask).to(DEV)
+    ud = u.to(dtype).to(DEV)
+    (Rs, Ks), (Rf, Kf) = _both(lambda: tm.NeoHookeanModel.from_mesh(md, quadrature_order=qo, E=1.0, nu=0.3),
+                               lambda A: A.residual_and_tangent(ud))
+    _same(Rs, Rf, "residual")
+    _same(Ks.values, Kf.values, "tangent")
+
+
+@pytest.mark.parametrize("etype,qo", [("tetra", 2), ("triangle", 2), ("tetra10", 2), ("hexahedron27", 3)])
+def test_symmetric_blocks_j2(etype, qo):
+    say(f"case symmetric J2 {etype} qorder {qo}")
+    m = _mesh(etype, seed=3)
+    E = m.elements().shape[0]
+    Q = fem.quadrature(etype, qo)[0].shape[0]
+    g = torch.Generator().manual_seed(7)
+    ep = 1e-4 * torch.randn(E, Q, 3, 3, generator=g, dtype=torch.float64)
+    ep = (0.5 * (ep + ep.transpose(-1, -2))).to(DEV)
+    al = (1e-4 * torch.rand(E, Q, generator=g, dtype=torch.float64)).to(DEV)
+    u = (2e-4 * torch.randn(m.points.shape, generator=g, dtype=torch.float64)).to(DEV)
+    (Rs, Ks), (Rf, Kf) = _both(lambda: tm.J2Plasticity.from_mesh(m.to(DEV), quadrature_order=qo, E=200.0, nu=0.3,
+                                                                  sig0=0.25, H=1.0),
+                               lambda A: A.residual_and_tangent(u, ep, al))
+    _same(Rs, Rf, "residual")
+    _same(Ks.values, Kf.values, "tangent")
+
+
+def test_symmetric_segment_list():
+    """tm_symmetric_segments on a tetra10 pattern: pairs i <= j keep their slot,
+    the others name the transposed block with bit 31 set."""
+    m = _mesh("tetra10").to(DEV)
+    A = tm.LaplaceElementAssembler.from_mesh(m, quadrature_order=2)
+    A()
+    pat = A._pattern_on(torch.device(DEV))
+    seg = pat.seg_slot.long().cpu()
+    sym = pat.sym_slot().long().cpu()
+    B = pat.n_basis
+    e, r = seg // (B * B), seg % (B * B)
+    i, j = r // B, r % B
+    want = torch.where(i <= j, seg, (e * B * B + j * B + i) - 2 ** 31)
+    assert torch.equal(sym, want)
+    assert int((i > j).sum()) > 0 and int((i <= j).sum()) > 0

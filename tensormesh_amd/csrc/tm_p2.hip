@@ -211,10 +211,13 @@ typedef float tm_floatx16 __attribute__((ext_vector_type(16)));
 // of 130 VGPRs).  The MFMA tile spreads a row over lanes, so that variant is
 // staged in LDS and written as whole slot rows with vector stores.
 // upper (TM_FLAG_UPPER): only the pairs i <= j.
+// EPW elements per wavefront, each on L = 64 / EPW lanes: the VALU path packs
+// two, so that the per-quadrature-point stages (Q active lanes) serve two
+// elements at once (tetra10 f64 1.15 -> 1.07 ms per Newton assembly).
 // BIG (tetra20, hexahedron27: element tangents of 60 x 60 and 81 x 81): one
 // wavefront per workgroup and the rule tables read through the caches instead
 // of LDS, so that a wave needs LDS only for its gradients (2 Q B D doubles).
-template <typename T, int D, int B, bool MFMA, int MQ, bool BIG>
+template <typename T, int D, int B, bool MFMA, int MQ, bool BIG, int EPW>
 __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem, int upper,
@@ -223,25 +226,28 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   constexpr int kRowVals = B * D * D;  // values of one slot row
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
   static_assert(!(BIG && MFMA), "the MFMA tile holds at most 32 x 32");
+  static_assert(EPW == 1 || !(MFMA || BIG), "the MFMA tile and BIG cells take a whole wavefront");
+  constexpr int L = kWave / EPW;  // lanes per element
+  constexpr int kEpb = kWpb * EPW;  // elements per workgroup
   constexpr bool kStage = MFMA;  // the MFMA tile spreads a row over lanes: staged row stores
   __shared__ double s_tab[BIG ? 1 : MQ * (1 + B + D * B)];
-  __shared__ double s_x[kWpb][B * D];
-  __shared__ double s_u[kWpb][B * D];
-  __shared__ double s_inv[kWpb][kMaxQ][D * D];
-  __shared__ double s_jxw[kWpb][kMaxQ];
-  __shared__ double s_g[kWpb][kMaxQ * B * D];
-  __shared__ double s_h[kWpb][kMaxQ * B * D];
-  __shared__ double s_P[kWpb][kMaxQ][D * D];
-  __shared__ double s_c2[kWpb][kMaxQ];
-  __shared__ double s_Fi[kWpb][kMaxQ][D * D];
-  __shared__ int s_v[kWpb][B];
-  __shared__ T s_out[kWpb][kStage ? B * kRowVals : 1];
+  __shared__ double s_x[kEpb][B * D];
+  __shared__ double s_u[kEpb][B * D];
+  __shared__ double s_inv[kEpb][kMaxQ][D * D];
+  __shared__ double s_jxw[kEpb][kMaxQ];
+  __shared__ double s_g[kEpb][kMaxQ * B * D];
+  __shared__ double s_h[kEpb][kMaxQ * B * D];
+  __shared__ double s_P[kEpb][kMaxQ][D * D];
+  __shared__ double s_c2[kEpb][kMaxQ];
+  __shared__ double s_Fi[kEpb][kMaxQ][D * D];
+  __shared__ int s_v[kEpb][B];
+  __shared__ T s_out[kEpb][kStage ? B * kRowVals : 1];
   __shared__ uint16_t s_pair[B * B];
 
-  static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
-  const int lane = threadIdx.x % kWave;
-  const int wv = threadIdx.x / kWave;
-  const int64_t e = (int64_t)blockIdx.x * kWpb + wv;
+  static_assert(B <= L && MQ <= L, "an element's lanes index its nodes and its quadrature points");
+  const int lane = threadIdx.x % L;   // lane within the element's group
+  const int wv = threadIdx.x / L;     // element slot within the workgroup
+  const int64_t e = (int64_t)blockIdx.x * kEpb + wv;
   const bool active = e < n_elem;
   const int tab_len = Q + Q * B + Q * D * B;
   if constexpr (!BIG)
@@ -253,7 +259,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
   __syncthreads();
   if (active) {
-    for (int t = lane; t < B * D; t += kWave) {
+    for (int t = lane; t < B * D; t += L) {
       const int b = t / D, d = t % D;
       s_x[wv][t] = (double)pts[(int64_t)s_v[wv][b] * D + d];
       s_u[wv][t] = (double)disp[(int64_t)s_v[wv][b] * D + d];
@@ -303,7 +309,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   }
   __syncthreads();
   if (active) {
-    for (int t = lane; t < Q * B * D; t += kWave) {
+    for (int t = lane; t < Q * B * D; t += L) {
       const int q = t / (B * D), b = (t / D) % B, d = t % D;
       double s = 0.0;
 #pragma unroll
@@ -360,7 +366,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   }
   __syncthreads();
   if (active) {
-    for (int t = lane; t < Q * B * D; t += kWave) {
+    for (int t = lane; t < Q * B * D; t += L) {
       const int q = t / (B * D), b = (t / D) % B, a = t % D;
       const double* g = &s_g[wv][(q * B + b) * D];
       double s = 0.0;
@@ -372,7 +378,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __syncthreads();
   // (no early return for inactive waves: the block meets once more below)
   if (rslot && active) {
-    for (int t = lane; t < B * D; t += kWave) {
+    for (int t = lane; t < B * D; t += L) {
       const int i = t / D, a = t % D;
       double acc = 0.0;
       for (int q = 0; q < Q; ++q) {
@@ -435,7 +441,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     // one lane per node pair (i, j): the whole D x D block from one read of
     // g_i, g_j, h_i, h_j per quadrature point
     constexpr int DD = D * D;
-    for (int p = lane; p < n_pairs; p += kWave) {
+    for (int p = lane; p < n_pairs; p += L) {
       const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       T* out;
       if constexpr (!kStage) {
@@ -486,7 +492,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
       static_assert(kRowBytes % kVec == 0, "slot rows are whole vectors");
       constexpr int kPerRow = kRowBytes / kVec;
       typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
-      for (int v = lane; v < B * kPerRow; v += kWave) {
+      for (int v = lane; v < B * kPerRow; v += L) {
         const int i = v / kPerRow, o = v % kPerRow;
         // upper: row i from the vector holding block (i, i) on (earlier blocks unused)
         if (upper && o < (i * D * D * (int)sizeof(T)) / kVec) continue;
@@ -509,30 +515,32 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   if (Q <= 0 || Q > (kBig ? kMaxQWave : kMaxQ))
     return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
-  const int64_t grid = (m->n_elem + kWpb - 1) / kWpb;
-  if (grid == 0) return TM_OK;
   const double mu = f->young / (2.0 * (1.0 + f->poisson));
   const double lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
   const bool mfma = (f->flags & TM_FLAG_MFMA) != 0;
   if (mfma && !(std::is_same<T, float>::value && B * D <= 32))
     return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
   if (mfma && (Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
+  // two elements per wavefront (32 lanes each) on the VALU path: the
+  // per-quadrature-point stages, which use Q lanes, then serve two elements
+  const int epw = (mfma || kBig) ? 1 : 2;
+  const int64_t per_block = (int64_t)kWpb * epw;
+  const int64_t grid = (m->n_elem + per_block - 1) / per_block;
+  if (grid == 0) return TM_OK;
   const dim3 g((unsigned)grid), blk(kWave * kWpb);
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
-#define TM_HYPER(MF, MQ)                                                                            \
-  hyper_wave_kernel<T, D, B, MF, MQ, kBig><<<g, blk, 0, stream>>>(                                  \
+#define TM_HYPER(MF, MQ, EPW)                                                                       \
+  hyper_wave_kernel<T, D, B, MF, MQ, kBig, EPW><<<g, blk, 0, stream>>>(                             \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \
       (T*)tslot, (T*)rslot, inc_inv)
   if constexpr (kBig) {
-    if (Q <= 16) TM_HYPER(false, 16); else TM_HYPER(false, 32);
-  } else if constexpr (std::is_same<T, float>::value && B * D <= 32) {
-    if (mfma) {
-      if (Q <= 4) TM_HYPER(true, 4); else TM_HYPER(true, 16);
-    } else {
-      if (Q <= 4) TM_HYPER(false, 4); else TM_HYPER(false, 16);
+    if (Q <= 16) TM_HYPER(false, 16, 1); else TM_HYPER(false, 32, 1);
+  } else if (mfma) {
+    if constexpr (std::is_same<T, float>::value) {  // float64 was rejected above
+      if (Q <= 4) TM_HYPER(true, 4, 1); else TM_HYPER(true, 16, 1);
     }
   } else {
-    if (Q <= 4) TM_HYPER(false, 4); else TM_HYPER(false, 16);
+    if (Q <= 4) TM_HYPER(false, 4, 2); else TM_HYPER(false, 16, 2);
   }
 #undef TM_HYPER
   hipError_t err = hipGetLastError();

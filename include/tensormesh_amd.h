@@ -118,6 +118,13 @@ typedef struct tm_pattern_view {
   const int32_t* seg_slot;   /* [n_elem*B*B] slot ids s = e*B*B + i*B + j, ascending per entry */
   int64_t n_blocks;          /* reduction plan (tm_reduce_plan), or 0 for one thread per entry */
   const int32_t* blk_ptr;    /* [n_blocks + 1] first entry of each reduction block */
+  /* Optional paired form (both NULL: segment k is entry k).  With them, nnz
+   * counts the listed segments, segment k is written to entry[k] and,
+   * transposed, to entry_t[k] (tm_segmented_reduce_pairs); outputs keep the
+   * pattern's full entry count.  For symmetric forms: the upper entries of a
+   * TM_FLAG_UPPER segment list fill the lower ones. */
+  const int32_t* entry;      /* [nnz] */
+  const int32_t* entry_t;    /* [nnz]; == entry[k] on the diagonal */
 } tm_pattern_view;
 
 /* Node -> (element, local index) incidence, ascending element id per node. */
@@ -454,6 +461,18 @@ int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* s
  * (projector.py:95-123 sums them).  n_slots = n_elem * n_basis^2. */
 int tm_symmetric_segments(int32_t n_basis, int64_t n_slots, const int32_t* seg_slot, int32_t* sym_slot,
                           tm_stream_t stream);
+
+/* Stage S with paired outputs: segment k (contributions seg_slot[seg_ptr[k]
+ * .. seg_ptr[k+1]), summed left to right as tm_segmented_reduce does) is
+ * written to out[entry[k]*hh + t] and, when entry_t[k] != entry[k], to
+ * out[entry_t[k]*hh + t'] with t' the transposed position of a D x D block
+ * (t for hh other than 4 and 9).  With the upper entries (row <= column) of
+ * a symmetric matrix and their mirrors, every slot block is read once.  No
+ * counterpart in the reference (projector.py:95-123 sums every entry). */
+int tm_segmented_reduce_pairs(int32_t dtype, int64_t n_seg, int32_t hh, const int32_t* seg_ptr,
+                              const int32_t* seg_slot, const int32_t* entry, const int32_t* entry_t,
+                              int64_t n_blocks, const int32_t* blk_ptr, const void* slot, void* out,
+                              tm_stream_t stream);
 
 /* ---------------------------------------------------------------------
  * Linear-form assembly (load vector), two stages on one stream.

@@ -48,7 +48,8 @@ TM_FLAG_UPPER = 2   # symmetric forms: element blocks i <= j only, symmetric seg
 class PatternView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("col_idx", ctypes.c_void_p),
                 ("seg_ptr", ctypes.c_void_p), ("seg_slot", ctypes.c_void_p),
-                ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p)]
+                ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p),
+                ("entry", ctypes.c_void_p), ("entry_t", ctypes.c_void_p)]
 
 
 class BlockRowsView(ctypes.Structure):
@@ -120,6 +121,10 @@ SIGNATURES = {
     "tm_reduce_plan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, c_i64p]),
     "tm_symmetric_segments": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p]),
+    "tm_segmented_reduce_pairs": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
     "tm_assemble_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                           ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),

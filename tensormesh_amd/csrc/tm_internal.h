@@ -60,6 +60,9 @@ constexpr int kSumParts = 1024;
 int sum_fixed(const double* v, int64_t n, double* part, int32_t dtype, void* out, tm_stream_t stream);
 // Zeroes `bytes` (a multiple of 4) of device memory with a kernel.
 int zero_fill(void* p, int64_t bytes, tm_stream_t stream);
+// Segmented reduction of a pattern view (paired outputs when pat->entry is set).
+int reduce_view(int32_t dtype, const tm_pattern_view* pat, int32_t hh, const void* slot, void* out,
+                tm_stream_t stream);
 
 }  // namespace tmh
 

@@ -395,8 +395,7 @@ extern "C" int tm_assemble_j2(const tm_mesh_view* mesh, const tm_form_desc* form
     if (st) return st;
   }
   if (tangent) {
-    st = tm_segmented_reduce(mesh->dtype, pat->nnz, (int32_t)(D * D), pat->seg_ptr, pat->seg_slot, pat->n_blocks,
-                             pat->blk_ptr, tslot_ws, tangent, stream_);
+    st = tmh::reduce_view(mesh->dtype, pat, (int32_t)(D * D), tslot_ws, tangent, stream_);
     if (st) return st;
   }
   if (residual) {

@@ -316,8 +316,7 @@ extern "C" int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc
   int st = tm_element_matrices(mesh, form, slot_ws, stream);
   if (st) return st;
   const int hh = form->form == TM_FORM_LINEAR_ELASTICITY ? tmh::cell_dim(mesh->cell) * tmh::cell_dim(mesh->cell) : 1;
-  return tm_segmented_reduce(mesh->dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, pat->n_blocks, pat->blk_ptr,
-                             slot_ws, values, stream);
+  return tmh::reduce_view(mesh->dtype, pat, hh, slot_ws, values, stream);
   TM_TRY_END
 }
 

@@ -23,6 +23,11 @@
 // names the transposed block -- value h = a*D + b of the entry is read from
 // position b*D + a of the slot -- so element kernels of symmetric forms store
 // only the blocks i <= j.
+//
+// Paired outputs (tm_segmented_reduce_pairs): segment k is summed once and
+// written to entry[k] and, transposed, to entry_t[k] -- the upper entries
+// (r <= c) of a symmetric matrix fill the lower ones, so every slot block is
+// read once instead of twice.
 #include <hip/hip_runtime.h>
 
 #include "tm_internal.h"
@@ -43,6 +48,26 @@ __device__ __forceinline__ int64_t slot_value(int32_t raw, int h) {
   return s * HH + hv;
 }
 
+// Writes the sum of segment k: out[k] (entry == NULL), or out[entry[k]] and,
+// transposed for D x D blocks, out[entry_t[k]] when that is another entry.
+template <typename T, int HH>
+__device__ __forceinline__ void write_entry(T* __restrict__ out, int64_t k, const T (&acc)[HH],
+                                            const int32_t* __restrict__ entry, const int32_t* __restrict__ entry_t) {
+  if (!entry) {
+#pragma unroll
+    for (int h = 0; h < HH; ++h) out[k * HH + h] = acc[h];
+    return;
+  }
+  constexpr int kD = HH == 9 ? 3 : HH == 4 ? 2 : 1;
+  const int64_t k0 = entry[k], k1 = entry_t[k];
+#pragma unroll
+  for (int h = 0; h < HH; ++h) out[k0 * HH + h] = acc[h];
+  if (k1 != k0) {
+#pragma unroll
+    for (int h = 0; h < HH; ++h) out[k1 * HH + (h % kD) * kD + h / kD] = acc[h];
+  }
+}
+
 // Bijection from the hardware dispatch order (block b runs on XCD b % 8) to a
 // logical order in which every XCD owns one contiguous range.
 __device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
@@ -55,6 +80,8 @@ template <typename T, int HH>
 __global__ void __launch_bounds__(kBlock) reduce_walk_kernel(const int32_t* __restrict__ seg_ptr,
                                                              const int32_t* __restrict__ seg_slot,
                                                              const T* __restrict__ slot, int64_t nnz,
+                                                             const int32_t* __restrict__ entry,
+                                                             const int32_t* __restrict__ entry_t,
                                                              T* __restrict__ out) {
   const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (k >= nnz) return;
@@ -66,15 +93,17 @@ __global__ void __launch_bounds__(kBlock) reduce_walk_kernel(const int32_t* __re
 #pragma unroll
     for (int h = 0; h < HH; ++h) acc[h] += slot[slot_value<HH>(raw, h)];
   }
-#pragma unroll
-  for (int h = 0; h < HH; ++h) out[k * HH + h] = acc[h];
+  write_entry<T, HH>(out, k, acc, entry, entry_t);
 }
 
 template <typename T, int HH>
 __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __restrict__ blk_ptr, int64_t n_blocks,
                                                                const int32_t* __restrict__ seg_ptr,
                                                                const int32_t* __restrict__ seg_slot,
-                                                               const T* __restrict__ slot, T* __restrict__ out) {
+                                                               const T* __restrict__ slot,
+                                                               const int32_t* __restrict__ entry,
+                                                               const int32_t* __restrict__ entry_t,
+                                                               T* __restrict__ out) {
   constexpr int kCap = kLdsBytes / (int)(sizeof(T) * HH);
   __shared__ T lds[kCap * HH];
   const int64_t b = xcd_logical(blockIdx.x, n_blocks);
@@ -92,8 +121,7 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
 #pragma unroll
         for (int h = 0; h < HH; ++h) acc[h] += slot[slot_value<HH>(raw, h)];
       }
-#pragma unroll
-      for (int h = 0; h < HH; ++h) out[(int64_t)k * HH + h] = acc[h];
+      write_entry<T, HH>(out, k, acc, entry, entry_t);
     }
     return;
   }
@@ -147,21 +175,22 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
 #pragma unroll
       for (int h = 0; h < HH; ++h) acc[h] += lds[u * HH + h];
     }
-#pragma unroll
-    for (int h = 0; h < HH; ++h) out[(int64_t)k * HH + h] = acc[h];
+    write_entry<T, HH>(out, k, acc, entry, entry_t);
   }
 }
 
 template <typename T, int HH>
-int launch_hh(int64_t nnz, const int32_t* seg_ptr, const int32_t* seg_slot, int64_t n_blocks,
-              const int32_t* blk_ptr, const void* slot, void* out, hipStream_t stream) {
+int launch_hh(int64_t nnz, const int32_t* seg_ptr, const int32_t* seg_slot, const int32_t* entry,
+              const int32_t* entry_t, int64_t n_blocks, const int32_t* blk_ptr, const void* slot, void* out,
+              hipStream_t stream) {
   const T* s = static_cast<const T*>(slot);
   T* o = static_cast<T*>(out);
   if (n_blocks > 0 && blk_ptr) {
-    reduce_stream_kernel<T, HH><<<n_blocks, kBlock, 0, stream>>>(blk_ptr, n_blocks, seg_ptr, seg_slot, s, o);
+    reduce_stream_kernel<T, HH><<<n_blocks, kBlock, 0, stream>>>(blk_ptr, n_blocks, seg_ptr, seg_slot, s, entry,
+                                                                  entry_t, o);
   } else {
     const int64_t grid = (nnz + kBlock - 1) / kBlock;
-    reduce_walk_kernel<T, HH><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, o);
+    reduce_walk_kernel<T, HH><<<grid, kBlock, 0, stream>>>(seg_ptr, seg_slot, s, nnz, entry, entry_t, o);
   }
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
@@ -172,16 +201,34 @@ int launch_hh(int64_t nnz, const int32_t* seg_ptr, const int32_t* seg_slot, int6
 }
 
 template <typename T>
-int launch(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot, int64_t n_blocks,
-           const int32_t* blk_ptr, const void* slot, void* out, hipStream_t stream) {
+int launch(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot, const int32_t* entry,
+           const int32_t* entry_t, int64_t n_blocks, const int32_t* blk_ptr, const void* slot, void* out,
+           hipStream_t stream) {
+#define TM_HH(H) launch_hh<T, H>(nnz, seg_ptr, seg_slot, entry, entry_t, n_blocks, blk_ptr, slot, out, stream)
   switch (hh) {
-    case 1: return launch_hh<T, 1>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-    case 2: return launch_hh<T, 2>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-    case 3: return launch_hh<T, 3>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-    case 4: return launch_hh<T, 4>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-    case 9: return launch_hh<T, 9>(nnz, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
+    case 1: return TM_HH(1);
+    case 2: return TM_HH(2);
+    case 3: return TM_HH(3);
+    case 4: return TM_HH(4);
+    case 9: return TM_HH(9);
     default: return tmh::fail(TM_ERR_UNSUPPORTED, "values per slot must be 1, 2, 3, 4 or 9");
   }
+#undef TM_HH
+}
+
+int reduce_checked(int32_t dtype, int64_t n_seg, int32_t hh, const int32_t* seg_ptr, const int32_t* seg_slot,
+                   const int32_t* entry, const int32_t* entry_t, int64_t n_blocks, const int32_t* blk_ptr,
+                   const void* slot, void* out, hipStream_t stream) {
+  if (n_seg < 0 || n_seg >= tmh::kMaxIndex || n_blocks < 0 || n_blocks > n_seg)
+    return tmh::fail(TM_ERR_SIZE, "nnz / plan size out of range");
+  if (n_seg == 0) return TM_OK;
+  if (!seg_ptr || !seg_slot || !slot || !out) return tmh::fail(TM_ERR_ARG, "NULL pointer");
+  if ((entry == nullptr) != (entry_t == nullptr)) return tmh::fail(TM_ERR_ARG, "entry and entry_t go together");
+  if (dtype == TM_F64)
+    return launch<double>(n_seg, hh, seg_ptr, seg_slot, entry, entry_t, n_blocks, blk_ptr, slot, out, stream);
+  if (dtype == TM_F32)
+    return launch<float>(n_seg, hh, seg_ptr, seg_slot, entry, entry_t, n_blocks, blk_ptr, slot, out, stream);
+  return tmh::fail(TM_ERR_ARG, "unknown dtype");
 }
 
 // Slot s = e*B*B + i*B + j with i > j becomes (e*B*B + j*B + i) | bit 31.
@@ -221,13 +268,24 @@ extern "C" int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const
                                    const int32_t* seg_slot, int64_t n_blocks, const int32_t* blk_ptr,
                                    const void* slot, void* out, tm_stream_t stream_) {
   TM_TRY_BEGIN
-  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
-  if (nnz < 0 || nnz >= tmh::kMaxIndex || n_blocks < 0 || n_blocks > nnz)
-    return tmh::fail(TM_ERR_SIZE, "nnz / plan size out of range");
-  if (nnz == 0) return TM_OK;
-  if (!seg_ptr || !seg_slot || !slot || !out) return tmh::fail(TM_ERR_ARG, "NULL pointer");
-  if (dtype == TM_F64) return launch<double>(nnz, hh, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-  if (dtype == TM_F32) return launch<float>(nnz, hh, seg_ptr, seg_slot, n_blocks, blk_ptr, slot, out, stream);
-  return tmh::fail(TM_ERR_ARG, "unknown dtype");
+  return reduce_checked(dtype, nnz, hh, seg_ptr, seg_slot, nullptr, nullptr, n_blocks, blk_ptr, slot, out,
+                        reinterpret_cast<hipStream_t>(stream_));
   TM_TRY_END
+}
+
+extern "C" int tm_segmented_reduce_pairs(int32_t dtype, int64_t n_seg, int32_t hh, const int32_t* seg_ptr,
+                                         const int32_t* seg_slot, const int32_t* entry, const int32_t* entry_t,
+                                         int64_t n_blocks, const int32_t* blk_ptr, const void* slot, void* out,
+                                         tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  if (n_seg > 0 && (!entry || !entry_t)) return tmh::fail(TM_ERR_ARG, "entry / entry_t is NULL");
+  return reduce_checked(dtype, n_seg, hh, seg_ptr, seg_slot, entry, entry_t, n_blocks, blk_ptr, slot, out,
+                        reinterpret_cast<hipStream_t>(stream_));
+  TM_TRY_END
+}
+
+int tmh::reduce_view(int32_t dtype, const tm_pattern_view* pat, int32_t hh, const void* slot, void* out,
+                     tm_stream_t stream) {
+  return reduce_checked(dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, pat->entry, pat->entry_t, pat->n_blocks,
+                        pat->blk_ptr, slot, out, reinterpret_cast<hipStream_t>(stream));
 }

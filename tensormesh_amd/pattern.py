@@ -223,10 +223,49 @@ class Pattern:
             self._sym_slot = sym
         return self._sym_slot
 
+    def sym_pairs(self):
+        """Paired segment list for symmetric element blocks: the upper entries
+        (row <= column) in CSR order, each with its contributions from
+        ``sym_slot`` (ascending element order, as in ``seg_slot``) and the
+        position of its mirror entry (column, row).  Returns device int32
+        tensors (entry, entry_t, seg_ptr, seg_slot); built once."""
+        if getattr(self, "_sym_pairs", None) is None:
+            dev = self.seg_slot.device
+            rp = self.row_ptr.cpu().numpy().astype(np.int64)
+            ci = self.col_idx.cpu().numpy().astype(np.int64)
+            sp = self.seg_ptr.cpu().numpy().astype(np.int64)
+            sym = self.sym_slot().cpu().numpy()
+            N = self.n_points
+            rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(rp))
+            up = np.flatnonzero(rows <= ci)
+            keys = rows * N + ci                      # ascending: CSR order
+            mirror = ci[up] * N + rows[up]
+            ent_t = np.searchsorted(keys, mirror)
+            if not np.array_equal(keys[np.minimum(ent_t, keys.size - 1)], mirror):
+                raise ValueError("element pattern is not structurally symmetric")
+            lens = sp[up + 1] - sp[up]
+            seg_ptr = np.zeros(up.size + 1, np.int64)
+            np.cumsum(lens, out=seg_ptr[1:])
+            idx = np.repeat(sp[up] - seg_ptr[:-1], lens) + np.arange(seg_ptr[-1], dtype=np.int64)
+            t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
+            self._sym_pairs = (t(up), t(ent_t), t(seg_ptr), t(sym[idx]))
+            self._sym_plans = {}
+            self._sym_slot = None   # only the paired list is read from here on
+        return self._sym_pairs
+
     def view(self, dtype: torch.dtype = torch.float64, hh: int = 1, symmetric: bool = False) -> nat.PatternView:
-        """ctypes view for the reductions; ``symmetric`` selects the segment list
-        of ``sym_slot`` (element kernels launched with TM_FLAG_UPPER)."""
-        bp = self.plan(nat.reduce_cap(dtype, hh))
-        seg = self.sym_slot() if symmetric else self.seg_slot
-        return nat.PatternView(self.nnz, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
-                               self.seg_ptr.data_ptr(), seg.data_ptr(), bp.numel() - 1, bp.data_ptr())
+        """ctypes view for the reductions; ``symmetric`` selects the paired
+        upper-entry list of ``sym_pairs`` (element kernels launched with
+        TM_FLAG_UPPER): each upper entry is summed once and mirrored."""
+        if not symmetric:
+            bp = self.plan(nat.reduce_cap(dtype, hh))
+            return nat.PatternView(self.nnz, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
+                                   self.seg_ptr.data_ptr(), self.seg_slot.data_ptr(), bp.numel() - 1, bp.data_ptr())
+        entry, entry_t, seg_ptr, seg_slot = self.sym_pairs()
+        cap = nat.reduce_cap(dtype, hh)
+        if cap not in self._sym_plans:
+            self._sym_plans[cap] = torch.from_numpy(nat.reduce_plan(seg_ptr.cpu().numpy(), cap)).to(seg_ptr.device)
+        bp = self._sym_plans[cap]
+        return nat.PatternView(entry.numel(), self.row_ptr.data_ptr(), self.col_idx.data_ptr(), seg_ptr.data_ptr(),
+                               seg_slot.data_ptr(), bp.numel() - 1, bp.data_ptr(), entry.data_ptr(),
+                               entry_t.data_ptr())

@@ -485,3 +485,50 @@ def test_cg_warns_when_the_budget_runs_out():
     assert rel <= 1e-10
     with pytest.warns(UserWarning, match="did not converge"):
         cg(Ki, fi, rtol=1e-12, max_iter=2, check_every=1)
+
+
+@pytest.mark.parametrize("etype,H", [("tetra10", 3), ("triangle6", 2), ("tetra10", 1)])
+def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
+    """Pattern.sym_pairs on the CPU: summing each upper entry once from the
+    symmetric slot list (bit 31 = read the block transposed) and mirroring it
+    gives, bit for bit, the plain reduction of full symmetric element blocks
+    (the paired form of tm_segmented_reduce_pairs, emulated in numpy)."""
+    m = unit_cube_tet(2, 0.1, 1, order=2) if etype == "tetra10" else unit_square_tri(3, 0.1, 1, order=2)
+    conn = m.elements()
+    E, B = conn.shape
+    pat = Pattern.build(conn, m.n_points)
+    seg = pat.seg_slot.long()
+    r = seg % (B * B)
+    i, j = r // B, r % B
+    sym = torch.where(i <= j, seg, (seg - r + j * B + i) - 2 ** 31).to(torch.int32)
+    pat._sym_slot = sym                      # tm_symmetric_segments, restated
+    entry, entry_t, sp, ss = (t.numpy().astype(np.int64) for t in pat.sym_pairs())
+    # random element blocks with K_e[j][i] = K_e[i][j]^T exactly
+    rng = np.random.default_rng(0)
+    K = rng.standard_normal((E, B, B, H, H))
+    K = 0.5 * (K + K.transpose(0, 2, 1, 4, 3))
+    slot = K.reshape(E * B * B, H * H)
+    # plain reduction: entry k sums its slots left to right
+    sp0, ss0 = pat.seg_ptr.numpy(), pat.seg_slot.numpy()
+    full = np.zeros((pat.nnz, H * H))
+    for k in range(pat.nnz):
+        for t in range(sp0[k], sp0[k + 1]):
+            full[k] += slot[ss0[t]]
+    # paired reduction
+    tr = np.arange(H * H).reshape(H, H).T.reshape(-1)
+    out = np.full((pat.nnz, H * H), np.nan)
+    for k in range(entry.size):
+        acc = np.zeros(H * H)
+        for t in range(sp[k], sp[k + 1]):
+            raw = int(ss[t])
+            s, flip = raw & 0x7FFFFFFF, raw < 0
+            acc += slot[s][tr] if flip else slot[s]
+        out[entry[k]] = acc
+        if entry_t[k] != entry[k]:
+            out[entry_t[k]] = acc[tr]
+    assert not np.isnan(out).any(), "every entry is written"
+    assert np.array_equal(out, full)
+    rows = np.repeat(np.arange(m.n_points), np.diff(pat.row_ptr.numpy()))
+    cols = pat.col_idx.numpy()
+    assert (rows[entry] <= cols[entry]).all()
+    assert np.array_equal(rows[entry_t], cols[entry]) and np.array_equal(cols[entry_t], rows[entry])

@@ -33,7 +33,7 @@ extern "C" {
 
 typedef struct ihipStream_t* tm_stream_t; /* == hipStream_t */
 
-#define TM_ABI_VERSION 1
+#define TM_ABI_VERSION 2  /* 2: tm_pattern_view gained entry / entry_t */
 
 enum tm_status {
   TM_OK = 0,

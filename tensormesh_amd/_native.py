@@ -15,7 +15,7 @@ from typing import Optional
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libtmhip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # enum values (include/tensormesh_amd.h)
 TM_F32, TM_F64 = 0, 1

@@ -212,6 +212,19 @@ typedef struct tm_pcg_view {
 int tm_abi_version(void);
 const char* tm_last_error(void);
 
+/* Workspace sizes, in bytes, for a mesh (host arithmetic, no device call).
+ *   TM_WS_ELEMENT_SLOTS: slot_ws of tm_assemble_bilinear / tm_element_matrices
+ *     and tslot_ws of the Neo-Hookean and J2 tangents: n_elem*B*B*hh values
+ *     (hh = 1 scalar forms, D*D block forms)
+ *   TM_WS_NODE_SLOTS: slot_ws of tm_assemble_linear (hh = 1) and rslot_ws of
+ *     the Neo-Hookean / J2 residuals (hh = D): n_elem*B*hh values
+ *   TM_WS_ENERGY: energy_ws of tm_assemble_j2: n_elem + TM_J2_ENERGY_PARTS
+ *     doubles (hh ignored)
+ * Values are of the mesh dtype.  Returns the size, or a negative tm_status.
+ * The reference sizes these buffers inside torch ops (no counterpart). */
+enum tm_workspace { TM_WS_ELEMENT_SLOTS = 0, TM_WS_NODE_SLOTS = 1, TM_WS_ENERGY = 2 };
+int64_t tm_workspace_bytes(int32_t op, const tm_mesh_view* mesh, int32_t hh);
+
 /* ---------------------------------------------------------------------
  * Pattern (host).  Replaces the edge / projector construction of
  * ElementAssembler.from_mesh (tensormesh/assemble/element_assembler.py:804-853):

@@ -103,6 +103,7 @@ class IncidenceView(ctypes.Structure):
 # name -> (restype, argtypes); every symbol declared in include/tensormesh_amd.h
 SIGNATURES = {
     "tm_abi_version": (ctypes.c_int, []),
+    "tm_workspace_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.POINTER(MeshView), ctypes.c_int32]),
     "tm_last_error": (ctypes.c_char_p, []),
     "tm_pattern_nnz": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64, c_i64p]),
     "tm_pattern_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64,

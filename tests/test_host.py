@@ -22,7 +22,7 @@ ROOT = Path(__file__).resolve().parents[1]
 def test_library_exports_every_header_symbol():
     L = nat.lib()
     header = (ROOT / "include" / "tensormesh_amd.h").read_text()
-    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(tm_\w+)\s*\(", header, flags=re.M))
+    names = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(tm_\w+)\s*\(", header, flags=re.M))
     assert names, "no declarations parsed"
     assert names == set(nat.SIGNATURES), names ^ set(nat.SIGNATURES)
     for n in names:
@@ -532,3 +532,17 @@ def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
     cols = pat.col_idx.numpy()
     assert (rows[entry] <= cols[entry]).all()
     assert np.array_equal(rows[entry_t], cols[entry]) and np.array_equal(cols[entry_t], rows[entry])
+
+
+def test_workspace_bytes():
+    """tm_workspace_bytes (host arithmetic): slot buffers of the element and
+    node stages and the J2 energy workspace, with its argument errors."""
+    L = nat.lib()
+    mv = nat.MeshView(nat.CELL["tetra10"], nat.TM_F64, 7, 30, None, None)
+    assert L.tm_workspace_bytes(0, ctypes.byref(mv), 9) == 7 * 10 * 10 * 9 * 8
+    assert L.tm_workspace_bytes(1, ctypes.byref(mv), 3) == 7 * 10 * 3 * 8
+    assert L.tm_workspace_bytes(2, ctypes.byref(mv), 0) == (7 + nat.J2_ENERGY_PARTS) * 8
+    mv32 = nat.MeshView(nat.CELL["hexahedron27"], nat.TM_F32, 5, 30, None, None)
+    assert L.tm_workspace_bytes(0, ctypes.byref(mv32), 1) == 5 * 27 * 27 * 4
+    assert L.tm_workspace_bytes(3, ctypes.byref(mv), 1) == -1     # TM_ERR_ARG
+    assert L.tm_workspace_bytes(0, ctypes.byref(mv), 0) == -1

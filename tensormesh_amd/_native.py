@@ -229,7 +229,7 @@ def stream_of(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
-LDS_BYTES = 32768   # LDS staging buffer of the planned segmented reduction (tm_reduce.hip)
+LDS_BYTES = 16384   # LDS staging buffer of the planned segmented reduction (tm_reduce.hip)
 
 
 def reduce_cap(dtype: torch.dtype, hh: int) -> int:

@@ -104,10 +104,13 @@ typedef struct tm_form_desc {
 #define TM_FLAG_MFMA 1       /* float32 block tangents: contract on v_mfma_f32_32x32x2f32 */
 #define TM_FLAG_UPPER 2      /* symmetric forms on the wave, Neo-Hookean and J2 kernels:
                                 compute and store only the element blocks (i, j) with
-                                i <= j; the pattern's seg_slot must then be the list of
-                                tm_symmetric_segments.  The reference stores every block
-                                (element_assembler.py:811-830); the assembled values are
-                                the same.  Ignored by the P1 kernels, which store all. */
+                                i <= j, packed: block p (row-major over the pairs i <= j)
+                                of element e at slot e*B(B+1)/2 + p, so slot workspaces
+                                need n_elem*B(B+1)/2*hh values; the pattern's seg_slot
+                                must then be the list of tm_symmetric_segments.  The
+                                reference stores every block (element_assembler.py:
+                                811-830); the assembled values are the same.  Not for
+                                the P1 kernels, the MFMA tangent or the block-row path. */
 
 /* Sparsity pattern plus the slot -> nnz permutation in segment form. */
 typedef struct tm_pattern_view {
@@ -220,9 +223,11 @@ const char* tm_last_error(void);
  *     the Neo-Hookean / J2 residuals (hh = D): n_elem*B*hh values
  *   TM_WS_ENERGY: energy_ws of tm_assemble_j2: n_elem + TM_J2_ENERGY_PARTS
  *     doubles (hh ignored)
+ *   TM_WS_ELEMENT_SLOTS_UPPER: as TM_WS_ELEMENT_SLOTS with TM_FLAG_UPPER:
+ *     n_elem*B(B+1)/2*hh values
  * Values are of the mesh dtype.  Returns the size, or a negative tm_status.
  * The reference sizes these buffers inside torch ops (no counterpart). */
-enum tm_workspace { TM_WS_ELEMENT_SLOTS = 0, TM_WS_NODE_SLOTS = 1, TM_WS_ENERGY = 2 };
+enum tm_workspace { TM_WS_ELEMENT_SLOTS = 0, TM_WS_NODE_SLOTS = 1, TM_WS_ENERGY = 2, TM_WS_ELEMENT_SLOTS_UPPER = 3 };
 int64_t tm_workspace_bytes(int32_t op, const tm_mesh_view* mesh, int32_t hh);
 
 /* ---------------------------------------------------------------------
@@ -465,10 +470,12 @@ int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* s
                         const void* slot, void* out, tm_stream_t stream);
 
 /* Segment list for symmetric element blocks (TM_FLAG_UPPER), built once per
- * pattern: sym_slot[t] = seg_slot[t] where that slot s = e*B*B + i*B + j has
- * i <= j, else (e*B*B + j*B + i) | 0x80000000 -- the transposed upper block,
- * which tm_segmented_reduce reads as value b*D + a for value a*D + b of a
- * D x D entry (hh = 4 or 9; bit 31 only selects the slot for other hh).
+ * pattern: the slot s = e*B*B + i*B + j of seg_slot[t] becomes the packed
+ * upper slot e*B(B+1)/2 + p(a, b) of (a, b) = (min(i, j), max(i, j)),
+ * p(a, b) = a*B - a(a-1)/2 + b - a, with bit 31 (0x80000000) set when i > j:
+ * the transposed block, which tm_segmented_reduce reads as value b*D + a for
+ * value a*D + b of a D x D entry (hh = 4 or 9; bit 31 only selects the slot
+ * for other hh).
  * Segment order, and so the order of every sum, is unchanged.  No
  * counterpart in the reference, which stores all B*B blocks
  * (projector.py:95-123 sums them).  n_slots = n_elem * n_basis^2. */

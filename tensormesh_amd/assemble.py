@@ -312,12 +312,13 @@ class ElementAssembler(_AssemblerBase):
             values = self._p1_launch(path, mesh, form, pat, conn, dtype, dev)
             if values is not None:
                 return values
-        slot = torch.empty(E * B * B * H * H, dtype=dtype, device=dev)
-        values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
         # the element-matrix (wave) kernel serves everything but P1 scalar forms
         sym = not p1_scalar and self._symmetric_blocks(H > 1)
         if sym:
             form.flags |= nat.TM_FLAG_UPPER
+        pairs = B * (B + 1) // 2 if sym else B * B      # packed upper blocks, or all
+        slot = torch.empty(E * pairs * H * H, dtype=dtype, device=dev)
+        values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
         pv = pat.view(dtype, H * H, symmetric=sym)
         nat.check(L.tm_assemble_bilinear(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(pv),
                                          slot.data_ptr(), values.data_ptr(), nat.stream_of(dev)),
@@ -412,7 +413,8 @@ class ElementAssembler(_AssemblerBase):
             form.flags |= nat.TM_FLAG_UPPER
         pv = pat.view(dtype, D * D, symmetric=sym)
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
-        tslot = torch.empty(E * B * B * D * D, dtype=dtype, device=dev) if want_tangent else None
+        pairs = B * (B + 1) // 2 if sym else B * B      # packed upper blocks, or all
+        tslot = torch.empty(E * pairs * D * D, dtype=dtype, device=dev) if want_tangent else None
         tan = torch.empty(pat.nnz * D * D, dtype=dtype, device=dev) if want_tangent else None
         rslot = torch.empty(E * B * D, dtype=dtype, device=dev) if want_residual else None
         res = torch.empty(self.n_points * D, dtype=dtype, device=dev) if want_residual else None

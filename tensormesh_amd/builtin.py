@@ -286,7 +286,8 @@ class J2Plasticity(ElementAssembler):
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
         new = lambda n, want, dt=dtype: torch.empty(n, dtype=dt, device=dev) if want else None
         ews, energy = new(E + nat.J2_ENERGY_PARTS, want_energy, torch.float64), new((), want_energy)
-        tslot, tan = new(E * B * B * D * D, want_tangent), new(pat.nnz * D * D, want_tangent)
+        pairs = B * (B + 1) // 2 if sym else B * B      # packed upper blocks, or all
+        tslot, tan = new(E * pairs * D * D, want_tangent), new(pat.nnz * D * D, want_tangent)
         rslot, res = new(E * B * D, want_residual), new(self.n_points * D, want_residual)
         nat.check(L.tm_assemble_j2(ctypes.byref(self._mesh_view(points)), ctypes.byref(form),
                                    ctypes.byref(self._material()), u.data_ptr(), ctypes.byref(st), nat.ptr(ews),

@@ -37,6 +37,9 @@ extern "C" int64_t tm_workspace_bytes(int32_t op, const tm_mesh_view* mesh, int3
       return mesh->n_elem * B * hh * s;
     case TM_WS_ENERGY:
       return (mesh->n_elem + TM_J2_ENERGY_PARTS) * 8;
+    case TM_WS_ELEMENT_SLOTS_UPPER:
+      if (hh < 1 || hh > 9) return tmh::fail(TM_ERR_ARG, "hh must be in [1, 9]");
+      return mesh->n_elem * (B * (B + 1) / 2) * hh * s;
     default:
       return tmh::fail(TM_ERR_ARG, "unknown workspace");
   }

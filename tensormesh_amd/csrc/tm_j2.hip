@@ -55,7 +55,6 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
                    const double* __restrict__ tab, int Q, J2Consts k, int64_t n_elem, T* __restrict__ eps_p,
                    T* __restrict__ alpha, int update, int upper, double* __restrict__ e_out,
                    T* __restrict__ rslot, T* __restrict__ tslot) {
-  constexpr int kRowVals = B * D * D;
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
   static_assert(!BIG || EPW == 1, "one element per wavefront");
   __shared__ double s_tab[BIG ? 1 : MQ * (1 + B + D * B)];
@@ -277,7 +276,8 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
             acc[a * D + b] += lw * (vgi[a] * vgj[b]) + mw * (vgi[b] * vgj[a]) - cw * (vhi[a] * vhj[b]) +
                               (a == b ? mw * dot : 0.0);
       }
-      T* out = tslot + (e * B + i) * kRowVals + j * DD;  // 72 contiguous bytes at f64
+      // block p of the element's pairs (packed when upper): 72 contiguous bytes at f64
+      T* out = tslot + (e * n_pairs + p) * DD;
 #pragma unroll
       for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
     }

@@ -136,7 +136,9 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   __syncthreads();
   if (!active) return;
   constexpr int HH = H * H;
-  T* out = slot + e * (int64_t)(B * B * HH);
+  // pair p of the table is block (i, j): full blocks at p = i*B + j, upper
+  // blocks packed at their index among the pairs i <= j
+  T* out = slot + e * (int64_t)(n_pairs * HH);
   if constexpr (FORM == TM_FORM_LINEAR_ELASTICITY) {
     // one lane per node pair: the H x H block from one read of g_i, g_j
     for (int p = lane; p < n_pairs; p += kWave) {
@@ -163,7 +165,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
             acc[a * H + c] += (lam * (gi[a] * gj[c]) + mu * (gi[c] * gj[a]) + (a == c ? mu * dot : 0.0)) * w;
       }
 #pragma unroll
-      for (int k = 0; k < HH; ++k) out[(i * B + j) * HH + k] = (T)acc[k];
+      for (int k = 0; k < HH; ++k) out[p * HH + k] = (T)acc[k];
     }
     return;
   }
@@ -184,7 +186,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       }
       acc += val * s_jxw[wv][q];
     }
-    out[i * B + j] = (T)acc;
+    out[p] = (T)acc;
   }
 }
 
@@ -445,8 +447,10 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
       const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       T* out;
       if constexpr (!kStage) {
-        const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
-        out = tslot + srow * kRowVals + j * DD;
+        // element order: block p of the element's pairs (packed when upper);
+        // incidence order (full blocks only): slot row inc_inv[e*B + i]
+        out = inc_inv ? tslot + (int64_t)inc_inv[e * B + i] * kRowVals + j * DD
+                      : tslot + (e * n_pairs + p) * DD;
       } else {
         out = &s_out[wv][(i * B + j) * DD];
       }
@@ -494,8 +498,6 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
       typedef int32_t vec_t __attribute__((ext_vector_type(kVec / 4)));
       for (int v = lane; v < B * kPerRow; v += L) {
         const int i = v / kPerRow, o = v % kPerRow;
-        // upper: row i from the vector holding block (i, i) on (earlier blocks unused)
-        if (upper && o < (i * D * D * (int)sizeof(T)) / kVec) continue;
         const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
         const vec_t x = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
         reinterpret_cast<vec_t*>(tslot + srow * kRowVals)[o] = x;
@@ -529,6 +531,8 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   if (grid == 0) return TM_OK;
   const dim3 g((unsigned)grid), blk(kWave * kWpb);
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
+  if (upper && (mfma || inc_inv))
+    return tmh::fail(TM_ERR_ARG, "TM_FLAG_UPPER needs the element-ordered VALU tangent (no MFMA, no block rows)");
 #define TM_HYPER(MF, MQ, EPW)                                                                       \
   hyper_wave_kernel<T, D, B, MF, MQ, kBig, EPW><<<g, blk, 0, stream>>>(                             \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \

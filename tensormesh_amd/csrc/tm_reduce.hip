@@ -19,10 +19,11 @@
 // neighbouring runs read the same element slot rows, which then stay in one
 // XCD's L2.
 //
-// Symmetric segment lists (tm_symmetric_segments): a slot id with bit 31 set
-// names the transposed block -- value h = a*D + b of the entry is read from
-// position b*D + a of the slot -- so element kernels of symmetric forms store
-// only the blocks i <= j.
+// Symmetric segment lists (tm_symmetric_segments): slot ids name packed
+// upper blocks (element kernels of symmetric forms store only the blocks
+// i <= j, contiguously per element), and a slot id with bit 31 set names the
+// transposed block -- value h = a*D + b of the entry is read from position
+// b*D + a of the slot.
 //
 // Paired outputs (tm_segmented_reduce_pairs): segment k is summed once and
 // written to entry[k] and, transposed, to entry_t[k] -- the upper entries
@@ -233,15 +234,19 @@ int reduce_checked(int32_t dtype, int64_t n_seg, int32_t hh, const int32_t* seg_
   return tmh::fail(TM_ERR_ARG, "unknown dtype");
 }
 
-// Slot s = e*B*B + i*B + j with i > j becomes (e*B*B + j*B + i) | bit 31.
+// Slot s = e*B*B + i*B + j becomes the packed upper slot e*P + p(a, b) of
+// (a, b) = (min(i, j), max(i, j)), P = B(B+1)/2, p(a, b) = a*B - a(a-1)/2 + b - a
+// (row-major over a <= b), with bit 31 set when i > j (read transposed).
 __global__ void __launch_bounds__(kBlock) symmetric_slots_kernel(const int32_t* __restrict__ seg_slot, int64_t n,
                                                                  int32_t B, int32_t* __restrict__ sym) {
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
   const int32_t s = seg_slot[t];
-  const int32_t BB = B * B;
-  const int32_t r = s % BB, i = r / B, j = r % B;
-  sym[t] = i <= j ? s : ((s - r + j * B + i) | kTransposed);
+  const int32_t BB = B * B, P = B * (B + 1) / 2;
+  const int32_t e = s / BB, r = s % BB, i = r / B, j = r % B;
+  const int32_t a = i < j ? i : j, b = i < j ? j : i;
+  const int32_t packed = e * P + a * B - a * (a - 1) / 2 + (b - a);
+  sym[t] = i <= j ? packed : (packed | kTransposed);
 }
 
 }  // namespace

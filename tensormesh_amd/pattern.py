@@ -211,9 +211,10 @@ class Pattern:
                        self.hash, mv(self.inc_ptr), mv(self.inc))
 
     def sym_slot(self) -> torch.Tensor:
-        """Segment list for symmetric element blocks (TM_FLAG_UPPER): the slot of
-        every pair i > j replaced by its transposed upper block (bit 31 set),
-        built once on the pattern's device by tm_symmetric_segments."""
+        """Segment list for symmetric element blocks (TM_FLAG_UPPER): every slot
+        renamed to its packed upper block (the element's pairs i <= j stored
+        contiguously), with bit 31 set where the pair has i > j (read
+        transposed); built once on the pattern's device by tm_symmetric_segments."""
         if getattr(self, "_sym_slot", None) is None:
             nat.require_device(self.seg_slot, "pattern")
             sym = torch.empty_like(self.seg_slot)

@@ -112,8 +112,8 @@ def test_symmetric_blocks_j2(etype, qo):
 
 
 def test_symmetric_segment_list():
-    """tm_symmetric_segments on a tetra10 pattern: pairs i <= j keep their slot,
-    the others name the transposed block with bit 31 set."""
+    """tm_symmetric_segments on a tetra10 pattern: every slot becomes the packed
+    upper block of (min(i, j), max(i, j)), with bit 31 set when i > j."""
     m = _mesh("tetra10").to(DEV)
     A = tm.LaplaceElementAssembler.from_mesh(m, quadrature_order=2)
     A()
@@ -123,6 +123,8 @@ def test_symmetric_segment_list():
     B = pat.n_basis
     e, r = seg // (B * B), seg % (B * B)
     i, j = r // B, r % B
-    want = torch.where(i <= j, seg, (e * B * B + j * B + i) - 2 ** 31)
+    a, b = torch.minimum(i, j), torch.maximum(i, j)
+    packed = e * (B * (B + 1) // 2) + a * B - a * (a - 1) // 2 + (b - a)
+    want = torch.where(i <= j, packed, packed - 2 ** 31)
     assert torch.equal(sym, want)
     assert int((i > j).sum()) > 0 and int((i <= j).sum()) > 0

@@ -498,9 +498,12 @@ def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
     E, B = conn.shape
     pat = Pattern.build(conn, m.n_points)
     seg = pat.seg_slot.long()
-    r = seg % (B * B)
+    e, r = seg // (B * B), seg % (B * B)
     i, j = r // B, r % B
-    sym = torch.where(i <= j, seg, (seg - r + j * B + i) - 2 ** 31).to(torch.int32)
+    a, b = torch.minimum(i, j), torch.maximum(i, j)
+    P = B * (B + 1) // 2
+    packed = e * P + a * B - a * (a - 1) // 2 + (b - a)
+    sym = torch.where(i <= j, packed, packed - 2 ** 31).to(torch.int32)
     pat._sym_slot = sym                      # tm_symmetric_segments, restated
     entry, entry_t, sp, ss = (t.numpy().astype(np.int64) for t in pat.sym_pairs())
     # random element blocks with K_e[j][i] = K_e[i][j]^T exactly
@@ -508,6 +511,8 @@ def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
     K = rng.standard_normal((E, B, B, H, H))
     K = 0.5 * (K + K.transpose(0, 2, 1, 4, 3))
     slot = K.reshape(E * B * B, H * H)
+    iu, ju = np.triu_indices(B)              # row-major pairs i <= j: the packed order
+    packed_slot = K[:, iu, ju].reshape(E * P, H * H)
     # plain reduction: entry k sums its slots left to right
     sp0, ss0 = pat.seg_ptr.numpy(), pat.seg_slot.numpy()
     full = np.zeros((pat.nnz, H * H))
@@ -522,7 +527,7 @@ def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
         for t in range(sp[k], sp[k + 1]):
             raw = int(ss[t])
             s, flip = raw & 0x7FFFFFFF, raw < 0
-            acc += slot[s][tr] if flip else slot[s]
+            acc += packed_slot[s][tr] if flip else packed_slot[s]
         out[entry[k]] = acc
         if entry_t[k] != entry[k]:
             out[entry_t[k]] = acc[tr]
@@ -544,5 +549,6 @@ def test_workspace_bytes():
     assert L.tm_workspace_bytes(2, ctypes.byref(mv), 0) == (7 + nat.J2_ENERGY_PARTS) * 8
     mv32 = nat.MeshView(nat.CELL["hexahedron27"], nat.TM_F32, 5, 30, None, None)
     assert L.tm_workspace_bytes(0, ctypes.byref(mv32), 1) == 5 * 27 * 27 * 4
-    assert L.tm_workspace_bytes(3, ctypes.byref(mv), 1) == -1     # TM_ERR_ARG
+    assert L.tm_workspace_bytes(3, ctypes.byref(mv), 9) == 7 * 55 * 9 * 8   # packed upper blocks
+    assert L.tm_workspace_bytes(4, ctypes.byref(mv), 1) == -1     # TM_ERR_ARG
     assert L.tm_workspace_bytes(0, ctypes.byref(mv), 0) == -1

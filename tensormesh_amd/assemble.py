@@ -412,7 +412,8 @@ class ElementAssembler(_AssemblerBase):
         if sym:
             form.flags |= nat.TM_FLAG_UPPER
         pv = pat.view(dtype, D * D, symmetric=sym)
-        iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
+        node_plan = pat.plan(nat.reduce_cap(dtype, D), node=True)
+        iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), node_plan.numel() - 1, node_plan.data_ptr())
         pairs = B * (B + 1) // 2 if sym else B * B      # packed upper blocks, or all
         tslot = torch.empty(E * pairs * D * D, dtype=dtype, device=dev) if want_tangent else None
         tan = torch.empty(pat.nnz * D * D, dtype=dtype, device=dev) if want_tangent else None

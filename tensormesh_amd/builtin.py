@@ -283,7 +283,8 @@ class J2Plasticity(ElementAssembler):
                             nat.TM_FLAG_UPPER if sym else 0)
         st = nat.J2State(eps_p.data_ptr(), alpha.data_ptr())
         pv = pat.view(dtype, D * D, symmetric=sym)
-        iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), 0, None)
+        node_plan = pat.plan(nat.reduce_cap(dtype, D), node=True)
+        iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), node_plan.numel() - 1, node_plan.data_ptr())
         new = lambda n, want, dt=dtype: torch.empty(n, dtype=dt, device=dev) if want else None
         ews, energy = new(E + nat.J2_ENERGY_PARTS, want_energy, torch.float64), new((), want_energy)
         pairs = B * (B + 1) // 2 if sym else B * B      # packed upper blocks, or all

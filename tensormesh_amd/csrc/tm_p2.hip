@@ -679,8 +679,8 @@ extern "C" int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_de
     if (st) return st;
   }
   if (residual) {
-    st = tm_segmented_reduce(mesh->dtype, mesh->n_points, (int32_t)D, inc->inc_ptr, inc->inc, 0, nullptr, rslot_ws,
-                             residual, stream_);
+    st = tm_segmented_reduce(mesh->dtype, mesh->n_points, (int32_t)D, inc->inc_ptr, inc->inc, inc->n_blocks,
+                             inc->blk_ptr, rslot_ws, residual, stream_);
     if (st) return st;
   }
   return TM_OK;

@@ -114,21 +114,24 @@ class NeoHookeanModel(ElementAssembler):
 
 class _J2Energy(torch.autograd.Function):
     """Energy on tm_assemble_j2; its backward is the HIP residual (the
-    reference's ``energy().backward()``).  Second derivatives come from
-    ``J2Plasticity.tangent``, not from double backward."""
+    reference's ``energy().backward()``).  When the displacement requires a
+    gradient, the forward launch assembles the residual beside the energy (one
+    element pass instead of two) and backward only scales it.  Second
+    derivatives come from ``J2Plasticity.tangent``, not from double backward."""
 
     @staticmethod
     def forward(ctx, u, eps_p, alpha, model, points):
-        ctx.model, ctx.points = model, points
-        ctx.save_for_backward(u, eps_p, alpha)
-        return model._native_j2(u, eps_p, alpha, points, want_energy=True)[0]
+        want_grad = ctx.needs_input_grad[0]
+        energy, res, _ = model._native_j2(u, eps_p, alpha, points, want_energy=True, want_residual=want_grad)
+        ctx.save_for_backward(res if want_grad else None)
+        ctx.u_shape, ctx.u_dtype = u.shape, u.dtype
+        return energy
 
     @staticmethod
     @once_differentiable
     def backward(ctx, g):
-        u, eps_p, alpha = ctx.saved_tensors
-        res = ctx.model._native_j2(u, eps_p, alpha, ctx.points, want_residual=True)[1]
-        return (g * res).reshape(u.shape).to(u.dtype), None, None, None, None
+        (res,) = ctx.saved_tensors
+        return (g * res).reshape(ctx.u_shape).to(ctx.u_dtype), None, None, None, None
 
 
 class J2Plasticity(ElementAssembler):

@@ -91,18 +91,22 @@ class ContactAssembler(FacetAssembler):
 
 
 class _ContactEnergy(torch.autograd.Function):
+    """Penalty energy on tm_contact_penalty; when the positions require a
+    gradient the same launch assembles dE/dx, which backward scales."""
+
     @staticmethod
     def forward(ctx, x, model):
-        ctx.model = model
-        ctx.save_for_backward(x)
-        return model._native(x, want_energy=True)[0]
+        want_grad = ctx.needs_input_grad[0]
+        energy, grad, _ = model._native(x, want_energy=True, want_grad=want_grad)
+        ctx.save_for_backward(grad if want_grad else None)
+        ctx.x_shape, ctx.x_dtype = x.shape, x.dtype
+        return energy
 
     @staticmethod
     @once_differentiable
     def backward(ctx, g):
-        (x,) = ctx.saved_tensors
-        grad = ctx.model._native(x, want_grad=True)[1]
-        return (g * grad).reshape(x.shape).to(x.dtype), None
+        (grad,) = ctx.saved_tensors
+        return (g * grad).reshape(ctx.x_shape).to(ctx.x_dtype), None
 
 
 class PenaltyContact(ContactAssembler):

@@ -44,8 +44,8 @@ struct J2Consts {
   double mu, bulk, sig0, hard, denom, r23;
 };
 
-// EPW elements per wavefront, each on L = 64 / EPW lanes (EPW = 4 packs the
-// small cells, whose stages use at most 16 lanes).
+// EPW elements per wavefront, each on L = 64 / EPW lanes (the small cells,
+// whose stages use at most 16 lanes, pack four with a tangent, eight without).
 // TAN: the launch assembles the tangent (the h = m g arrays exist only then).
 // BIG (tetra20, hexahedron27): one wavefront per workgroup, rule tables read
 // through the caches instead of LDS.
@@ -295,7 +295,10 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
   if (Q <= 0 || Q > (kBig ? 32 : 16)) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   // four small elements (B <= 4) per wavefront when the rule has at most 4 points
-  const int epw = (B <= 4 && Q <= 4) ? 4 : 1;
+  // small cells (B <= 4, rules of <= 4 points): eight elements per wavefront
+  // without a tangent, four with one (its pair loop wants the lanes)
+  const bool tan = tslot && !update;
+  const int epw = (B <= 4 && Q <= 4) ? (tan ? 4 : 8) : 1;
   const int64_t per_block = (int64_t)kWpb * epw;
   const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
@@ -315,7 +318,10 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
   if constexpr (kBig) {
     if (Q <= 16) TM_J2(16, 1); else TM_J2(32, 1);
   } else if constexpr (B <= 4) {
-    if (epw == 4) TM_J2(4, 4); else if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
+    if (epw == 8) TM_J2_TAN(4, 8, false);
+    else if (epw == 4) TM_J2(4, 4);
+    else if (Q <= 4) TM_J2(4, 1);
+    else TM_J2(16, 1);
   } else {
     if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
   }

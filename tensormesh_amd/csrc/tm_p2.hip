@@ -37,24 +37,26 @@ constexpr int kMaxQWave = 32;  // element-matrix kernel (a 3 x 3 x 3 hexahedron 
 // MQ: largest number of quadrature points the LDS arrays hold.  upper: only
 // the blocks i <= j (symmetric forms, TM_FLAG_UPPER); products are formed so
 // that block (j, i) would be the exact transpose of block (i, j).
-template <typename T, int D, int B, int H, int FORM, int MQ>
+template <typename T, int D, int B, int H, int FORM, int MQ, int EPW>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     wave_element_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                         const T* __restrict__ coeff, const double* __restrict__ tab, int Q,
                         double lam, double mu, int64_t n_elem, int upper, T* __restrict__ slot) {
+  constexpr int L = kWave / EPW;                 // lanes per element
+  constexpr int kEpb = kWavesPerBlock * EPW;     // elements per workgroup
   __shared__ double s_tab[MQ * (1 + B + D * B)];
   __shared__ uint16_t s_pair[B * B];
-  __shared__ double s_x[kWavesPerBlock][B * D];
-  __shared__ double s_inv[kWavesPerBlock][MQ][D * D];
-  __shared__ double s_jxw[kWavesPerBlock][MQ];
-  __shared__ double s_kq[kWavesPerBlock][MQ];
-  __shared__ double s_g[kWavesPerBlock][MQ * B * D];
-  __shared__ int s_v[kWavesPerBlock][B];
+  __shared__ double s_x[kEpb][B * D];
+  __shared__ double s_inv[kEpb][MQ][D * D];
+  __shared__ double s_jxw[kEpb][MQ];
+  __shared__ double s_kq[kEpb][MQ];
+  __shared__ double s_g[kEpb][MQ * B * D];
+  __shared__ int s_v[kEpb][B];
 
-  static_assert(B <= kWave && MQ <= kWave, "lanes index the nodes and the quadrature points");
-  const int lane = threadIdx.x % kWave;
-  const int wv = threadIdx.x / kWave;
-  const int64_t e = (int64_t)blockIdx.x * kWavesPerBlock + wv;
+  static_assert(B <= L && MQ <= L, "an element's lanes index its nodes and its quadrature points");
+  const int lane = threadIdx.x % L;   // lane within the element's group
+  const int wv = threadIdx.x / L;     // element slot within the workgroup
+  const int64_t e = (int64_t)blockIdx.x * kEpb + wv;
   const bool active = e < n_elem;
 
   const int tab_len = Q + Q * B + Q * D * B;
@@ -69,7 +71,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   }
   __syncthreads();
   if (active) {
-    for (int t = lane; t < B * D; t += kWave) {
+    for (int t = lane; t < B * D; t += L) {
       const int b = t / D, d = t % D;
       s_x[wv][t] = (double)pts[(int64_t)s_v[wv][b] * D + d];
     }
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   }
   __syncthreads();
   if (active) {
-    for (int t = lane; t < Q * B * D; t += kWave) {
+    for (int t = lane; t < Q * B * D; t += L) {
       const int q = t / (B * D), b = (t / D) % B, d = t % D;
       double s = 0.0;
 #pragma unroll
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   T* out = slot + e * (int64_t)(n_pairs * HH);
   if constexpr (FORM == TM_FORM_LINEAR_ELASTICITY) {
     // one lane per node pair: the H x H block from one read of g_i, g_j
-    for (int p = lane; p < n_pairs; p += kWave) {
+    for (int p = lane; p < n_pairs; p += L) {
       const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       double acc[HH];
 #pragma unroll
@@ -170,7 +172,7 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     return;
   }
   static_assert(FORM == TM_FORM_LINEAR_ELASTICITY || H == 1, "scalar forms have one value per pair");
-  for (int p = lane; p < n_pairs; p += kWave) {
+  for (int p = lane; p < n_pairs; p += L) {
     const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
     double acc = 0.0;
     for (int q = 0; q < Q; ++q) {
@@ -580,15 +582,21 @@ int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hip
   const T* pts = static_cast<const T*>(m->points);
   const T* coeff = static_cast<const T*>(f->coeff);
   T* slot = static_cast<T*>(slot_ws);
-  const int64_t grid = (m->n_elem + kWavesPerBlock - 1) / kWavesPerBlock;
+  // two elements per wavefront (32 lanes each) where their LDS arrays fit a
+  // 40 KB workgroup: the per-quadrature-point stage (Q active lanes) then
+  // serves two elements
+  constexpr int kElemBytes = 8 * (B * D + MQ * (D * D + 2 + B * D)) + 4 * B;
+  constexpr int kEpw = (2 * kWavesPerBlock * kElemBytes <= 40960 && MQ <= kWave / 2 && B <= kWave / 2) ? 2 : 1;
+  const int64_t per_block = (int64_t)kWavesPerBlock * kEpw;
+  const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
   const dim3 block(kWave * kWavesPerBlock);
   double lam = 0.0, mu = 0.0;
   *hh_out = 1;
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
 #define TM_WAVE(H, FORM)                                                                                       \
-  wave_element_kernel<T, D, B, H, FORM, MQ><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, lam, \
-                                                                        mu, m->n_elem, upper, slot)
+  wave_element_kernel<T, D, B, H, FORM, MQ, kEpw><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, \
+                                                                              lam, mu, m->n_elem, upper, slot)
   switch (f->form) {
     case TM_FORM_LAPLACE:
       TM_WAVE(1, TM_FORM_LAPLACE);
@@ -629,6 +637,7 @@ int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* s
   if (Q <= 0 || Q > kMaxQWave) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   if (Q <= 4) return launch_wave<T, D, B, 4>(m, f, slot_ws, stream, hh_out);
+  if (Q <= 8) return launch_wave<T, D, B, 8>(m, f, slot_ws, stream, hh_out);
   if (Q <= 16) return launch_wave<T, D, B, 16>(m, f, slot_ws, stream, hh_out);
   return launch_wave<T, D, B, kMaxQWave>(m, f, slot_ws, stream, hh_out);
 }

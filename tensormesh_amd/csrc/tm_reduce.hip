@@ -142,30 +142,31 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
     for (; t < n; t += kBlock) lds[t] = slot[seg_slot[cb + t] & INT32_MAX];
   } else {
-    // consecutive threads read consecutive values of a slot; kGather
-    // independent (index, value) gathers per iteration keep the memory
-    // pipeline full
+    // the run's slot ids go to LDS in one coalesced pass, so that each value
+    // gather below is a single global round trip; consecutive threads then
+    // read consecutive values of a slot, kGather independent loads in flight
+    __shared__ int32_t s_ids[kCap];
+    for (int32_t t = threadIdx.x; t < n; t += kBlock) s_ids[t] = seg_slot[cb + t];
+    __syncthreads();
     constexpr int kGather = 4;
     const int32_t nv = n * HH;
     int32_t u = threadIdx.x;
     for (; u + (kGather - 1) * kBlock < nv; u += kGather * kBlock) {
-      int32_t t[kGather], h[kGather], sl[kGather];
+      int32_t t[kGather], h[kGather];
 #pragma unroll
       for (int k = 0; k < kGather; ++k) {
         t[k] = (u + k * kBlock) / HH;
         h[k] = u + k * kBlock - t[k] * HH;
       }
-#pragma unroll
-      for (int k = 0; k < kGather; ++k) sl[k] = seg_slot[cb + t[k]];
       T v[kGather];
 #pragma unroll
-      for (int k = 0; k < kGather; ++k) v[k] = slot[slot_value<HH>(sl[k], h[k])];
+      for (int k = 0; k < kGather; ++k) v[k] = slot[slot_value<HH>(s_ids[t[k]], h[k])];
 #pragma unroll
       for (int k = 0; k < kGather; ++k) lds[u + k * kBlock] = v[k];
     }
     for (; u < nv; u += kBlock) {
       const int32_t t = u / HH, h = u - t * HH;
-      lds[u] = slot[slot_value<HH>(seg_slot[cb + t], h)];
+      lds[u] = slot[slot_value<HH>(s_ids[t], h)];
     }
   }
   __syncthreads();

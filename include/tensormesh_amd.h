@@ -109,8 +109,9 @@ typedef struct tm_form_desc {
                                 need n_elem*B(B+1)/2*hh values; the pattern's seg_slot
                                 must then be the list of tm_symmetric_segments.  The
                                 reference stores every block (element_assembler.py:
-                                811-830); the assembled values are the same.  Not for
-                                the P1 kernels, the MFMA tangent or the block-row path. */
+                                811-830); the assembled values are the same.  Rejected
+                                (TM_ERR_ARG) by the P1 scalar kernels, the MFMA tangent
+                                and the block-row path. */
 
 /* Sparsity pattern plus the slot -> nnz permutation in segment form. */
 typedef struct tm_pattern_view {

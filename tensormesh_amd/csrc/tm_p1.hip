@@ -295,6 +295,8 @@ extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc*
   if (!form) return tmh::fail(TM_ERR_ARG, "form is NULL");
   if (mesh->n_elem > 0 && !slot_ws) return tmh::fail(TM_ERR_ARG, "slot workspace is NULL");
   const bool p1_scalar = (mesh->cell == TM_TRI3 || mesh->cell == TM_TET4) && form->form != TM_FORM_LINEAR_ELASTICITY;
+  if (p1_scalar && (form->flags & TM_FLAG_UPPER))
+    return tmh::fail(TM_ERR_ARG, "TM_FLAG_UPPER: the P1 scalar kernels store every element block");
   if (!p1_scalar) {
     int hh = 0;
     return tm_p2_element_matrices(mesh, form, slot_ws, stream, &hh);

@@ -128,3 +128,40 @@ def test_symmetric_segment_list():
     want = torch.where(i <= j, packed, packed - 2 ** 31)
     assert torch.equal(sym, want)
     assert int((i > j).sum()) > 0 and int((i <= j).sum()) > 0
+
+
+def test_symmetric_flag_is_rejected_where_blocks_are_stored_whole():
+    """TM_FLAG_UPPER on the P1 scalar kernels, the MFMA tangent and the
+    block-row path fails with TM_ERR_ARG before any launch (those store every
+    element block, which a symmetric segment list would misread)."""
+    import ctypes
+    from tensormesh_amd import _native as nat
+    from tensormesh_amd.assemble import _n_quad
+    L = nat.lib()
+    m = _mesh("tetra").to(DEV)
+    A = tm.LaplaceElementAssembler.from_mesh(m, quadrature_order=1)
+    tab, tab_dev = A._tables(torch.device(DEV))
+    mv = A._mesh_view(m.points)
+    E, B = A._conn32(torch.device(DEV)).shape
+    form = nat.FormDesc(nat.FORM["laplace"], _n_quad("tetra", 1), tab.ctypes.data, tab_dev.data_ptr(), None, 0.0,
+                        0.0, nat.TM_FLAG_UPPER)
+    slot = torch.empty(E * B * B, dtype=torch.float64, device=DEV)
+    assert L.tm_element_matrices(ctypes.byref(mv), ctypes.byref(form), slot.data_ptr(),
+                                 nat.stream_of(torch.device(DEV))) == -1
+    assert "P1" in L.tm_last_error().decode()
+    m10 = _mesh("tetra10")
+    md = tm.Mesh(m10.points.float(), {"tetra10": m10.elements()}, m10.boundary_mask).to(DEV)
+    model = tm.NeoHookeanModel.from_mesh(md, quadrature_order=2, E=1.0, nu=0.3)
+    tab, tab_dev = model._tables(torch.device(DEV))
+    mv = model._mesh_view(md.points)
+    E, B = model._conn32(torch.device(DEV)).shape
+    u = torch.zeros_like(md.points)
+    form = nat.FormDesc(0, _n_quad("tetra10", 2), tab.ctypes.data, tab_dev.data_ptr(), None, 1.0, 0.3,
+                        nat.TM_FLAG_UPPER | nat.TM_FLAG_MFMA)
+    pat = model._pattern_on(torch.device(DEV))
+    pv = pat.view(torch.float32, 9, symmetric=True)
+    tslot = torch.empty(E * B * B * 9, dtype=torch.float32, device=DEV)
+    tan = torch.empty(pat.nnz * 9, dtype=torch.float32, device=DEV)
+    st = L.tm_assemble_neohookean(ctypes.byref(mv), ctypes.byref(form), u.data_ptr(), ctypes.byref(pv), None,
+                                  tslot.data_ptr(), tan.data_ptr(), None, None, nat.stream_of(torch.device(DEV)))
+    assert st == -1 and "TM_FLAG_UPPER" in L.tm_last_error().decode()

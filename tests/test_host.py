@@ -552,3 +552,17 @@ def test_workspace_bytes():
     assert L.tm_workspace_bytes(3, ctypes.byref(mv), 9) == 7 * 55 * 9 * 8   # packed upper blocks
     assert L.tm_workspace_bytes(4, ctypes.byref(mv), 1) == -1     # TM_ERR_ARG
     assert L.tm_workspace_bytes(0, ctypes.byref(mv), 0) == -1
+
+
+def test_symmetric_slots_default_policy():
+    """symmetric_slots=None stores symmetric blocks for the D x D block forms
+    (the callers pass True there) and full blocks for scalar forms (False);
+    an explicit True / False overrides either."""
+    m = unit_square_tri(2, 0.0, 0, order=2)
+    A = tm.LaplaceElementAssembler.from_mesh(m, quadrature_order=2)
+    assert A.symmetric_slots is None
+    assert A._symmetric_blocks(True) is True and A._symmetric_blocks(False) is False
+    A.symmetric_slots = False
+    assert A._symmetric_blocks(True) is False
+    A.symmetric_slots = True
+    assert A._symmetric_blocks(False) is True

@@ -197,13 +197,30 @@ class FacetAssembler(nn.Module):
             assert value.shape[0] == self.n_points, f"the shape of {key} should be [n_point, ...], but got {value.shape}"
         fn = self.forward if func is None else func
         B = ELEMENT_SHAPE[self._topo.etype][2]
-        match = match_facet(fn, point_data, B) if points.is_cuda else None
+        match = self._match(fn, func is None, point_data, B) if points.is_cuda else None
         if match is not None:
             f = None if match.key is None else point_data[match.key]
             needs_grad = torch.is_grad_enabled() and (points.requires_grad or (f is not None and f.requires_grad))
             if not needs_grad:
                 return self._native(points, f, match)
         return self._torch(fn, points, point_data)
+
+    def _match(self, fn, is_forward: bool, point_data, n_basis: int) -> Optional[FacetMatch]:
+        """match_facet, cached per (integrand, the module's scalar attributes
+        for ``forward``, data layout): the probe runs once, not on every call."""
+        from .assemble import _scalar_state
+        func_id = getattr(fn, "__func__", fn)
+        layout = tuple(sorted((k, tuple(v.shape[1:]), v.dim()) for k, v in point_data.items()))
+        key = (func_id, _scalar_state(self) if is_forward else None, layout, n_basis)
+        cache = self.__dict__.setdefault("_match_cache", {})
+        try:
+            if key in cache:
+                return cache[key]
+        except TypeError:   # unhashable callable
+            return match_facet(fn, point_data, n_basis)
+        m = match_facet(fn, point_data, n_basis)
+        cache[key] = m
+        return m
 
     def _geometry(self, points: torch.Tensor):
         """phi [F, Qf, B], physical gradients [F, Qf, B, D], FxW [F, Qf]."""

@@ -430,7 +430,10 @@ typedef struct tm_contact_plane {
  *                        cell mesh's pattern; fpat->seg_ptr [nnz + 1] and
  *                        fpat->seg_slot list, per pattern entry, the facet
  *                        slots f*B*B + b*B + c that land on it (ascending);
- *                        entries with none come out 0.  tslot_ws [n_sel*B*B*D*D]
+ *                        entries with none come out 0.  With fpat->entry and
+ *                        entry_t (= entry) set, fpat lists only the entries
+ *                        that facets touch and only those are written: the
+ *                        caller zeroes hess.  tslot_ws [n_sel*B*B*D*D]
  * facet_tab_dev / n_quad as tm_assemble_facet_linear.  With no selected facet
  * (n_sel = 0) the requested outputs are set to zero. */
 int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view* facets, const double* facet_tab_dev,

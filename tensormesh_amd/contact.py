@@ -190,9 +190,13 @@ class PenaltyContact(ContactAssembler):
             entry = pat.slot2nnz.to(dev).long()[eslot.reshape(-1)]
             order = torch.argsort(entry, stable=True)
             counts = torch.bincount(entry, minlength=pat.nnz)
-            seg_ptr = torch.zeros(pat.nnz + 1, dtype=torch.int32, device=dev)
-            seg_ptr[1:] = torch.cumsum(counts, 0).to(torch.int32)
-            self._dev_cache[key] = (pat, seg_ptr, order.to(torch.int32).contiguous())
+            # only the entries some facet touches: the reduction writes those, the
+            # caller zeroes the rest of the Hessian (most of the cell pattern)
+            touched = torch.nonzero(counts).reshape(-1)
+            seg_ptr = torch.zeros(touched.numel() + 1, dtype=torch.int32, device=dev)
+            seg_ptr[1:] = torch.cumsum(counts[touched], 0).to(torch.int32)
+            self._dev_cache[key] = (pat, seg_ptr, order.to(torch.int32).contiguous(),
+                                    touched.to(torch.int32).contiguous())
         return self._dev_cache[key]
 
     def _wrap(self, values: torch.Tensor, pat: Pattern, dev) -> SparseMatrix:
@@ -228,9 +232,16 @@ class PenaltyContact(ContactAssembler):
         rslot, grad = new(F * B * D, want_grad), new(self.n_points * D, want_grad)
         pv, pat, tslot, hess = None, None, None, None
         if want_hess:
-            pat, seg_ptr, seg_slot = self._facet_segments(dev)
-            pv = nat.PatternView(pat.nnz, None, None, seg_ptr.data_ptr(), seg_slot.data_ptr(), 0, None)
-            tslot, hess = new(F * B * B * D * D, True), new(pat.nnz * D * D, True)
+            pat, seg_ptr, seg_slot, touched = self._facet_segments(dev)
+            key = f"contact_plan@{dev}@{dtype}"
+            if key not in self._dev_cache:
+                bp = nat.reduce_plan(seg_ptr.cpu().numpy(), nat.reduce_cap(dtype, D * D))
+                self._dev_cache[key] = torch.from_numpy(bp).to(dev)
+            bp = self._dev_cache[key]
+            pv = nat.PatternView(touched.numel(), None, None, seg_ptr.data_ptr(), seg_slot.data_ptr(),
+                                 bp.numel() - 1, bp.data_ptr(), touched.data_ptr(), touched.data_ptr())
+            tslot = new(F * B * B * D * D, True)
+            hess = torch.zeros(max(pat.nnz * D * D, 1), dtype=dtype, device=dev)
         nat.check(nat.lib().tm_contact_penalty(ctypes.byref(mesh), ctypes.byref(fv), tab.data_ptr(), nq,
                                                ctypes.byref(plane), x.data_ptr(), nat.ptr(ews), nat.ptr(energy),
                                                ctypes.byref(iv), nat.ptr(rslot), nat.ptr(grad),

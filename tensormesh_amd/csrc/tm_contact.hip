@@ -211,8 +211,7 @@ extern "C" int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view*
     if (st) return st;
   }
   if (hess) {
-    st = tm_segmented_reduce(mesh->dtype, fpat->nnz, (int32_t)(D * D), fpat->seg_ptr, fpat->seg_slot,
-                             fpat->n_blocks, fpat->blk_ptr, tslot_ws, hess, stream_);
+    st = tmh::reduce_view(mesh->dtype, fpat, (int32_t)(D * D), tslot_ws, hess, stream_);
     if (st) return st;
   }
   return TM_OK;

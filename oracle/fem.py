@@ -12,6 +12,7 @@ gradients; integer pattern work uses numpy/scipy exactly as
 
 from __future__ import annotations
 
+import hashlib
 import itertools
 from typing import Dict, Optional, Tuple
 
@@ -58,6 +59,50 @@ def pattern(conn: np.ndarray, n_points: int):
                                        shape=(n_points, n_points)).tocsr()
     eids = np.asarray(eids_csr[elem_u, elem_v]).ravel().astype(np.int64)
     return row, col, eids
+
+
+def reference_edges(conn, n_points: int) -> torch.Tensor:
+    """The ``edges`` buffer ``ElementAssembler.from_mesh`` builds and later
+    hands to ``SparseMatrix`` as ``edges[0]`` / ``edges[1]``, at the dtype it
+    really has.
+
+    Restates element_assembler.py:811-825 (slot pairs stacked per (i, j),
+    flattened, scipy ``coo -> csr -> coo``), :863 (``torch.from_numpy`` of the
+    stacked scipy ``row``/``col``: scipy chooses their int dtype) and :857-876
+    (``assembler.type(dtype)`` casts floating buffers only; ``edges`` keeps
+    its integer dtype).
+    """
+    conn = torch.as_tensor(conn).long()
+    n_elem, nb = conn.shape
+    elem_u, elem_v = [], []
+    for i in range(nb):
+        for j in range(nb):
+            elem_u.append(conn[:, i])
+            elem_v.append(conn[:, j])
+    elem_u = torch.stack(elem_u, -1).flatten().numpy().copy()
+    elem_v = torch.stack(elem_v, -1).flatten().numpy().copy()
+    tmp = scipy.sparse.coo_matrix((np.ones_like(elem_u), (elem_u, elem_v)),
+                                  shape=(n_points, n_points)).tocsr().tocoo()
+    return torch.from_numpy(np.stack([tmp.row, tmp.col], 0))
+
+
+def layout_hash(row: torch.Tensor, col: torch.Tensor) -> str:
+    """``SparseMatrix.__init__``'s digest (tensormesh/sparse/matrix.py:81-86):
+    SHA-256 of the bytes of the row tensor it was handed followed by those of
+    the col tensor, each at its own dtype."""
+    row_cpu, col_cpu = row.detach().cpu(), col.detach().cpu()
+    return hashlib.sha256(row_cpu.numpy().tobytes() + col_cpu.numpy().tobytes()).hexdigest()
+
+
+def block_coo_indices(row: torch.Tensor, col: torch.Tensor, block_size: int):
+    """Index half of ``SparseMatrix.from_block_coo`` (matrix.py:362-378), in
+    the reference's own torch ops so that torch's type promotion fixes the
+    dtype of the indices handed to ``SparseMatrix`` (int32 edges times an
+    int64 ``arange`` give int64)."""
+    row_exp = row[:, None].repeat(1, block_size * block_size)
+    col_exp = col[:, None].repeat(1, block_size * block_size)
+    i, j = torch.meshgrid(torch.arange(block_size), torch.arange(block_size), indexing="ij")
+    return (row_exp * block_size + i.flatten()).flatten(), (col_exp * block_size + j.flatten()).flatten()
 
 
 # ----------------------------------------------------- reference element ---

@@ -471,8 +471,9 @@ class ElementAssembler(_AssemblerBase):
     def _wrap_output(self, values: torch.Tensor, device) -> SparseMatrix:
         pat = self._pattern_on(device)
         if values.dim() == 1:
+            # digest of the scipy int32 edges (pattern.edge_index_dtype), on first use
             return SparseMatrix(values, pat.row, pat.col, (self.n_points, self.n_points),
-                                layout_hash=pat.hash, row_ptr=pat.row_ptr)
+                                layout_hash=lambda: pat.hash, row_ptr=pat.row_ptr, index_dtype=pat.edge_dtype)
         H = values.shape[1]
 
         def make_block():
@@ -707,6 +708,24 @@ class NodeAssembler(_AssemblerBase):
         point_data["x"] = points
         return self._torch_linear(fn, points, point_data, scalar_data, batch_size)
 
+    def _linear_args(self, points: torch.Tensor, f: Optional[torch.Tensor], const: float):
+        """ctypes arguments of tm_assemble_linear(_noalloc) for a device call:
+        (mesh view, form descriptor, constant or None, incidence view, the
+        tensors the views point into).  ``points`` and ``f`` must be
+        contiguous device tensors of one dtype."""
+        dev = points.device
+        tab, tab_dev = self._tables(dev)
+        inc_ptr, inc = (t.to(dev) for t in self._incidence)
+        cap = nat.reduce_cap(points.dtype, 1)
+        plan = self._cached(f"nodeplan{cap}", dev,
+                            lambda: torch.from_numpy(nat.reduce_plan(inc_ptr.cpu().numpy(), cap)).to(dev))
+        mesh = self._mesh_view(points)
+        form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
+                            None if f is None else f.data_ptr(), 0.0, 0.0)
+        iv = nat.IncidenceView(inc_ptr.data_ptr(), inc.data_ptr(), plan.numel() - 1, plan.data_ptr())
+        cval = None if f is not None else ctypes.c_double(const)
+        return mesh, form, cval, iv, (tab, tab_dev, inc_ptr, inc, plan, points, f, self._conn32(dev))
+
     def _native_linear(self, points, f, const) -> torch.Tensor:
         nat.require_device(points, "points")
         L = nat.lib()
@@ -719,20 +738,11 @@ class NodeAssembler(_AssemblerBase):
         if f is not None:
             nat.require_device(f, "point data")
             f = f.detach().to(points.dtype).contiguous()
-        tab, tab_dev = self._tables(dev)
-        inc_ptr, inc = (t.to(dev) for t in self._incidence)
-        cap = nat.reduce_cap(points.dtype, 1)
-        plan = self._cached(f"nodeplan{cap}", dev,
-                            lambda: torch.from_numpy(nat.reduce_plan(inc_ptr.cpu().numpy(), cap)).to(dev))
+        mesh, form, cval, iv, _keep = self._linear_args(points, f, const)
         slot = torch.empty(E * B, dtype=points.dtype, device=dev)
         out = torch.empty(points.shape[0], dtype=points.dtype, device=dev)
-        mesh = self._mesh_view(points)
-        form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
-                            None if f is None else f.data_ptr(), 0.0, 0.0)
-        iv = nat.IncidenceView(inc_ptr.data_ptr(), inc.data_ptr(), plan.numel() - 1, plan.data_ptr())
-        cval = ctypes.c_double(const)
         nat.check(L.tm_assemble_linear(ctypes.byref(mesh), ctypes.byref(form),
-                                       ctypes.byref(cval) if f is None else None, ctypes.byref(iv),
+                                       None if cval is None else ctypes.byref(cval), ctypes.byref(iv),
                                        slot.data_ptr(), out.data_ptr(), nat.stream_of(dev)),
                   "tm_assemble_linear")
         return out

@@ -23,10 +23,22 @@ from . import _native as nat
 
 
 def layout_hash(row: np.ndarray, col: np.ndarray) -> str:
-    """SHA-256 of the int64 row bytes followed by the int64 col bytes, the
-    digest ``SparseMatrix.__init__`` computes (tensormesh/sparse/matrix.py:81-86)."""
-    return hashlib.sha256(np.ascontiguousarray(row, dtype=np.int64).tobytes()
-                          + np.ascontiguousarray(col, dtype=np.int64).tobytes()).hexdigest()
+    """SHA-256 of the row bytes followed by the col bytes, each at the dtype it
+    comes in: the digest ``SparseMatrix.__init__`` computes from the tensors it
+    is handed (tensormesh/sparse/matrix.py:81-86)."""
+    return hashlib.sha256(np.ascontiguousarray(row).tobytes() + np.ascontiguousarray(col).tobytes()).hexdigest()
+
+
+INT32_MAX = 2 ** 31 - 1
+
+
+def edge_index_dtype(n_points: int, n_slots: int) -> np.dtype:
+    """Index dtype of the edges ``ElementAssembler.from_mesh`` hands to
+    ``SparseMatrix``: scipy's ``coo -> csr -> coo`` of the slot pairs
+    (element_assembler.py:821-825, 863) keeps int32 unless the matrix side or
+    the slot count (the COO entries before duplicates are summed) exceeds the
+    int32 range, where scipy switches to int64."""
+    return np.dtype(np.int32) if max(n_points, n_slots) <= INT32_MAX else np.dtype(np.int64)
 
 
 def repeated_vertex(conn: torch.Tensor) -> bool:
@@ -51,13 +63,36 @@ class Pattern:
     slot2nnz: torch.Tensor     # int32 [S]
     row: torch.Tensor          # int64 [nnz]
     col: torch.Tensor          # int64 [nnz]
-    hash: str
     inc_ptr: Optional[torch.Tensor] = None   # int32 [N+1]
     inc: Optional[torch.Tensor] = None       # int32 [E*B]
 
     def __post_init__(self):
         self._plans = {}
         self._colpos = None
+        self._hash = None
+
+    @property
+    def edge_dtype(self) -> torch.dtype:
+        """torch dtype of the reference's scalar-output edges (``edge_index_dtype``)."""
+        dt = edge_index_dtype(self.n_points, self.n_elem * self.n_basis * self.n_basis)
+        return torch.int32 if dt == np.int32 else torch.int64
+
+    @property
+    def hash(self) -> str:
+        """Layout digest of the scalar output, as the reference's
+        ``SparseMatrix`` computes it from the scipy edges (int32 row bytes +
+        int32 col bytes at every size the kernels accept; see
+        ``edge_index_dtype``).  Computed on first use from ``row_ptr`` /
+        ``col_idx`` and kept, so the 64 MB host round trip of the reference's
+        per-construction hash is paid once per pattern, and only when a
+        caller asks for the digest."""
+        if self._hash is None:
+            dt = edge_index_dtype(self.n_points, self.n_elem * self.n_basis * self.n_basis)
+            rp = self.row_ptr.cpu().numpy()
+            row = np.repeat(np.arange(self.n_points, dtype=dt), np.diff(rp))
+            col = self.col_idx.cpu().numpy().astype(dt, copy=False)
+            self._hash = layout_hash(row, col)
+        return self._hash
 
     # blocks of the row-centric P1 kernels (tm_rows.hip): at most ROWS_CAP
     # incidences and ROWS_MAX rows per workgroup
@@ -149,7 +184,6 @@ class Pattern:
                   "tm_pattern_build")
         row = np.repeat(np.arange(n_points, dtype=np.int64), np.diff(row_ptr))
         col = col_idx.astype(np.int64)
-        h = layout_hash(row, col)
         inc_ptr = inc = None
         if incidence:
             inc_ptr_np = np.empty(n_points + 1, np.int32)
@@ -160,7 +194,7 @@ class Pattern:
         dev = conn.device if device is None else torch.device(device)
         t = lambda a: torch.from_numpy(a).to(dev)
         return Pattern(n_points, E, B, nnz, t(row_ptr), t(col_idx), t(seg_ptr), t(seg_slot), t(slot2nnz),
-                       t(row), t(col), h,
+                       t(row), t(col),
                        None if inc_ptr is None else inc_ptr.to(dev), None if inc is None else inc.to(dev))
 
     @staticmethod
@@ -193,8 +227,7 @@ class Pattern:
                   "tm_pattern_device_fill")
         row = torch.repeat_interleave(torch.arange(n_points, device=dev), (row_ptr[1:] - row_ptr[:-1]).long())
         col = col_idx.long()
-        h = layout_hash(row.cpu().numpy(), col.cpu().numpy())
-        pat = Pattern(n_points, E, B, nnz, row_ptr, col_idx, seg_ptr, seg_slot, slot2nnz, row, col, h, inc_ptr, inc)
+        pat = Pattern(n_points, E, B, nnz, row_ptr, col_idx, seg_ptr, seg_slot, slot2nnz, row, col, inc_ptr, inc)
         if colpos is not None:
             too_long = bool(((row_ptr[1:] - row_ptr[:-1]) > 255).any().item())
             if not too_long and not repeated_vertex(conn):
@@ -206,9 +239,11 @@ class Pattern:
     def to(self, device) -> "Pattern":
         dev = torch.device(device)
         mv = lambda x: None if x is None else x.to(dev)
-        return Pattern(self.n_points, self.n_elem, self.n_basis, self.nnz, mv(self.row_ptr), mv(self.col_idx),
-                       mv(self.seg_ptr), mv(self.seg_slot), mv(self.slot2nnz), mv(self.row), mv(self.col),
-                       self.hash, mv(self.inc_ptr), mv(self.inc))
+        p = Pattern(self.n_points, self.n_elem, self.n_basis, self.nnz, mv(self.row_ptr), mv(self.col_idx),
+                    mv(self.seg_ptr), mv(self.seg_slot), mv(self.slot2nnz), mv(self.row), mv(self.col),
+                    mv(self.inc_ptr), mv(self.inc))
+        p._hash = self._hash
+        return p
 
     def sym_slot(self) -> torch.Tensor:
         """Segment list for symmetric element blocks (TM_FLAG_UPPER): every slot

@@ -5,10 +5,15 @@ Mirrors tensormesh/sparse/matrix.py:26-643 (``edata``/``row``/``col``/
 ``to_dense``, ``@``, ``solve``) without its ``torch_sla.SparseTensor`` base.
 Differences that matter for the hot path:
 
-* the SHA-256 ``layout_hash`` is the same digest as the reference's
-  (int64 row bytes + int64 col bytes) but is computed lazily, or handed over
-  by the assembler that built the pattern once, instead of copying
-  ``row``/``col`` to the host on every construction (matrix.py:81-86);
+* the SHA-256 ``layout_hash`` is the reference's digest -- the row bytes
+  followed by the col bytes at the dtype the constructor is handed
+  (matrix.py:81-86): int32 for the scipy edges of a scalar element assembly,
+  int64 for ``from_block_coo`` output, condensed blocks and every matrix
+  derived from another (``grad``, ``to``, ``detach``, arithmetic, transpose
+  rebuild from the int64 ``row_indices``, matrix.py:123-140,237-242) -- but
+  it is computed lazily, or handed over by the assembler that built the
+  pattern once, instead of copying ``row``/``col`` to the host on every
+  construction;
 * ``row_ptr`` (int32 CSR) travels with matrices whose entries are in CSR
   order, which the native SpMV and solver use.
 """
@@ -16,7 +21,7 @@ Differences that matter for the hot path:
 from __future__ import annotations
 
 import hashlib
-from typing import Optional, Tuple
+from typing import Callable, Optional, Tuple, Union
 
 import numpy as np
 import scipy.sparse
@@ -25,8 +30,14 @@ import torch
 
 class SparseMatrix:
     def __init__(self, edata: torch.Tensor, row: torch.Tensor, col: torch.Tensor,
-                 shape: Tuple[int, int], *, layout_hash: Optional[str] = None,
-                 row_ptr: Optional[torch.Tensor] = None):
+                 shape: Tuple[int, int], *, layout_hash: Union[str, Callable[[], str], None] = None,
+                 row_ptr: Optional[torch.Tensor] = None, index_dtype: Optional[torch.dtype] = None):
+        """Keyword-only extras (not in the reference signature):
+        ``layout_hash`` is the digest of ``row``/``col`` when the caller
+        already knows it, or a zero-argument function that returns it on first
+        use; ``index_dtype`` names the dtype the reference would have handed
+        over for both index arrays when the caller passes them already widened
+        to int64 (the assembler's scipy edges are int32)."""
         assert edata.dim() == 1, f"edata must be 1D, got {tuple(edata.shape)}"
         assert row.shape == edata.shape and col.shape == edata.shape, \
             f"row/col/edata must have the same length, got {row.shape}, {col.shape}, {edata.shape}"
@@ -34,7 +45,10 @@ class SparseMatrix:
         self.row_indices = row.long()
         self.col_indices = col.long()
         self.shape = (int(shape[0]), int(shape[1]))
+        # the digest hashes the indices at the dtype they were handed in
+        self._index_dtypes = (row.dtype, col.dtype) if index_dtype is None else (index_dtype, index_dtype)
         self._layout_hash = layout_hash
+        self._layout64 = None
         self.row_ptr = row_ptr
 
     # ---- reference-compatible accessors (matrix.py:88-120) ----
@@ -66,13 +80,30 @@ class SparseMatrix:
     def device(self) -> torch.device:
         return self.values.device
 
+    @staticmethod
+    def _digest(row: torch.Tensor, col: torch.Tensor) -> str:
+        return hashlib.sha256(row.detach().cpu().numpy().tobytes()
+                              + col.detach().cpu().numpy().tobytes()).hexdigest()
+
     @property
     def layout_hash(self) -> str:
+        """SHA-256 of the row bytes + col bytes at the dtypes given to the
+        constructor (matrix.py:81-86)."""
+        if callable(self._layout_hash):
+            self._layout_hash = self._layout_hash()
         if self._layout_hash is None:
-            self._layout_hash = hashlib.sha256(
-                self.row_indices.detach().cpu().numpy().astype(np.int64).tobytes()
-                + self.col_indices.detach().cpu().numpy().astype(np.int64).tobytes()).hexdigest()
+            rd, cd = self._index_dtypes
+            self._layout_hash = self._digest(self.row_indices.to(rd), self.col_indices.to(cd))
         return self._layout_hash
+
+    def _int64_layout(self) -> str:
+        """Digest of the int64 indices: the layout hash of every matrix the
+        reference derives from this one (it rebuilds them from ``row_indices``)."""
+        if self._index_dtypes == (torch.int64, torch.int64):
+            return self.layout_hash
+        if self._layout64 is None:
+            self._layout64 = self._digest(self.row_indices, self.col_indices)
+        return self._layout64
 
     def has_same_layout(self, other) -> bool:
         if isinstance(other, str):
@@ -88,8 +119,13 @@ class SparseMatrix:
         return self._like(self.values.grad)
 
     def _like(self, values: torch.Tensor) -> "SparseMatrix":
+        """A matrix on the same indices, as the reference rebuilds one from the
+        int64 ``row_indices`` (``grad``, ``_wrap``; matrix.py:123-140): its
+        digest is that of the int64 indices, which differs from this matrix's
+        own when this one was built from int32 edges."""
+        h = self._layout_hash if self._index_dtypes == (torch.int64, torch.int64) else self._layout64
         return SparseMatrix(values, self.row_indices, self.col_indices, self.shape,
-                            layout_hash=self._layout_hash, row_ptr=self.row_ptr)
+                            layout_hash=h, row_ptr=self.row_ptr)
 
     # ---- conversions ----
     def to_dense(self) -> torch.Tensor:
@@ -126,8 +162,9 @@ class SparseMatrix:
         v = self.values.to(*args, **kwargs)
         dev = v.device
         rp = None if self.row_ptr is None else self.row_ptr.to(dev)
+        h = self._layout_hash if self._index_dtypes == (torch.int64, torch.int64) else self._layout64
         return SparseMatrix(v, self.row_indices.to(dev), self.col_indices.to(dev), self.shape,
-                            layout_hash=self._layout_hash, row_ptr=rp)
+                            layout_hash=h, row_ptr=rp)
 
     def cuda(self, device=None) -> "SparseMatrix":
         return self.to("cuda" if device is None else device)
@@ -146,7 +183,11 @@ class SparseMatrix:
 
     # ---- arithmetic on a shared layout ----
     def _check_same(self, other: "SparseMatrix") -> None:
-        if self.shape != other.shape or not self.has_same_layout(other):
+        # structural comparison: the index tensors themselves, else their
+        # int64 digests (the dtype a matrix was built from does not matter here)
+        same = (self.row_indices is other.row_indices and self.col_indices is other.col_indices) or \
+            (self.row_indices.shape == other.row_indices.shape and self._int64_layout() == other._int64_layout())
+        if self.shape != other.shape or not same:
             raise ValueError("arithmetic between SparseMatrix objects needs an identical layout")
 
     def __add__(self, other):
@@ -205,11 +246,19 @@ class SparseMatrix:
         return SparseMatrix(tensor[rows, cols], rows, cols, tuple(tensor.shape))
 
     @staticmethod
-    def from_scipy_coo(matrix, device="cpu", dtype=torch.float64) -> "SparseMatrix":
+    def from_scipy_coo(matrix, device="cpu", dtype=torch.float) -> "SparseMatrix":
+        """matrix.py:269-291: values pass through float32 before ``dtype``
+        (as the reference does), indices become int64."""
         m = matrix.tocoo()
-        return SparseMatrix(torch.tensor(m.data, dtype=dtype, device=device),
-                            torch.tensor(m.row.astype(np.int64), device=device),
-                            torch.tensor(m.col.astype(np.int64), device=device), m.shape)
+        edata = torch.from_numpy(m.data.astype(np.float32)).to(device).type(dtype)
+        return SparseMatrix(edata, torch.from_numpy(m.row.astype(np.int64)).to(device),
+                            torch.from_numpy(m.col.astype(np.int64)).to(device), m.shape)
+
+    @staticmethod
+    def from_sparse_coo(matrix: torch.Tensor) -> "SparseMatrix":
+        """matrix.py:293-312: coalesce, then the COO indices as they are."""
+        matrix = matrix.coalesce()
+        return SparseMatrix(matrix.values(), matrix.indices()[0], matrix.indices()[1], tuple(matrix.shape))
 
     def __repr__(self) -> str:
         return f"SparseMatrix(shape={self.shape}, nnz={self.nnz}, dtype={self.dtype}, device={self.device})"

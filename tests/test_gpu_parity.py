@@ -65,8 +65,17 @@ def test_cfg1_laplace_load_condense_against_golden():
     b = SourceAssembler.from_mesh(mesh)(point_data={"f": f})
     ok, err = close64(b, torch.from_numpy(d["b"]))
     assert ok, err
+    # layout digest as the reference computes it (scipy int32 edges)
+    edges = fem.reference_edges(torch.from_numpy(d["cells"]), mesh.points.shape[0])
+    assert K.layout_hash == fem.layout_hash(edges[0], edges[1])
+    assert K.has_same_layout(fem.layout_hash(edges[0], edges[1]))
+    # matrices the reference rebuilds from the int64 row_indices hash int64
+    assert K.detach().layout_hash == fem.layout_hash(edges[0].long(), edges[1].long())
     cond = tm.Condenser(mesh.boundary_mask)
     Ki, fi = cond(K, b)
+    ref_c = fem.condense(K.values.cpu(), edges[0].long(), edges[1].long(), mesh.points.shape[0],
+                         mesh.boundary_mask.cpu())
+    assert Ki.layout_hash == fem.layout_hash(ref_c["K_ii"][1], ref_c["K_ii"][2])
     assert np.array_equal(Ki.row.cpu().numpy(), d["Kii_row"]) and np.array_equal(Ki.col.cpu().numpy(), d["Kii_col"])
     assert torch.equal(Ki.values.cpu(), K.values.cpu()[cond.inner_idx.cpu()])
     ok, err = close64(Ki.values, torch.from_numpy(d["Kii"]))
@@ -111,6 +120,8 @@ def test_tet10_elasticity_against_golden():
     mesh = Mesh(torch.from_numpy(d["points"]), {"tetra10": torch.from_numpy(d["cells"])}).to(DEV)
     K = tm.LinearElasticityElementAssembler.from_mesh(mesh, quadrature_order=2, E=1.0, nu=0.3)()
     assert np.array_equal(K.row.cpu().numpy(), d["row"]) and np.array_equal(K.col.cpu().numpy(), d["col"])
+    edges = fem.reference_edges(torch.from_numpy(d["cells"]), mesh.points.shape[0])
+    assert K.layout_hash == fem.layout_hash(*fem.block_coo_indices(edges[0], edges[1], 3))
     ok, err = close64(K.values, torch.from_numpy(d["values"]))
     assert ok, err
 
@@ -161,6 +172,9 @@ def test_cfg2_full_size_fp64_against_oracle(path):
     K = A(element_data={"kappa": Kp.to(DEV)})
     assert K.nnz == 4018753
     assert torch.equal(K.row.cpu(), ref_r) and torch.equal(K.col.cpu(), ref_c)
+    if path == "rows":
+        edges = fem.reference_edges(C, P.shape[0])
+        assert K.layout_hash == fem.layout_hash(edges[0], edges[1])
     ok, err = close64(K.values, ref_v)
     say(f"cfg2 fp64 max rel err {err:.3e}")
     assert ok, err

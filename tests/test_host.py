@@ -60,7 +60,8 @@ def test_pattern_builder_matches_reference_construction(kind, n):
         s = seg_slot[seg_ptr[k]:seg_ptr[k + 1]]
         assert np.all(np.diff(s) > 0)
         assert np.array_equal(np.sort(np.nonzero(eids == k)[0]), s)
-    assert pat.hash == layout_hash(row, col)
+    edges = fem.reference_edges(conn, m.n_points)
+    assert pat.hash == fem.layout_hash(edges[0], edges[1])
 
 
 def test_incidence_is_index_add_order():

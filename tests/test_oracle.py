@@ -47,6 +47,57 @@ def test_pattern_sizes_of_the_workloads():
     assert fem.pattern(c, p.shape[0])[0].size == 4018753
 
 
+# ---------------------------------------------------------- layout hash
+# Digest prefixes of the reference's own construction (scipy int32 edges,
+# SHA-256 of row bytes + col bytes) reproduced independently of this
+# repository's code: the 20x20 triangle workload and the 4x4 split square.
+_REFERENCE_DIGEST_PREFIX = {20: "f28f12ec", 4: "45799f"}
+
+
+def _packed_digest(pairs, fmt):
+    import hashlib
+    import struct
+    rows = [a for a, _ in pairs]
+    cols = [b for _, b in pairs]
+    return hashlib.sha256(struct.pack(f"<{len(rows)}{fmt}", *rows)
+                          + struct.pack(f"<{len(cols)}{fmt}", *cols)).hexdigest()
+
+
+@pytest.mark.parametrize("n", [4, 20])
+def test_layout_hash_known_answers_and_encoding(n):
+    p, c, _ = unit_square_tri_arrays(n, 0.2 if n == 20 else 0.0, 0)
+    edges = fem.reference_edges(torch.from_numpy(c), p.shape[0])
+    # scipy keeps 32-bit indices at this size; the digest hashes them as such
+    assert edges.dtype == torch.int32
+    h = fem.layout_hash(edges[0], edges[1])
+    assert h.startswith(_REFERENCE_DIGEST_PREFIX[n])
+    # independent encoding: brute-force sorted unique pairs packed as
+    # little-endian int32 by struct (no numpy / scipy / torch byte path)
+    pairs = sorted({(int(a), int(b)) for e in c for a in e for b in e})
+    assert h == _packed_digest(pairs, "i")
+    # the int64 digest of the same pairs differs (what a wrong dtype gives)
+    assert h != _packed_digest(pairs, "q")
+
+
+def test_layout_hash_block_and_condensed_are_int64():
+    p, c, _ = unit_cube_tet_arrays(1, 0.0, 0)
+    edges = fem.reference_edges(torch.from_numpy(c), p.shape[0])
+    r, cc = fem.block_coo_indices(edges[0], edges[1], 3)
+    assert r.dtype == torch.int64 and cc.dtype == torch.int64
+    pairs = sorted({(int(a), int(b)) for e in c for a in e for b in e})
+    blk = [(3 * a + i, 3 * b + j) for a, b in pairs for i in range(3) for j in range(3)]
+    assert fem.layout_hash(r, cc) == _packed_digest(blk, "q")
+    # condensed blocks carry the Condenser's int64 local numbering
+    mask = torch.zeros(p.shape[0], dtype=torch.bool)
+    mask[0] = True
+    out = fem.condense(torch.ones(edges.shape[1]), edges[0].long(), edges[1].long(), p.shape[0], mask)
+    _, kr, kc, _ = out["K_ii"]
+    assert kr.dtype == torch.int64
+    loc = {g: k for k, g in enumerate(i for i in range(p.shape[0]) if i != 0)}
+    inner = [(loc[a], loc[b]) for a, b in pairs if a != 0 and b != 0]
+    assert fem.layout_hash(kr, kc) == _packed_digest(inner, "q")
+
+
 # ------------------------------------------------------ reference element
 # degree of exactness of each rule of the reference's tables; its tetrahedron
 # "order 5" (14 points) and "order 6" (15 points) rules reach degrees 4 and 5

@@ -339,6 +339,23 @@ int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, c
                            const tm_pattern_view* pat, const tm_incidence_view* inc, void* tslot_ws,
                            void* tangent, void* rslot_ws, void* residual, tm_stream_t stream);
 
+/* Neo-Hookean energy at displacement disp [n_points, D]: replaces
+ * NeoHookeanModel.energy (builtin.py:316-333) through ElementAssembler.energy
+ * (element_assembler.py:447-659), energy = sum over elements and quadrature
+ * points of psi(F) * jxw with psi as for tm_assemble_neohookean.
+ *   energy [1]            mesh dtype; summed in a fixed order (each element's
+ *                         points in order, then TM_J2_ENERGY_PARTS chunk
+ *                         partials, then one workgroup)
+ *   energy_ws             double [n_elem + TM_J2_ENERGY_PARTS]; receives the
+ *                         per-element energies [n_elem]
+ *   residual [n_points*D] dE/du from the same element pass when non-NULL
+ *                         (the reference's energy().backward()); needs inc
+ *                         and rslot_ws [n_elem*B*D]
+ * form->flags is ignored (no tangent is formed). */
+int tm_neohookean_energy(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp, void* energy_ws,
+                         void* energy, const tm_incidence_view* inc, void* rslot_ws, void* residual,
+                         tm_stream_t stream);
+
 /* Neo-Hookean residual and tangent through incidence-ordered slot rows: the
  * element kernel writes the block row (e, a) (B blocks of D x D values, and D
  * residual values) at position inc_inv[e*B + a] of the node-sorted incidence
@@ -435,7 +452,9 @@ typedef struct tm_contact_plane {
  *                        that facets touch and only those are written: the
  *                        caller zeroes hess.  tslot_ws [n_sel*B*B*D*D]
  * facet_tab_dev / n_quad as tm_assemble_facet_linear.  With no selected facet
- * (n_sel = 0) the requested outputs are set to zero. */
+ * (n_sel = 0) energy and grad are set to zero, and so is hess for a full view
+ * (fpat->entry NULL); with a subset view hess is left to the caller's zeroing,
+ * as in every other call. */
 int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view* facets, const double* facet_tab_dev,
                        int32_t n_quad, const tm_contact_plane* plane, const void* x, void* energy_ws,
                        void* energy, const tm_incidence_view* inc, void* rslot_ws, void* grad,

@@ -144,6 +144,9 @@ SIGNATURES = {
                                               ctypes.POINTER(PatternView), ctypes.POINTER(IncidenceView),
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                               ctypes.c_void_p]),
+    "tm_neohookean_energy": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(IncidenceView),
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "tm_reduce_p1_rows": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RowsView), ctypes.c_void_p,
                                          ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
     "tm_element_matrices_inc": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,

@@ -39,7 +39,7 @@ import torch.nn as nn
 
 from . import _native as nat
 from .element import ELEMENT_SHAPE, reference_tables, torch_tables
-from .forms import FormMatch, match_bilinear, match_linear
+from .forms import FormMatch, MatchCache, match_bilinear, match_linear
 from .pattern import Pattern, layout_hash
 from .sparse import SparseMatrix
 
@@ -63,13 +63,6 @@ def shape_grad_jxw(points: torch.Tensor, conn: torch.Tensor, etype: str, qorder:
 def _n_quad(etype: str, qorder: int) -> int:
     from .element import quadrature
     return int(quadrature(etype, qorder)[0].shape[0])
-
-
-def _scalar_state(obj) -> tuple:
-    """Plain scalar attributes of a module (e.g. E, nu, c): part of the form
-    recognition cache key, so changing one re-probes the integrand."""
-    return tuple(sorted((k, v) for k, v in obj.__dict__.items()
-                        if isinstance(v, (int, float, bool, str)) and not k.startswith("_")))
 
 
 class _Broadcast:
@@ -242,7 +235,12 @@ class ElementAssembler(_AssemblerBase):
             assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
 
         fn = self.forward if func is None else func
-        match = None if self.reference_path else self._match(fn, func is None, element_data, point_data, scalar_data)
+        # the device decides the path: tensors on the ROCm device take the HIP
+        # kernels when the form is built in; CPU tensors run the reference's
+        # generic torch path, as the reference itself does on the CPU
+        on_device = points.device.type == "cuda"
+        match = None if (self.reference_path or not on_device) else \
+            self._match(fn, func is None, element_data, point_data, scalar_data)
         if match is not None:
             coeff = None
             if match.coeff_key is not None:
@@ -257,26 +255,16 @@ class ElementAssembler(_AssemblerBase):
         return self._torch_bilinear(fn, points, point_data, element_data, scalar_data, batch_size)
 
     def _match(self, fn, is_forward: bool, element_data, point_data, scalar_data) -> Optional[FormMatch]:
-        """Form recognition, cached per (integrand, data layout): the probe runs
-        once, not on every call."""
-        func_id = getattr(fn, "__func__", fn)
-        owner_state = _scalar_state(self) if is_forward else None
-
+        """Form recognition, cached per (integrand, its free state, data
+        layout): the probe runs again only when something the integrand can
+        read changes (``tensormesh_amd.forms.MatchCache``)."""
         def sig(d):
             return tuple(sorted((k, tuple(v.shape[1:]) if torch.is_tensor(v) else None) for k, v in d.items()))
 
-        key = (func_id, owner_state, sig(element_data), sig(point_data), tuple(sorted(scalar_data)))
-        cache = self.__dict__.setdefault("_match_cache", {})
-        try:
-            if key in cache:
-                return cache[key]
-        except TypeError:   # unhashable callable
-            return match_bilinear(fn, self.dimension, element_data, point_data, scalar_data,
-                                  owner=self if is_forward else None)
-        m = match_bilinear(fn, self.dimension, element_data, point_data, scalar_data,
-                           owner=self if is_forward else None)
-        cache[key] = m
-        return m
+        layout = (is_forward, sig(element_data), sig(point_data), tuple(sorted(scalar_data)))
+        cache = self.__dict__.setdefault("_match_cache", MatchCache())
+        return cache.get(fn, layout, lambda: match_bilinear(fn, self.dimension, element_data, point_data,
+                                                            scalar_data, owner=self if is_forward else None))
 
     # ---- native path ----
     def _native_values(self, match: FormMatch, points: torch.Tensor, coeff: Optional[torch.Tensor]) -> torch.Tensor:
@@ -693,13 +681,7 @@ class NodeAssembler(_AssemblerBase):
         for key, value in point_data.items():
             assert value.shape[0] == points.shape[0], f"the shape of {key} should be [n_point, ...], but got {value.shape}"
         fn = self.forward if func is None else func
-        key = (getattr(fn, "__func__", fn), _scalar_state(self) if func is None else None,
-               tuple(sorted((k, tuple(v.shape[1:])) for k, v in point_data.items())), tuple(sorted(scalar_data)))
-        cache = self.__dict__.setdefault("_match_cache", {})
-        try:
-            match = cache[key] if key in cache else cache.setdefault(key, match_linear(fn, point_data, scalar_data))
-        except TypeError:
-            match = match_linear(fn, point_data, scalar_data)
+        match = self._match(fn, point_data, scalar_data) if points.device.type == "cuda" else None
         if match is not None:
             f = None if match.key is None else point_data[match.key]
             needs_grad = torch.is_grad_enabled() and (points.requires_grad or (f is not None and f.requires_grad))
@@ -707,6 +689,12 @@ class NodeAssembler(_AssemblerBase):
                 return self._native_linear(points, f, match.const)
         point_data["x"] = points
         return self._torch_linear(fn, points, point_data, scalar_data, batch_size)
+
+    def _match(self, fn, point_data, scalar_data):
+        """match_linear, cached per (integrand, its free state, data layout)."""
+        layout = (tuple(sorted((k, tuple(v.shape[1:])) for k, v in point_data.items())), tuple(sorted(scalar_data)))
+        cache = self.__dict__.setdefault("_match_cache", MatchCache())
+        return cache.get(fn, layout, lambda: match_linear(fn, point_data, scalar_data))
 
     def _linear_args(self, points: torch.Tensor, f: Optional[torch.Tensor], const: float):
         """ctypes arguments of tm_assemble_linear(_noalloc) for a device call:

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+from typing import Optional
 
 import torch
 from torch.autograd.function import once_differentiable
@@ -96,7 +97,48 @@ class NeoHookeanModel(ElementAssembler):
         return (self.mu / 2) * (I1 - dim) - self.mu * log_J + (self.lam / 2) * (log_J ** 2)
 
     def energy(self, u):
+        """Total strain energy (builtin.py:316-333).  On the ROCm device it
+        runs on tm_neohookean_energy, differentiable in ``u`` (backward is the
+        HIP residual, assembled in the same launch when ``u`` requires a
+        gradient); on the CPU, or with ``reference_path``, it runs the
+        reference's generic energy path."""
+        if (u.device.type == "cuda" and not self.reference_path and not self._points.requires_grad
+                and u.dim() == 2 and tuple(u.shape) == tuple(self._points.shape)):
+            return _NHEnergy.apply(u, self)
         return super().energy(point_data={"displacement": u})
+
+    def _native_energy(self, u: torch.Tensor, want_residual: bool, points: Optional[torch.Tensor] = None):
+        """(energy [] in the mesh dtype, residual [N*D] or None) on tm_neohookean_energy."""
+        points = self._points if points is None else points
+        nat.require_device(points, "points")
+        L = nat.lib()
+        dev = points.device
+        dtype = points.dtype
+        points = points.detach().contiguous()
+        u = u.detach().to(dtype).contiguous()
+        conn = self._conn32(dev)
+        E, B = conn.shape
+        D = self.dimension
+        pat = self._pattern_on(dev)
+        tab, tab_dev = self._tables(dev)
+        mesh = self._mesh_view(points)
+        form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
+                            None, float(self.E), float(self.nu), 0)
+        ews = torch.empty(E + nat.J2_ENERGY_PARTS, dtype=torch.float64, device=dev)
+        energy = torch.empty((), dtype=dtype, device=dev)
+        res = rslot = None
+        iv = None
+        if want_residual:
+            node_plan = pat.plan(nat.reduce_cap(dtype, D), node=True)
+            iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), node_plan.numel() - 1,
+                                   node_plan.data_ptr())
+            rslot = torch.empty(E * B * D, dtype=dtype, device=dev)
+            res = torch.empty(self.n_points * D, dtype=dtype, device=dev)
+        p = lambda t: None if t is None else t.data_ptr()
+        nat.check(L.tm_neohookean_energy(ctypes.byref(mesh), ctypes.byref(form), u.data_ptr(), ews.data_ptr(),
+                                         energy.data_ptr(), None if iv is None else ctypes.byref(iv), p(rslot),
+                                         p(res), nat.stream_of(dev)), "tm_neohookean_energy")
+        return energy, res
 
     # The reference obtains these by autograd of ``energy`` (element_assembler.py:447-659);
     # here they are assembled directly on the HIP kernels.
@@ -110,6 +152,29 @@ class NeoHookeanModel(ElementAssembler):
 
     def residual_and_tangent(self, u, points=None):
         return self._native_hyperelastic(u, points, self.E, self.nu, True, True)
+
+
+class _NHEnergy(torch.autograd.Function):
+    """Neo-Hookean energy on tm_neohookean_energy; its backward is the HIP
+    residual (the reference's ``energy(u).backward()``, as in
+    examples/solid/hyperelastic_beam/hyperelastic_beam.py:133-175).  When
+    ``u`` requires a gradient the forward launch assembles the residual beside
+    the energy and backward only scales it.  Second derivatives come from
+    ``NeoHookeanModel.tangent``, not from double backward."""
+
+    @staticmethod
+    def forward(ctx, u, model):
+        want_grad = ctx.needs_input_grad[0]
+        energy, res = model._native_energy(u, want_residual=want_grad)
+        ctx.save_for_backward(res if want_grad else None)
+        ctx.u_shape, ctx.u_dtype = u.shape, u.dtype
+        return energy
+
+    @staticmethod
+    @once_differentiable
+    def backward(ctx, g):
+        (res,) = ctx.saved_tensors
+        return (g * res).reshape(ctx.u_shape).to(ctx.u_dtype), None
 
 
 class _J2Energy(torch.autograd.Function):

@@ -191,7 +191,9 @@ extern "C" int tm_contact_penalty(const tm_mesh_view* mesh, const tm_facet_view*
     const int64_t s_val = mesh->dtype == TM_F64 ? 8 : 4;
     int st = energy ? tmh::zero_fill(energy, s_val, stream_) : TM_OK;
     if (!st && grad) st = tmh::zero_fill(grad, mesh->n_points * D * s_val, stream_);
-    if (!st && hess) st = tmh::zero_fill(hess, fpat->nnz * D * D * s_val, stream_);
+    // a subset view (fpat->entry set) names only touched entries: the caller
+    // zeroes hess then, so nothing is written; a full view covers every entry
+    if (!st && hess && !fpat->entry) st = tmh::zero_fill(hess, fpat->nnz * D * D * s_val, stream_);
     return st;
   }
   Plane pl{plane->kappa, {plane->normal[0], plane->normal[1], plane->normal[2]}, plane->offset};

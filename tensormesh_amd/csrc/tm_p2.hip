@@ -225,7 +225,8 @@ template <typename T, int D, int B, bool MFMA, int MQ, bool BIG, int EPW>
 __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem, int upper,
-                      T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv) {
+                      T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv,
+                      double* __restrict__ energy_e) {
   constexpr int kMaxQ = MQ;
   constexpr int kRowVals = B * D * D;  // values of one slot row
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
@@ -243,6 +244,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __shared__ double s_h[kEpb][kMaxQ * B * D];
   __shared__ double s_P[kEpb][kMaxQ][D * D];
   __shared__ double s_c2[kEpb][kMaxQ];
+  __shared__ double s_psi[kEpb][kMaxQ];   // psi * jxw per point (energy launches)
   __shared__ double s_Fi[kEpb][kMaxQ][D * D];
   __shared__ int s_v[kEpb][B];
   __shared__ T s_out[kEpb][kStage ? B * kRowVals : 1];
@@ -358,8 +360,18 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
       Fi[1][2] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * r;
       Fi[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * r;
     }
-    const double c2 = lam * log(Jd) - mu;
+    const double lj = log(Jd);
+    const double c2 = lam * lj - mu;
     s_c2[wv][q] = c2;
+    if (energy_e) {
+      // psi = mu/2 (I1 - d) - mu ln J + lam/2 (ln J)^2, I1 = sum F_aA^2 (builtin.py:301-314)
+      double I1 = 0.0;
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int A = 0; A < D; ++A) I1 += F[a][A] * F[a][A];
+      s_psi[wv][q] = (0.5 * mu * (I1 - (double)D) - mu * lj + 0.5 * lam * lj * lj) * s_jxw[wv][q];
+    }
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
@@ -369,7 +381,14 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
       }
   }
   __syncthreads();
-  if (active) {
+  if (energy_e && active && lane == 0) {
+    // the element's energy, its points summed in order
+    double s = 0.0;
+    for (int q = 0; q < Q; ++q) s += s_psi[wv][q];
+    energy_e[e] = s;
+  }
+  if (active && tslot) {
+    // h = F^-T g, read by the tangent only
     for (int t = lane; t < Q * B * D; t += L) {
       const int q = t / (B * D), b = (t / D) % B, a = t % D;
       const double* g = &s_g[wv][(q * B + b) * D];
@@ -510,7 +529,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
 
 template <typename T, int D, int B>
 int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
-                 const int32_t* inc_inv, hipStream_t stream) {
+                 const int32_t* inc_inv, double* energy_e, hipStream_t stream) {
   // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
   // per workgroup, the BIG variant
   constexpr bool kBig = B * D > 32;
@@ -538,7 +557,7 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
 #define TM_HYPER(MF, MQ, EPW)                                                                       \
   hyper_wave_kernel<T, D, B, MF, MQ, kBig, EPW><<<g, blk, 0, stream>>>(                             \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \
-      (T*)tslot, (T*)rslot, inc_inv)
+      (T*)tslot, (T*)rslot, inc_inv, energy_e)
   if constexpr (kBig) {
     if (Q <= 16) TM_HYPER(false, 16, 1); else TM_HYPER(false, 32, 1);
   } else if (mfma) {
@@ -559,19 +578,12 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
 
 template <typename T>
 int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
-                   const int32_t* inc_inv,
-                   hipStream_t s) {
+                   const int32_t* inc_inv, double* energy_e, hipStream_t s) {
   switch (m->cell) {
-    case TM_TRI3: return launch_hyper<T, 2, 3>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_TET4: return launch_hyper<T, 3, 4>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_TRI6: return launch_hyper<T, 2, 6>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_TET10: return launch_hyper<T, 3, 10>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_QUAD4: return launch_hyper<T, 2, 4>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_QUAD9: return launch_hyper<T, 2, 9>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_HEX8: return launch_hyper<T, 3, 8>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_TRI10: return launch_hyper<T, 2, 10>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_TET20: return launch_hyper<T, 3, 20>(m, f, disp, tslot, rslot, inc_inv, s);
-    case TM_HEX27: return launch_hyper<T, 3, 27>(m, f, disp, tslot, rslot, inc_inv, s);
+#define TM_CASE(C, D, B) \
+  case C: return launch_hyper<T, D, B>(m, f, disp, tslot, rslot, inc_inv, energy_e, s);
+    TM_ALL_CELLS(TM_CASE)
+#undef TM_CASE
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }
@@ -657,8 +669,8 @@ int dispatch_cell(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, h
 
 int tm_hyper_element_rows(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp, void* tslot,
                           void* rslot, const int32_t* inc_inv, hipStream_t stream) {
-  return mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, tslot, rslot, inc_inv, stream)
-                               : dispatch_hyper<float>(mesh, form, disp, tslot, rslot, inc_inv, stream);
+  return mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, tslot, rslot, inc_inv, nullptr, stream)
+                               : dispatch_hyper<float>(mesh, form, disp, tslot, rslot, inc_inv, nullptr, stream);
 }
 
 int tm_p2_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
@@ -680,13 +692,44 @@ extern "C" int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_de
   if (residual && (!inc || !rslot_ws)) return tmh::fail(TM_ERR_ARG, "residual needs incidence and a workspace");
   void* ts = tangent ? tslot_ws : nullptr;
   void* rs = residual ? rslot_ws : nullptr;
-  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, nullptr, stream)
-                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, nullptr, stream);
+  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, nullptr, nullptr, stream)
+                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, nullptr, nullptr, stream);
   if (st) return st;
   if (tangent) {
     st = tmh::reduce_view(mesh->dtype, pat, (int32_t)(D * D), tslot_ws, tangent, stream_);
     if (st) return st;
   }
+  if (residual) {
+    st = tm_segmented_reduce(mesh->dtype, mesh->n_points, (int32_t)D, inc->inc_ptr, inc->inc, inc->n_blocks,
+                             inc->blk_ptr, rslot_ws, residual, stream_);
+    if (st) return st;
+  }
+  return TM_OK;
+  TM_TRY_END
+}
+
+extern "C" int tm_neohookean_energy(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
+                                    void* energy_ws, void* energy, const tm_incidence_view* inc, void* rslot_ws,
+                                    void* residual, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+  if (!mesh || !form || !disp || !energy || !energy_ws) return tmh::fail(TM_ERR_ARG, "NULL argument");
+  if (mesh->dtype != TM_F64 && mesh->dtype != TM_F32) return tmh::fail(TM_ERR_ARG, "unknown dtype");
+  const int64_t B = tmh::cell_basis(mesh->cell), D = tmh::cell_dim(mesh->cell);
+  if (B < 0) return tmh::fail(TM_ERR_ARG, "unknown cell");
+  if (mesh->n_elem < 0 || mesh->n_elem * B * D >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "mesh too large");
+  if (mesh->n_elem > 0 && (!mesh->conn || !mesh->points)) return tmh::fail(TM_ERR_ARG, "NULL buffer");
+  if (residual && (!inc || !rslot_ws)) return tmh::fail(TM_ERR_ARG, "residual needs incidence and a workspace");
+  // energy launches form no tangent: the tangent flags do not apply
+  tm_form_desc f = *form;
+  f.flags = 0;
+  double* e_out = static_cast<double*>(energy_ws);
+  void* rs = residual ? rslot_ws : nullptr;
+  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, &f, disp, nullptr, rs, nullptr, e_out, stream)
+                                 : dispatch_hyper<float>(mesh, &f, disp, nullptr, rs, nullptr, e_out, stream);
+  if (st) return st;
+  st = tmh::sum_fixed(e_out, mesh->n_elem, e_out + mesh->n_elem, mesh->dtype, energy, stream_);
+  if (st) return st;
   if (residual) {
     st = tm_segmented_reduce(mesh->dtype, mesh->n_points, (int32_t)D, inc->inc_ptr, inc->inc, inc->n_blocks,
                              inc->blk_ptr, rslot_ws, residual, stream_);

@@ -42,6 +42,7 @@ import torch.nn as nn
 
 from . import _native as nat
 from .element import ELEMENT_SHAPE, facet_nodes, facet_reference_tables, facet_tables
+from .forms import MatchCache, PolynomialGuard, probe
 
 
 @dataclass
@@ -68,49 +69,50 @@ def _rand(gen, *shape):
 
 def match_facet(fn: Callable, point_data: Dict[str, torch.Tensor], n_basis: int) -> Optional[FacetMatch]:
     """Recognise ``f * v`` / ``c * v`` (scalar or vector) by evaluating the
-    integrand on random inputs; None for anything else."""
+    integrand on random inputs under the polynomial guard of
+    ``tensormesh_amd.forms``; None for anything else."""
     try:
         names = list(inspect.signature(fn).parameters)
     except (TypeError, ValueError):
         return None
     gen = torch.Generator().manual_seed(8765)
+    guard = PolynomialGuard()
     try:
-        with torch.no_grad():
-            if names == ["v"]:
+        if names == ["v"]:
+            v = _rand(gen, n_basis)
+            out = probe(fn, {"v": v}, guard)
+            if out.shape == (n_basis,):
+                c = (out / v).mean()
+                const = c.reshape(1)
+            elif out.dim() == 2 and out.shape[0] == n_basis and 1 <= out.shape[1] <= 4:
+                const = (out / v[:, None]).mean(0)
+            else:
+                return None
+            for _ in range(3):
                 v = _rand(gen, n_basis)
-                out = torch.as_tensor(fn(v), dtype=torch.float64)
-                if out.shape == (n_basis,):
-                    c = (out / v).mean()
-                    const = c.reshape(1)
-                elif out.dim() == 2 and out.shape[0] == n_basis and 1 <= out.shape[1] <= 4:
-                    const = (out / v[:, None]).mean(0)
-                else:
+                ref = v * const[0] if out.dim() == 1 else v[:, None] * const[None, :]
+                got = probe(fn, {"v": v}, guard)
+                if got.shape != ref.shape or not torch.allclose(got, ref, rtol=1e-12, atol=1e-14):
                     return None
-                for _ in range(3):
-                    v = _rand(gen, n_basis)
-                    ref = v * const[0] if out.dim() == 1 else v[:, None] * const[None, :]
-                    got = torch.as_tensor(fn(v), dtype=torch.float64)
-                    if got.shape != ref.shape or not torch.allclose(got, ref, rtol=1e-12, atol=1e-14):
-                        return None
-                return FacetMatch(None, const, int(const.numel()))
-            if len(names) == 2 and "v" in names:
-                key = [n for n in names if n != "v"][0]
-                if key not in point_data:
+            return FacetMatch(None, const, int(const.numel())) if guard.ok else None
+        if len(names) == 2 and "v" in names:
+            key = [n for n in names if n != "v"][0]
+            if key not in point_data:
+                return None
+            data = point_data[key]
+            if data.dim() == 1:
+                H, fshape = 1, ()
+            elif data.dim() == 2 and 1 <= data.shape[1] <= 4:
+                H, fshape = int(data.shape[1]), (int(data.shape[1]),)
+            else:
+                return None
+            for _ in range(3):
+                v, f = _rand(gen, n_basis), _rand(gen, *fshape)
+                got = probe(fn, {"v": v, key: f}, guard)
+                ref = f * v if H == 1 and not fshape else v[:, None] * f[None, :]
+                if got.shape != ref.shape or not torch.allclose(got, ref, rtol=1e-12, atol=1e-14):
                     return None
-                data = point_data[key]
-                if data.dim() == 1:
-                    H, fshape = 1, ()
-                elif data.dim() == 2 and 1 <= data.shape[1] <= 4:
-                    H, fshape = int(data.shape[1]), (int(data.shape[1]),)
-                else:
-                    return None
-                for _ in range(3):
-                    v, f = _rand(gen, n_basis), _rand(gen, *fshape)
-                    got = torch.as_tensor(fn(**{"v": v, key: f}), dtype=torch.float64)
-                    ref = f * v if H == 1 and not fshape else v[:, None] * f[None, :]
-                    if got.shape != ref.shape or not torch.allclose(got, ref, rtol=1e-12, atol=1e-14):
-                        return None
-                return FacetMatch(key, None, H)
+            return FacetMatch(key, None, H) if guard.ok else None
     except Exception:
         return None
     return None
@@ -206,21 +208,13 @@ class FacetAssembler(nn.Module):
         return self._torch(fn, points, point_data)
 
     def _match(self, fn, is_forward: bool, point_data, n_basis: int) -> Optional[FacetMatch]:
-        """match_facet, cached per (integrand, the module's scalar attributes
-        for ``forward``, data layout): the probe runs once, not on every call."""
-        from .assemble import _scalar_state
-        func_id = getattr(fn, "__func__", fn)
-        layout = tuple(sorted((k, tuple(v.shape[1:]), v.dim()) for k, v in point_data.items()))
-        key = (func_id, _scalar_state(self) if is_forward else None, layout, n_basis)
-        cache = self.__dict__.setdefault("_match_cache", {})
-        try:
-            if key in cache:
-                return cache[key]
-        except TypeError:   # unhashable callable
-            return match_facet(fn, point_data, n_basis)
-        m = match_facet(fn, point_data, n_basis)
-        cache[key] = m
-        return m
+        """match_facet, cached per (integrand, its free state, data layout):
+        the probe runs again only when something the integrand can read
+        changes (``tensormesh_amd.forms.MatchCache``)."""
+        layout = (is_forward, tuple(sorted((k, tuple(v.shape[1:]), v.dim()) for k, v in point_data.items())),
+                  n_basis)
+        cache = self.__dict__.setdefault("_match_cache", MatchCache())
+        return cache.get(fn, layout, lambda: match_facet(fn, point_data, n_basis))
 
     def _geometry(self, points: torch.Tensor):
         """phi [F, Qf, B], physical gradients [F, Qf, B, D], FxW [F, Qf]."""

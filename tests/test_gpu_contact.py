@@ -128,3 +128,39 @@ def test_contact_float32_against_float64():
     # penetrations are differences of positions ~ 0.1: float32 rounding of x
     # gives ~1e-6 relative error in them
     assert dk < 1e-5 and dr < 1e-4 and abs(e32 - e64) < 1e-4 * e64
+
+
+def test_no_selected_facet_through_the_abi():
+    """tm_contact_penalty with n_sel = 0: a full Hessian view is zeroed, a
+    subset view (entry set) leaves the output to the caller's zeroing."""
+    import ctypes
+    from tensormesh_amd import _native as nat
+    say("case contact n_sel=0 through the C-ABI")
+    L = nat.lib()
+    dev = torch.device(DEV)
+    m = _mesh("tetra", n=2).to(DEV)
+    conn = m.elements().to(torch.int32).contiguous()
+    mv = nat.MeshView(nat.CELL["tetra"], nat.TM_F64, conn.shape[0], m.points.shape[0], conn.data_ptr(),
+                      m.points.data_ptr())
+    fv = nat.FacetView(0, None, None)
+    tab = torch.zeros(64, dtype=torch.float64, device=DEV)
+    plane = nat.ContactPlane(1.0, (ctypes.c_double * 3)(0.0, 1.0, 0.0), 0.0)
+    nnz, D = 12, 3
+    idx = torch.arange(nnz, dtype=torch.int32, device=DEV)
+    seg_ptr = torch.zeros(nnz + 1, dtype=torch.int32, device=DEV)
+    tslot = torch.zeros(9, dtype=torch.float64, device=DEV)
+    hess = torch.full((2 * nnz * D * D,), 7.0, dtype=torch.float64, device=DEV)
+    st = nat.stream_of(dev)
+    sub = nat.PatternView(nnz, None, None, seg_ptr.data_ptr(), idx.data_ptr(), 0, None, idx.data_ptr(),
+                          idx.data_ptr())
+    nat.check(L.tm_contact_penalty(ctypes.byref(mv), ctypes.byref(fv), tab.data_ptr(), 1, ctypes.byref(plane),
+                                   m.points.data_ptr(), None, None, None, None, None, ctypes.byref(sub),
+                                   tslot.data_ptr(), hess.data_ptr(), st), "tm_contact_penalty")
+    torch.cuda.synchronize()
+    assert bool((hess == 7.0).all()), "a subset view must not write the Hessian"
+    full = nat.PatternView(nnz, None, None, seg_ptr.data_ptr(), idx.data_ptr(), 0, None, None, None)
+    nat.check(L.tm_contact_penalty(ctypes.byref(mv), ctypes.byref(fv), tab.data_ptr(), 1, ctypes.byref(plane),
+                                   m.points.data_ptr(), None, None, None, None, None, ctypes.byref(full),
+                                   tslot.data_ptr(), hess.data_ptr(), st), "tm_contact_penalty")
+    torch.cuda.synchronize()
+    assert bool((hess[:nnz * D * D] == 0.0).all()) and bool((hess[nnz * D * D:] == 7.0).all())

@@ -13,7 +13,7 @@ import tensormesh_amd as tm
 from oracle import fem
 from tensormesh_amd import _native as nat
 from tensormesh_amd.forms import match_bilinear, match_linear
-from tensormesh_amd.mesh import quadratic_arrays, unit_cube_tet, unit_cube_tet_arrays, unit_square_tri
+from tensormesh_amd.mesh import lognormal_coefficient, quadratic_arrays, unit_cube_tet, unit_cube_tet_arrays, unit_square_tri
 from tensormesh_amd.pattern import Pattern, layout_hash
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -104,11 +104,94 @@ def test_form_recognition():
     assert match_linear(lambda x, v: x[0] * v, {"x": torch.ones(3, 2)}, {}) is None
 
 
-def test_builtin_forms_refuse_cpu_tensors():
-    m = unit_square_tri(4, 0.1, 0)
-    K = tm.LaplaceElementAssembler.from_mesh(m)
-    with pytest.raises(nat.NativeError, match="ROCm device"):
-        K()
+def test_builtin_forms_on_cpu_tensors_run_the_torch_path():
+    """On CPU tensors the built-in classes run the reference's generic torch
+    path (as the reference does on the CPU), with the reference's values,
+    pattern and layout digest; the HIP kernels serve device tensors only."""
+    m = unit_square_tri(20, 0.2, 0)
+    K = tm.LaplaceElementAssembler.from_mesh(m)()
+    v, r, c, _ = fem.assemble_bilinear(m.points, m.elements(), "triangle", 2, "laplace")
+    assert K.device.type == "cpu" and torch.equal(K.row, r) and torch.equal(K.col, c)
+    assert torch.allclose(K.values, v, rtol=1e-13, atol=1e-15)
+    edges = fem.reference_edges(m.elements(), m.n_points)
+    assert K.layout_hash == fem.layout_hash(edges[0], edges[1])
+    kap = torch.from_numpy(lognormal_coefficient(m.elements().shape[0], 5))
+    Kd = tm.DiffusionElementAssembler.from_mesh(m)(element_data={"kappa": kap})
+    vd = fem.assemble_bilinear(m.points, m.elements(), "triangle", 2, "diffusion_elem", kappa=kap)[0]
+    assert torch.allclose(Kd.values, vd, rtol=1e-13, atol=1e-15)
+    f = 2 * math.pi ** 2 * torch.sin(math.pi * m.points[:, 0]) * torch.sin(math.pi * m.points[:, 1])
+    b = tm.SourceNodeAssembler.from_mesh(m)(point_data={"f": f})
+    assert torch.allclose(b, fem.assemble_linear(m.points, m.elements(), "triangle", 2, f), rtol=1e-13, atol=1e-16)
+
+
+def test_near_miss_forms_are_not_recognised():
+    """Integrands that agree with a built-in at (almost) every point but are
+    not polynomials keep the generic path: the polynomial guard rejects the
+    operation, whatever the values at the probe points."""
+    near = lambda gradu, gradv: torch.where(gradu[0] > 0.999999, 2.0 * (gradu @ gradv), gradu @ gradv)
+    assert match_bilinear(near, 3, {}, {}, {}) is None
+
+    def branch(gradu, gradv):
+        if gradu[0] > 0.999999:
+            return 2.0 * (gradu @ gradv)
+        return gradu @ gradv
+    assert match_bilinear(branch, 3, {}, {}, {}) is None
+    clamp = lambda u, v: torch.clamp(u, -10.0, 10.0) * v          # equal to u * v on the probe range
+    assert match_bilinear(clamp, 3, {}, {}, {}) is None
+    detached = lambda gradu, gradv: gradu.detach() @ gradv
+    assert match_bilinear(detached, 3, {}, {}, {}) is None
+    via_abs = lambda v: (v.abs() + v - v.abs()) * 1.5             # = 1.5 v, through a non-polynomial op
+    assert match_linear(via_abs, {}, {}) is None
+    # polynomial spellings of the built-ins are still recognised
+    assert match_bilinear(lambda gradu, gradv: (gradu * gradv).sum(), 3, {}, {}, {}).form == "laplace"
+    assert match_bilinear(lambda gradu, gradv: torch.einsum("i,i->", gradu, gradv) / 1.0, 2, {}, {}, {}).form \
+        == "laplace"
+
+
+def test_recognition_cache_follows_closures_attributes_and_globals():
+    """A cached recognition is redone when anything the integrand reads
+    changes (closure cell, module attribute, tensor attribute in place)."""
+    m = unit_square_tri(3, 0.0, 0)
+    node = tm.NodeAssembler.from_mesh(m)
+    scale = [1.0]
+    fn = lambda v: scale[0] * v
+    pd = {}
+    assert abs(node._match(fn, pd, {}).const - 1.0) < 1e-15
+    scale[0] = 5.0
+    assert abs(node._match(fn, pd, {}).const - 5.0) < 1e-15
+
+    def make(c):
+        return lambda v: c * v
+    f2 = make(2.0)
+    assert abs(node._match(f2, pd, {}).const - 2.0) < 1e-15
+
+    class Src(tm.NodeAssembler):
+        def forward(self, v):
+            return self.k * v
+    src = Src.from_mesh(m)
+    src.k = torch.tensor(2.0, dtype=torch.float64)
+    assert abs(src._match(src.forward, pd, {}).const - 2.0) < 1e-15
+    src.k.fill_(7.0)                                              # in place: version counter moves
+    assert abs(src._match(src.forward, pd, {}).const - 7.0) < 1e-15
+    src.k = 3.0
+    assert abs(src._match(src.forward, pd, {}).const - 3.0) < 1e-15
+
+    el = tm.ElementAssembler.from_mesh(m)
+    c = {"v": 1.0}
+    form = lambda gradu, gradv: c["v"] * (gradu @ gradv)
+    assert el._match(form, False, {}, {}, {}).form == "laplace"
+    c["v"] = 2.0                                                  # now a scaled Laplacian: not built in
+    assert el._match(form, False, {}, {}, {}) is None
+    fa = tm.FacetAssembler.from_mesh(m)
+    sc = [1.0]
+    ffn = lambda v: sc[0] * v
+    assert float(fa._match(ffn, False, {}, 3).const[0]) == pytest.approx(1.0, abs=1e-15)
+    sc[0] = 5.0
+    assert float(fa._match(ffn, False, {}, 3).const[0]) == pytest.approx(5.0, abs=1e-15)
+    # the cache is bounded: per-call lambdas do not pile up
+    for i in range(100):
+        node._match(lambda v, i=i: float(i) * v, pd, {})
+    assert len(node._match_cache._d) <= node._match_cache.size
 
 
 def test_user_form_torch_path_matches_oracle():
@@ -227,7 +310,7 @@ def test_row_centric_summation_reproduces_oracle_on_cpu():
 
 # ------------------------------------------------------------------ facets ---
 def _facet_mesh(kind, order=1, mask="all"):
-    from tensormesh_amd.mesh import quadratic_arrays, unit_cube_tet_arrays, unit_square_tri_arrays
+    from tensormesh_amd.mesh import lognormal_coefficient, quadratic_arrays, unit_cube_tet_arrays, unit_square_tri_arrays
     if kind == "tri":
         p, c, bm = unit_square_tri_arrays(5, 0.2, 1)
         et = "triangle" if order == 1 else "triangle6"

@@ -33,7 +33,7 @@ extern "C" {
 
 typedef struct ihipStream_t* tm_stream_t; /* == hipStream_t */
 
-#define TM_ABI_VERSION 2  /* 2: tm_pattern_view gained entry / entry_t */
+#define TM_ABI_VERSION 3  /* 2: tm_pattern_view gained entry / entry_t; 3: tm_rows_view gained dest / ent_start */
 
 enum tm_status {
   TM_OK = 0,
@@ -129,6 +129,14 @@ typedef struct tm_pattern_view {
    * TM_FLAG_UPPER segment list fill the lower ones. */
   const int32_t* entry;      /* [nnz] */
   const int32_t* entry_t;    /* [nnz]; == entry[k] on the diagonal */
+  /* Optional contiguous contributions (tm_slot_dest_build), with a
+   * TM_FLAG_UPPER segment list: the element kernels of tm_assemble_bilinear,
+   * tm_assemble_neohookean and tm_assemble_j2 launched with TM_FLAG_UPPER
+   * write packed block p of element e at slot (slot_dest[e*P + p] &
+   * INT32_MAX), P = B(B+1)/2, transposed when bit 31 is set; segment k's
+   * contributions are then the consecutive slots [seg_ptr[k], seg_ptr[k+1])
+   * and seg_slot is not read.  Same values and summation order. */
+  const int32_t* slot_dest;  /* [n_elem*P] or NULL */
 } tm_pattern_view;
 
 /* Node -> (element, local index) incidence, ascending element id per node. */
@@ -153,6 +161,14 @@ typedef struct tm_rows_view {
   const int32_t* blk_ptr;    /* [n_blocks + 1] */
   const int32_t* blk_info;   /* [n_blocks + 1][4] {blk_ptr[b], inc_ptr[blk_ptr[b]], row_ptr[blk_ptr[b]], 0},
                                 or NULL (then read through blk_ptr) */
+  /* Optional contribution order (tm_rows_dest_build), both or neither: each
+   * workgroup lists its rows' contributions entry by entry, ascending
+   * element within an entry; dest gives, per incidence, the list position of
+   * its B values (natural vertex order, uint16 each, 4 per incidence) and
+   * ent_start, per entry, where its contributions begin in its workgroup's
+   * list.  With them an entry is a contiguous sum (same order, same values). */
+  const uint16_t* dest;      /* [n_elem*B][4] */
+  const uint16_t* ent_start; /* [nnz] */
 } tm_rows_view;
 
 /* Block rows of a node-pair pattern for the block-row reduction: runs of
@@ -313,6 +329,16 @@ int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc* form,
  * Replaces the same reference chain as tm_assemble_bilinear. */
 int tm_assemble_p1_rows(const tm_mesh_view* mesh, const tm_form_desc* form, const tm_rows_view* rows,
                         void* values, tm_stream_t stream);
+
+/* Contribution order of the fused P1 path (dest / ent_start of tm_rows_view)
+ * from a rows view with colpos and blk_info set: one workgroup per plan
+ * block; blocks over the kernel's staging caps (512 incidences, 64 rows,
+ * 1024 entries) are skipped (the fused kernel walks colpos for them).
+ * Device pointers; dest [n_elem*B][4] uint16 (unused lanes of triangles are
+ * 0), ent_start [nnz] uint16.  No reference counterpart: it is the
+ * ReduceProjector permutation (projector.py:95-123) arranged per workgroup. */
+int tm_rows_dest_build(int32_t n_basis, const tm_rows_view* rows, uint16_t* dest, uint16_t* ent_start,
+                       tm_stream_t stream);
 
 /* Adjoint of the P1 scalar assembly (TRI3 / TET4; laplace, diffusion, mass).
  * Given grad_values = dL/dvalues [nnz] and slot2nnz [n_elem*B*B] (the slot ->
@@ -504,6 +530,16 @@ int tm_segmented_reduce(int32_t dtype, int64_t nnz, int32_t hh, const int32_t* s
  * (projector.py:95-123 sums them).  n_slots = n_elem * n_basis^2. */
 int tm_symmetric_segments(int32_t n_basis, int64_t n_slots, const int32_t* seg_slot, int32_t* sym_slot,
                           tm_stream_t stream);
+
+/* Slot positions for contiguous contributions (tm_pattern_view.slot_dest)
+ * from a TM_FLAG_UPPER segment list of n contributions (each packed upper
+ * block listed once, as in the paired list of the upper entries):
+ * slot_dest[seg_slot[t] & INT32_MAX] = t | (seg_slot[t] & INT32_MIN).
+ * Needs elements without repeated vertices (every packed block then
+ * appears exactly once).  Device pointers.  The reference has no
+ * counterpart: it is the inverse of the ReduceProjector permutation
+ * (projector.py:95-123) for the symmetric storage of this build. */
+int tm_slot_dest_build(int64_t n, const int32_t* seg_slot, int32_t* slot_dest, tm_stream_t stream);
 
 /* Stage S with paired outputs: segment k (contributions seg_slot[seg_ptr[k]
  * .. seg_ptr[k+1]), summed left to right as tm_segmented_reduce does) is

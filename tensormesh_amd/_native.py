@@ -15,7 +15,7 @@ from typing import Optional
 import torch
 
 LIB_PATH = Path(__file__).resolve().parent / "libtmhip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # enum values (include/tensormesh_amd.h)
 TM_F32, TM_F64 = 0, 1
@@ -49,7 +49,7 @@ class PatternView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("col_idx", ctypes.c_void_p),
                 ("seg_ptr", ctypes.c_void_p), ("seg_slot", ctypes.c_void_p),
                 ("n_blocks", ctypes.c_int64), ("blk_ptr", ctypes.c_void_p),
-                ("entry", ctypes.c_void_p), ("entry_t", ctypes.c_void_p)]
+                ("entry", ctypes.c_void_p), ("entry_t", ctypes.c_void_p), ("slot_dest", ctypes.c_void_p)]
 
 
 class BlockRowsView(ctypes.Structure):
@@ -66,7 +66,8 @@ class RowsView(ctypes.Structure):
     _fields_ = [("nnz", ctypes.c_int64), ("row_ptr", ctypes.c_void_p), ("inc_ptr", ctypes.c_void_p),
                 ("inc", ctypes.c_void_p), ("colpos", ctypes.c_void_p), ("entry_mask", ctypes.c_void_p),
                 ("n_blocks", ctypes.c_int64),
-                ("blk_ptr", ctypes.c_void_p), ("blk_info", ctypes.c_void_p)]
+                ("blk_ptr", ctypes.c_void_p), ("blk_info", ctypes.c_void_p),
+                ("dest", ctypes.c_void_p), ("ent_start", ctypes.c_void_p)]
 
 
 class PcgView(ctypes.Structure):
@@ -122,6 +123,7 @@ SIGNATURES = {
     "tm_reduce_plan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p, c_i64p]),
     "tm_symmetric_segments": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p]),
+    "tm_slot_dest_build": (ctypes.c_int, [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "tm_segmented_reduce_pairs": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -133,6 +135,8 @@ SIGNATURES = {
                         + [ctypes.c_void_p] * 6),
     "tm_assemble_p1_rows": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                            ctypes.POINTER(RowsView), ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_rows_dest_build": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(RowsView), ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
     "tm_assemble_p1_adjoint": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc), ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

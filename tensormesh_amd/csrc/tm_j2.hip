@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     j2_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                    const double* __restrict__ tab, int Q, J2Consts k, int64_t n_elem, T* __restrict__ eps_p,
                    T* __restrict__ alpha, int update, int upper, double* __restrict__ e_out,
-                   T* __restrict__ rslot, T* __restrict__ tslot) {
+                   T* __restrict__ rslot, T* __restrict__ tslot, const int32_t* __restrict__ slot_dest) {
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
   static_assert(!BIG || EPW == 1, "one element per wavefront");
   __shared__ double s_tab[BIG ? 1 : MQ * (1 + B + D * B)];
@@ -276,17 +276,33 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
             acc[a * D + b] += lw * (vgi[a] * vgj[b]) + mw * (vgi[b] * vgj[a]) - cw * (vhi[a] * vhj[b]) +
                               (a == b ? mw * dot : 0.0);
       }
-      // block p of the element's pairs (packed when upper): 72 contiguous bytes at f64
-      T* out = tslot + (e * n_pairs + p) * DD;
+      // block p of the element's pairs (packed when upper): 72 contiguous bytes at
+      // f64; with slot_dest at its contribution-ordered slot, maybe transposed
+      if (slot_dest) {
+        const int32_t d = slot_dest[e * n_pairs + p];
+        T* out = tslot + (int64_t)(d & INT32_MAX) * DD;
+        if (d < 0) {
 #pragma unroll
-      for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
+          for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = 0; b < D; ++b) out[a * D + b] = (T)acc[b * D + a];
+        } else {
+#pragma unroll
+          for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
+        }
+      } else {
+        T* out = tslot + (e * n_pairs + p) * DD;
+#pragma unroll
+        for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
+      }
     }
   }
 }
 
 template <typename T, int D, int B>
 int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, const void* disp,
-              const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s) {
+              const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s,
+              const int32_t* slot_dest) {
   // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
   // per workgroup, the BIG variant, with rules of up to 32 points
   constexpr bool kBig = B * D > 32;
@@ -307,7 +323,7 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
 #define TM_J2_TAN(MQ, EPW, TAN)                                                                        \
   j2_wave_kernel<T, D, B, MQ, EPW, TAN, kBig><<<g, blk, 0, s>>>(                                       \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, k, m->n_elem, (T*)st->eps_p,       \
-      (T*)st->alpha, update, upper, e_out, (T*)rslot, (T*)tslot)
+      (T*)st->alpha, update, upper, e_out, (T*)rslot, (T*)tslot, upper ? slot_dest : nullptr)
 #define TM_J2(MQ, EPW)            \
   do {                            \
     if (tslot && !update)         \
@@ -337,18 +353,13 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
 
 template <typename T>
 int dispatch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, const void* disp,
-                const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s) {
+                const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s,
+                const int32_t* slot_dest = nullptr) {
   switch (m->cell) {
-    case TM_TRI3: return launch_j2<T, 2, 3>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_TET4: return launch_j2<T, 3, 4>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_TRI6: return launch_j2<T, 2, 6>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_TET10: return launch_j2<T, 3, 10>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_QUAD4: return launch_j2<T, 2, 4>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_QUAD9: return launch_j2<T, 2, 9>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_HEX8: return launch_j2<T, 3, 8>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_TRI10: return launch_j2<T, 2, 10>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_TET20: return launch_j2<T, 3, 20>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
-    case TM_HEX27: return launch_j2<T, 3, 27>(m, f, k, disp, st, update, e_out, rslot, tslot, s);
+#define TM_CASE(C, D, B) \
+  case C: return launch_j2<T, D, B>(m, f, k, disp, st, update, e_out, rslot, tslot, s, slot_dest);
+    TM_ALL_CELLS(TM_CASE)
+#undef TM_CASE
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
   }
 }
@@ -393,8 +404,11 @@ extern "C" int tm_assemble_j2(const tm_mesh_view* mesh, const tm_form_desc* form
   double* e_out = energy ? static_cast<double*>(energy_ws) : nullptr;
   void* rs = residual ? rslot_ws : nullptr;
   void* ts = tangent ? tslot_ws : nullptr;
-  st = mesh->dtype == TM_F64 ? dispatch_j2<double>(mesh, form, k, disp, state, 0, e_out, rs, ts, s)
-                             : dispatch_j2<float>(mesh, form, k, disp, state, 0, e_out, rs, ts, s);
+  if (tangent && pat->slot_dest && !(form->flags & TM_FLAG_UPPER))
+    return tmh::fail(TM_ERR_ARG, "slot_dest needs TM_FLAG_UPPER (symmetric element blocks)");
+  const int32_t* sd = tangent ? pat->slot_dest : nullptr;
+  st = mesh->dtype == TM_F64 ? dispatch_j2<double>(mesh, form, k, disp, state, 0, e_out, rs, ts, s, sd)
+                             : dispatch_j2<float>(mesh, form, k, disp, state, 0, e_out, rs, ts, s, sd);
   if (st) return st;
   if (energy) {
     st = tmh::sum_fixed(e_out, mesh->n_elem, e_out + mesh->n_elem, mesh->dtype, energy, stream_);

@@ -284,11 +284,13 @@ int check_mesh(const tm_mesh_view* m) {
 
 // Defined in tm_p2.hip (P2 and block forms).
 int tm_p2_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
-                           hipStream_t stream, int* hh_out);
+                           hipStream_t stream, int* hh_out, const int32_t* slot_dest);
 
-extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
-                                   tm_stream_t stream_) {
-  TM_TRY_BEGIN
+namespace {
+// element matrices into slot_ws; slot_dest (symmetric blocks only): their
+// contribution-ordered positions
+int element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws, tm_stream_t stream_,
+                     const int32_t* slot_dest) {
   hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   int st = check_mesh(mesh);
   if (st) return st;
@@ -299,7 +301,7 @@ extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc*
     return tmh::fail(TM_ERR_ARG, "TM_FLAG_UPPER: the P1 scalar kernels store every element block");
   if (!p1_scalar) {
     int hh = 0;
-    return tm_p2_element_matrices(mesh, form, slot_ws, stream, &hh);
+    return tm_p2_element_matrices(mesh, form, slot_ws, stream, &hh, slot_dest);
   }
   if (mesh->dtype == TM_F64) {
     return mesh->cell == TM_TET4 ? launch_p1<double, 3>(mesh, form, slot_ws, stream)
@@ -307,6 +309,13 @@ extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc*
   }
   return mesh->cell == TM_TET4 ? launch_p1<float, 3>(mesh, form, slot_ws, stream)
                                : launch_p1<float, 2>(mesh, form, slot_ws, stream);
+}
+}  // namespace
+
+extern "C" int tm_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
+                                   tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  return element_matrices(mesh, form, slot_ws, stream_, nullptr);
   TM_TRY_END
 }
 
@@ -315,7 +324,9 @@ extern "C" int tm_assemble_bilinear(const tm_mesh_view* mesh, const tm_form_desc
                                     tm_stream_t stream) {
   TM_TRY_BEGIN
   if (!pat) return tmh::fail(TM_ERR_ARG, "pattern is NULL");
-  int st = tm_element_matrices(mesh, form, slot_ws, stream);
+  if (pat->slot_dest && !(form && (form->flags & TM_FLAG_UPPER)))
+    return tmh::fail(TM_ERR_ARG, "slot_dest needs TM_FLAG_UPPER (symmetric element blocks)");
+  int st = element_matrices(mesh, form, slot_ws, stream, pat->slot_dest);
   if (st) return st;
   const int hh = form->form == TM_FORM_LINEAR_ELASTICITY ? tmh::cell_dim(mesh->cell) * tmh::cell_dim(mesh->cell) : 1;
   return tmh::reduce_view(mesh->dtype, pat, hh, slot_ws, values, stream);

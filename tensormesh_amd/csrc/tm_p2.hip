@@ -41,7 +41,8 @@ template <typename T, int D, int B, int H, int FORM, int MQ, int EPW>
 __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     wave_element_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                         const T* __restrict__ coeff, const double* __restrict__ tab, int Q,
-                        double lam, double mu, int64_t n_elem, int upper, T* __restrict__ slot) {
+                        double lam, double mu, int64_t n_elem, int upper, T* __restrict__ slot,
+                        const int32_t* __restrict__ slot_dest) {
   constexpr int L = kWave / EPW;                 // lanes per element
   constexpr int kEpb = kWavesPerBlock * EPW;     // elements per workgroup
   __shared__ double s_tab[MQ * (1 + B + D * B)];
@@ -166,8 +167,23 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
           for (int c = 0; c < H; ++c)
             acc[a * H + c] += (lam * (gi[a] * gj[c]) + mu * (gi[c] * gj[a]) + (a == c ? mu * dot : 0.0)) * w;
       }
+      if (slot_dest) {
+        // contribution-ordered slot, transposed where the pair is read as (j, i)
+        const int32_t d = slot_dest[e * n_pairs + p];
+        T* o = slot + (int64_t)(d & INT32_MAX) * HH;
+        if (d < 0) {
 #pragma unroll
-      for (int k = 0; k < HH; ++k) out[p * HH + k] = (T)acc[k];
+          for (int a = 0; a < H; ++a)
+#pragma unroll
+            for (int c = 0; c < H; ++c) o[a * H + c] = (T)acc[c * H + a];
+        } else {
+#pragma unroll
+          for (int k = 0; k < HH; ++k) o[k] = (T)acc[k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < HH; ++k) out[p * HH + k] = (T)acc[k];
+      }
     }
     return;
   }
@@ -188,7 +204,10 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
       }
       acc += val * s_jxw[wv][q];
     }
-    out[p] = (T)acc;
+    if (slot_dest)
+      slot[slot_dest[e * n_pairs + p] & INT32_MAX] = (T)acc;
+    else
+      out[p] = (T)acc;
   }
 }
 
@@ -226,7 +245,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem, int upper,
                       T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv,
-                      double* __restrict__ energy_e) {
+                      double* __restrict__ energy_e, const int32_t* __restrict__ slot_dest) {
   constexpr int kMaxQ = MQ;
   constexpr int kRowVals = B * D * D;  // values of one slot row
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
@@ -467,11 +486,19 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     for (int p = lane; p < n_pairs; p += L) {
       const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       T* out;
+      bool trans = false;
       if constexpr (!kStage) {
         // element order: block p of the element's pairs (packed when upper);
-        // incidence order (full blocks only): slot row inc_inv[e*B + i]
-        out = inc_inv ? tslot + (int64_t)inc_inv[e * B + i] * kRowVals + j * DD
-                      : tslot + (e * n_pairs + p) * DD;
+        // incidence order (full blocks only): slot row inc_inv[e*B + i];
+        // contribution order (upper): slot slot_dest[e*P + p], maybe transposed
+        if (slot_dest) {
+          const int32_t d = slot_dest[e * n_pairs + p];
+          out = tslot + (int64_t)(d & INT32_MAX) * DD;
+          trans = d < 0;
+        } else {
+          out = inc_inv ? tslot + (int64_t)inc_inv[e * B + i] * kRowVals + j * DD
+                        : tslot + (e * n_pairs + p) * DD;
+        }
       } else {
         out = &s_out[wv][(i * B + j) * DD];
       }
@@ -503,8 +530,15 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
           for (int b = 0; b < D; ++b)
             acc[a * D + b] += lw * (vhi[a] * vhj[b]) - cw * (vhi[b] * vhj[a]) + (a == b ? mw : 0.0);
       }
+      if (trans) {
 #pragma unroll
-      for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int b = 0; b < D; ++b) out[a * D + b] = (T)acc[b * D + a];
+      } else {
+#pragma unroll
+        for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
+      }
     }
   }
   if constexpr (kStage) {
@@ -529,7 +563,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
 
 template <typename T, int D, int B>
 int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
-                 const int32_t* inc_inv, double* energy_e, hipStream_t stream) {
+                 const int32_t* inc_inv, double* energy_e, const int32_t* slot_dest, hipStream_t stream) {
   // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
   // per workgroup, the BIG variant
   constexpr bool kBig = B * D > 32;
@@ -557,7 +591,7 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
 #define TM_HYPER(MF, MQ, EPW)                                                                       \
   hyper_wave_kernel<T, D, B, MF, MQ, kBig, EPW><<<g, blk, 0, stream>>>(                             \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \
-      (T*)tslot, (T*)rslot, inc_inv, energy_e)
+      (T*)tslot, (T*)rslot, inc_inv, energy_e, upper ? slot_dest : nullptr)
   if constexpr (kBig) {
     if (Q <= 16) TM_HYPER(false, 16, 1); else TM_HYPER(false, 32, 1);
   } else if (mfma) {
@@ -578,10 +612,10 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
 
 template <typename T>
 int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
-                   const int32_t* inc_inv, double* energy_e, hipStream_t s) {
+                   const int32_t* inc_inv, double* energy_e, hipStream_t s, const int32_t* slot_dest = nullptr) {
   switch (m->cell) {
 #define TM_CASE(C, D, B) \
-  case C: return launch_hyper<T, D, B>(m, f, disp, tslot, rslot, inc_inv, energy_e, s);
+  case C: return launch_hyper<T, D, B>(m, f, disp, tslot, rslot, inc_inv, energy_e, slot_dest, s);
     TM_ALL_CELLS(TM_CASE)
 #undef TM_CASE
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
@@ -589,7 +623,8 @@ int dispatch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* dis
 }
 
 template <typename T, int D, int B, int MQ>
-int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream, int* hh_out) {
+int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream, int* hh_out,
+                const int32_t* slot_dest) {
   const int Q = f->n_quad;
   const T* pts = static_cast<const T*>(m->points);
   const T* coeff = static_cast<const T*>(f->coeff);
@@ -608,7 +643,8 @@ int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hip
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
 #define TM_WAVE(H, FORM)                                                                                       \
   wave_element_kernel<T, D, B, H, FORM, MQ, kEpw><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, \
-                                                                              lam, mu, m->n_elem, upper, slot)
+                                                                              lam, mu, m->n_elem, upper, slot,  \
+                                                                              upper ? slot_dest : nullptr)
   switch (f->form) {
     case TM_FORM_LAPLACE:
       TM_WAVE(1, TM_FORM_LAPLACE);
@@ -644,21 +680,22 @@ int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hip
 
 template <typename T, int D, int B>
 int launch_scalar_or_block(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream,
-                           int* hh_out) {
+                           int* hh_out, const int32_t* sd) {
   const int Q = f->n_quad;
   if (Q <= 0 || Q > kMaxQWave) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
-  if (Q <= 4) return launch_wave<T, D, B, 4>(m, f, slot_ws, stream, hh_out);
-  if (Q <= 8) return launch_wave<T, D, B, 8>(m, f, slot_ws, stream, hh_out);
-  if (Q <= 16) return launch_wave<T, D, B, 16>(m, f, slot_ws, stream, hh_out);
-  return launch_wave<T, D, B, kMaxQWave>(m, f, slot_ws, stream, hh_out);
+  if (Q <= 4) return launch_wave<T, D, B, 4>(m, f, slot_ws, stream, hh_out, sd);
+  if (Q <= 8) return launch_wave<T, D, B, 8>(m, f, slot_ws, stream, hh_out, sd);
+  if (Q <= 16) return launch_wave<T, D, B, 16>(m, f, slot_ws, stream, hh_out, sd);
+  return launch_wave<T, D, B, kMaxQWave>(m, f, slot_ws, stream, hh_out, sd);
 }
 
 template <typename T>
-int dispatch_cell(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream, int* hh) {
+int dispatch_cell(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hipStream_t stream, int* hh,
+                  const int32_t* sd) {
   switch (m->cell) {
 #define TM_CASE(C, D, B) \
-  case C: return launch_scalar_or_block<T, D, B>(m, f, slot_ws, stream, hh);
+  case C: return launch_scalar_or_block<T, D, B>(m, f, slot_ws, stream, hh, sd);
     TM_ALL_CELLS(TM_CASE)
 #undef TM_CASE
     default: return tmh::fail(TM_ERR_ARG, "unknown cell");
@@ -674,9 +711,9 @@ int tm_hyper_element_rows(const tm_mesh_view* mesh, const tm_form_desc* form, co
 }
 
 int tm_p2_element_matrices(const tm_mesh_view* mesh, const tm_form_desc* form, void* slot_ws,
-                           hipStream_t stream, int* hh_out) {
-  if (mesh->dtype == TM_F64) return dispatch_cell<double>(mesh, form, slot_ws, stream, hh_out);
-  return dispatch_cell<float>(mesh, form, slot_ws, stream, hh_out);
+                           hipStream_t stream, int* hh_out, const int32_t* slot_dest) {
+  if (mesh->dtype == TM_F64) return dispatch_cell<double>(mesh, form, slot_ws, stream, hh_out, slot_dest);
+  return dispatch_cell<float>(mesh, form, slot_ws, stream, hh_out, slot_dest);
 }
 
 extern "C" int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_desc* form, const void* disp,
@@ -692,8 +729,11 @@ extern "C" int tm_assemble_neohookean(const tm_mesh_view* mesh, const tm_form_de
   if (residual && (!inc || !rslot_ws)) return tmh::fail(TM_ERR_ARG, "residual needs incidence and a workspace");
   void* ts = tangent ? tslot_ws : nullptr;
   void* rs = residual ? rslot_ws : nullptr;
-  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, nullptr, nullptr, stream)
-                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, nullptr, nullptr, stream);
+  if (tangent && pat->slot_dest && !(form->flags & TM_FLAG_UPPER))
+    return tmh::fail(TM_ERR_ARG, "slot_dest needs TM_FLAG_UPPER (symmetric element blocks)");
+  const int32_t* sd = tangent ? pat->slot_dest : nullptr;
+  int st = mesh->dtype == TM_F64 ? dispatch_hyper<double>(mesh, form, disp, ts, rs, nullptr, nullptr, stream, sd)
+                                 : dispatch_hyper<float>(mesh, form, disp, ts, rs, nullptr, nullptr, stream, sd);
   if (st) return st;
   if (tangent) {
     st = tmh::reduce_view(mesh->dtype, pat, (int32_t)(D * D), tslot_ws, tangent, stream_);

@@ -79,6 +79,7 @@ __device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
   return x < r ? x * (q + 1) + y : r * (q + 1) + (x - r) * q + y;
 }
 
+// seg_slot == NULL: contiguous contributions (contribution t is slot t)
 template <typename T, int HH>
 __global__ void __launch_bounds__(kBlock) reduce_walk_kernel(const int32_t* __restrict__ seg_ptr,
                                                              const int32_t* __restrict__ seg_slot,
@@ -92,7 +93,7 @@ __global__ void __launch_bounds__(kBlock) reduce_walk_kernel(const int32_t* __re
 #pragma unroll
   for (int h = 0; h < HH; ++h) acc[h] = T(0);
   for (int32_t t = seg_ptr[k]; t < seg_ptr[k + 1]; ++t) {
-    const int32_t raw = seg_slot[t];
+    const int32_t raw = seg_slot ? seg_slot[t] : t;
 #pragma unroll
     for (int h = 0; h < HH; ++h) acc[h] += slot[slot_value<HH>(raw, h)];
   }
@@ -120,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
 #pragma unroll
       for (int h = 0; h < HH; ++h) acc[h] = T(0);
       for (int32_t t = seg_ptr[k]; t < seg_ptr[k + 1]; ++t) {
-        const int32_t raw = seg_slot[t];
+        const int32_t raw = seg_slot ? seg_slot[t] : t;
 #pragma unroll
         for (int h = 0; h < HH; ++h) acc[h] += slot[slot_value<HH>(raw, h)];
       }
@@ -128,7 +129,20 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
     return;
   }
-  if constexpr (HH == 1) {
+  if (!seg_slot) {
+    // contiguous contributions: the run's slots are one contiguous range
+    const T* src = slot + (int64_t)cb * HH;
+    const int32_t nv = n * HH;
+    int32_t u = threadIdx.x;
+    for (; u + 3 * kBlock < nv; u += 4 * kBlock) {
+      const T v0 = src[u], v1 = src[u + kBlock], v2 = src[u + 2 * kBlock], v3 = src[u + 3 * kBlock];
+      lds[u] = v0;
+      lds[u + kBlock] = v1;
+      lds[u + 2 * kBlock] = v2;
+      lds[u + 3 * kBlock] = v3;
+    }
+    for (; u < nv; u += kBlock) lds[u] = src[u];
+  } else if constexpr (HH == 1) {
     int32_t t = threadIdx.x;
     // four independent gathers per iteration keep the memory pipeline full
     for (; t + 3 * kBlock < n; t += 4 * kBlock) {
@@ -220,13 +234,15 @@ int launch(int64_t nnz, int hh, const int32_t* seg_ptr, const int32_t* seg_slot,
 #undef TM_HH
 }
 
+// contiguous: contribution t is slot t (seg_slot is not read and may be NULL)
 int reduce_checked(int32_t dtype, int64_t n_seg, int32_t hh, const int32_t* seg_ptr, const int32_t* seg_slot,
                    const int32_t* entry, const int32_t* entry_t, int64_t n_blocks, const int32_t* blk_ptr,
-                   const void* slot, void* out, hipStream_t stream) {
+                   const void* slot, void* out, hipStream_t stream, bool contiguous = false) {
   if (n_seg < 0 || n_seg >= tmh::kMaxIndex || n_blocks < 0 || n_blocks > n_seg)
     return tmh::fail(TM_ERR_SIZE, "nnz / plan size out of range");
   if (n_seg == 0) return TM_OK;
-  if (!seg_ptr || !seg_slot || !slot || !out) return tmh::fail(TM_ERR_ARG, "NULL pointer");
+  if (contiguous) seg_slot = nullptr;
+  if (!seg_ptr || (!seg_slot && !contiguous) || !slot || !out) return tmh::fail(TM_ERR_ARG, "NULL pointer");
   if ((entry == nullptr) != (entry_t == nullptr)) return tmh::fail(TM_ERR_ARG, "entry and entry_t go together");
   if (dtype == TM_F64)
     return launch<double>(n_seg, hh, seg_ptr, seg_slot, entry, entry_t, n_blocks, blk_ptr, slot, out, stream);
@@ -295,5 +311,31 @@ extern "C" int tm_segmented_reduce_pairs(int32_t dtype, int64_t n_seg, int32_t h
 int tmh::reduce_view(int32_t dtype, const tm_pattern_view* pat, int32_t hh, const void* slot, void* out,
                      tm_stream_t stream) {
   return reduce_checked(dtype, pat->nnz, hh, pat->seg_ptr, pat->seg_slot, pat->entry, pat->entry_t, pat->n_blocks,
-                        pat->blk_ptr, slot, out, reinterpret_cast<hipStream_t>(stream));
+                        pat->blk_ptr, slot, out, reinterpret_cast<hipStream_t>(stream), pat->slot_dest != nullptr);
+}
+
+namespace {
+__global__ void __launch_bounds__(kBlock) slot_dest_kernel(const int32_t* __restrict__ seg_slot, int64_t n,
+                                                           int32_t* __restrict__ dest) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int32_t raw = seg_slot[t];
+  dest[raw & INT32_MAX] = (int32_t)t | (raw & kTransposed);
+}
+}  // namespace
+
+extern "C" int tm_slot_dest_build(int64_t n, const int32_t* seg_slot, int32_t* slot_dest, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  if (n < 0 || n >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "n out of range");
+  if (n == 0) return TM_OK;
+  if (!seg_slot || !slot_dest) return tmh::fail(TM_ERR_ARG, "NULL pointer");
+  const int64_t grid = (n + kBlock - 1) / kBlock;
+  slot_dest_kernel<<<grid, kBlock, 0, reinterpret_cast<hipStream_t>(stream_)>>>(seg_slot, n, slot_dest);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("slot destinations launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
+  TM_TRY_END
 }

@@ -390,6 +390,24 @@ __device__ __forceinline__ uint32_t rotate_pos(uint32_t cp, int i) {
   return r;
 }
 
+// Destination word of incidence (e, i) rotated like the staged values:
+// field p (16 bits) of the result is field (i + p) mod B of w.
+template <int B>
+__device__ __forceinline__ uint64_t rotate_dest(uint64_t w, int i) {
+  if constexpr (B == 4) {
+    const int sh = 16 * i;
+    return i == 0 ? w : (w >> sh) | (w << (64 - sh));
+  } else {
+    uint64_t r = 0;
+#pragma unroll
+    for (int p = 0; p < B; ++p) {
+      const int src = i + p < B ? i + p : i + p - B;
+      r |= ((w >> (16 * src)) & 0xffffull) << (16 * p);
+    }
+    return r;
+  }
+}
+
 // Entry -> local row map of the staged rows (s_rp must be visible).
 __device__ __forceinline__ void build_entry_rows(const int32_t* s_rp, int32_t n_rows, uint16_t* s_erow) {
   for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock)
@@ -412,7 +430,8 @@ __global__ void __launch_bounds__(kBlock)
                    RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
                    const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
                    const uint32_t* __restrict__ entry_mask, const int32_t* __restrict__ blk_ptr,
-                   const int32_t* __restrict__ blk_info, int64_t n_blocks, T* __restrict__ values) {
+                   const int32_t* __restrict__ blk_info, int64_t n_blocks, const uint64_t* __restrict__ dest,
+                   const uint16_t* __restrict__ ent_start, T* __restrict__ values) {
   constexpr int B = D + 1;
   static_assert(kCapInc == 2 * kBlock, "phase 1 stages two incidences per thread");
   __shared__ T s_val[kCapInc * B];
@@ -425,10 +444,95 @@ __global__ void __launch_bounds__(kBlock)
   const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
   const int32_t r0 = bb.r0, r1 = bb.r1, t0 = bb.t0, k0 = bb.k0, k1 = bb.k1;
   const int32_t n_inc = bb.t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
+  constexpr bool kRot = FORM == TM_FORM_LAPLACE || FORM == TM_FORM_DIFFUSION_ELEM;
+  if (dest && n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
+    // contribution-ordered staging: s_val is the workgroup's contribution
+    // list (entry by entry, ascending element within an entry); phase 1
+    // writes each local-row value at its list position, phase 2 sums every
+    // entry's contiguous range
+    T* s_c = s_val;
+    if (kRot && (int32_t)threadIdx.x < n_rows) {
+      double xr[D];
+      load_node<T, D>(pts, r0 + (int32_t)threadIdx.x, xr);
+#pragma unroll
+      for (int j = 0; j < D; ++j) s_xr[threadIdx.x * D + j] = xr[j];
+    }
+    const int32_t ta = threadIdx.x, tb = threadIdx.x + kBlock;
+    const bool ha = ta < n_inc, hb = tb < n_inc;
+    const int32_t ea = ha ? inc[t0 + ta] : 0;
+    const int32_t eb = hb ? inc[t0 + tb] : 0;
+    const uint64_t wa = ha ? dest[t0 + ta] : 0ull;
+    const uint64_t wb = hb ? dest[t0 + tb] : 0ull;
+    // the thread's entry ranges, loaded ahead of phase 1
+    constexpr int kEpt = kMaxEnt / kBlock;
+    int32_t cs[kEpt], ce[kEpt];
+#pragma unroll
+    for (int u = 0; u < kEpt; ++u) {
+      const int32_t kl = (int32_t)threadIdx.x + u * kBlock;
+      cs[u] = kl < n_ent ? (int32_t)ent_start[k0 + kl] : 0;
+      ce[u] = kl + 1 < n_ent ? (int32_t)ent_start[k0 + kl + 1] : n_inc * B;
+    }
+    const int ia = ea % B, ib = eb % B;
+    if constexpr (kRot) {
+      RotData<D> da, db;
+      if (ha) load_rot<T, D, FORM>(conn, coeff, ea / B, ia, da);
+      if (hb) load_rot<T, D, FORM>(conn, coeff, eb / B, ib, db);
+      if (ha) gather_rot<T, D>(pts, da);
+      if (hb) gather_rot<T, D>(pts, db);
+      __syncthreads();  // s_xr
+      if (ha) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) da.x[0][j] = s_xr[(da.c[0] - r0) * D + j];
+        double Kr[B];
+        rot_row<D, FORM>(da, c, Kr);
+        const uint64_t w = rotate_dest<B>(wa, ia);
+#pragma unroll
+        for (int p = 0; p < B; ++p) s_c[(w >> (16 * p)) & 0xffffu] = (T)Kr[p];
+      }
+      if (hb) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) db.x[0][j] = s_xr[(db.c[0] - r0) * D + j];
+        double Kr[B];
+        rot_row<D, FORM>(db, c, Kr);
+        const uint64_t w = rotate_dest<B>(wb, ib);
+#pragma unroll
+        for (int p = 0; p < B; ++p) s_c[(w >> (16 * p)) & 0xffffu] = (T)Kr[p];
+      }
+    } else {
+      ElemData<D> da, db;
+      if (ha) load_conn<D>(conn, ea / B, da);
+      if (hb) load_conn<D>(conn, eb / B, db);
+      if (ha) load_geometry<T, D, FORM>(pts, coeff, ea / B, da);
+      if (hb) load_geometry<T, D, FORM>(pts, coeff, eb / B, db);
+      if (ha) {
+        double Kr[B];
+        row_from<D, FORM>(da, c, ia, Kr);
+#pragma unroll
+        for (int j = 0; j < B; ++j) s_c[(wa >> (16 * j)) & 0xffffu] = (T)Kr[j];
+      }
+      if (hb) {
+        double Kr[B];
+        row_from<D, FORM>(db, c, ib, Kr);
+#pragma unroll
+        for (int j = 0; j < B; ++j) s_c[(wb >> (16 * j)) & 0xffffu] = (T)Kr[j];
+      }
+    }
+    __syncthreads();
+    // phase 2: entry kl sums its contributions in list order (ascending element)
+#pragma unroll
+    for (int u = 0; u < kEpt; ++u) {
+      const int32_t kl = (int32_t)threadIdx.x + u * kBlock;
+      if (kl < n_ent) {
+        T acc = T(0);
+        for (int32_t t = cs[u]; t < ce[u]; ++t) acc += s_c[t];
+        values[k0 + kl] = acc;
+      }
+    }
+    return;
+  }
   if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
     // loads that depend only on the block bounds go out first
     stage_row_maps(row_ptr, inc_ptr, bb, s_rp, s_ip);
-    constexpr bool kRot = FORM == TM_FORM_LAPLACE || FORM == TM_FORM_DIFFUSION_ELEM;
     if (kRot && (int32_t)threadIdx.x < n_rows) {
       double xr[D];
       load_node<T, D>(pts, r0 + (int32_t)threadIdx.x, xr);
@@ -619,9 +723,11 @@ int launch(const tm_mesh_view* m, const tm_form_desc* f, const tm_rows_view* rv,
   T* out = static_cast<T*>(values);
   const int64_t nb = rv->n_blocks;
   if (nb == 0) return TM_OK;
+  const uint64_t* dest = (rv->dest && rv->ent_start) ? reinterpret_cast<const uint64_t*>(rv->dest) : nullptr;
 #define TM_ROWS_LAUNCH(FORM)                                                                              \
   p1_rows_kernel<T, D, FORM><<<nb, kBlock, 0, s>>>(m->conn, pts, coeff, c, rv->row_ptr, rv->inc_ptr, rv->inc, \
-                                                   rv->colpos, rv->entry_mask, rv->blk_ptr, rv->blk_info, nb, out)
+                                                   rv->colpos, rv->entry_mask, rv->blk_ptr, rv->blk_info, nb,     \
+                                                   dest, rv->ent_start, out)
   switch (f->form) {
     case TM_FORM_LAPLACE: TM_ROWS_LAUNCH(TM_FORM_LAPLACE); break;
     case TM_FORM_DIFFUSION_ELEM:
@@ -700,5 +806,86 @@ extern "C" int tm_reduce_p1_rows(int32_t dtype, int32_t cell, const tm_rows_view
   TM_TRY_BEGIN
   return tm_rows_reduce_launch(dtype, cell, rows, slot, inc_order != 0, values,
                                reinterpret_cast<hipStream_t>(stream_));
+  TM_TRY_END
+}
+
+namespace {
+
+// One workgroup per plan block: every row's incidences, in order, number
+// their contributions per entry (row-local ranks), the workgroup scans the
+// per-entry counts into entry starts, and each incidence's ranks become list
+// positions.  Blocks over the staging caps are left alone.
+__global__ void __launch_bounds__(kBlock)
+    rows_dest_kernel(int B, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
+                     const uint32_t* __restrict__ colpos, const int32_t* __restrict__ blk_ptr,
+                     const int32_t* __restrict__ blk_info, int64_t n_blocks, uint16_t* __restrict__ dest,
+                     uint16_t* __restrict__ ent_start) {
+  __shared__ uint16_t s_cnt[kMaxEnt];
+  __shared__ uint16_t s_start[kMaxEnt];
+  const int64_t blk = blockIdx.x;
+  const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
+  const int32_t n_inc = bb.t1 - bb.t0, n_rows = bb.r1 - bb.r0, n_ent = bb.k1 - bb.k0;
+  if (!(n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt)) return;  // uniform per workgroup
+  for (int32_t k = threadIdx.x; k < n_ent; k += kBlock) s_cnt[k] = 0;
+  __syncthreads();
+  // ranks within each entry, one thread per row (rows own disjoint entries)
+  for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock) {
+    const int32_t r = bb.r0 + lr;
+    const int32_t kr = row_ptr[r] - bb.k0;
+    for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+      const uint32_t cp = colpos[t];
+      for (int j = 0; j < 4; ++j) {
+        uint16_t rank = 0;
+        if (j < B) {
+          const int32_t kl = kr + (int32_t)((cp >> (8 * j)) & 0xffu);
+          rank = s_cnt[kl];
+          s_cnt[kl] = rank + 1;
+        }
+        dest[(int64_t)t * 4 + j] = rank;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int32_t k = 0; k < n_ent; ++k) {
+      s_start[k] = (uint16_t)run;
+      run += s_cnt[k];
+    }
+  }
+  __syncthreads();
+  for (int32_t k = threadIdx.x; k < n_ent; k += kBlock) ent_start[bb.k0 + k] = s_start[k];
+  for (int32_t lr = threadIdx.x; lr < n_rows; lr += kBlock) {
+    const int32_t r = bb.r0 + lr;
+    const int32_t kr = row_ptr[r] - bb.k0;
+    for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+      const uint32_t cp = colpos[t];
+      for (int j = 0; j < B; ++j) {
+        const int32_t kl = kr + (int32_t)((cp >> (8 * j)) & 0xffu);
+        dest[(int64_t)t * 4 + j] = (uint16_t)(dest[(int64_t)t * 4 + j] + s_start[kl]);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int tm_rows_dest_build(int32_t n_basis, const tm_rows_view* rows, uint16_t* dest, uint16_t* ent_start,
+                                  tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  if (!rows) return tmh::fail(TM_ERR_ARG, "NULL rows view");
+  if (n_basis != 3 && n_basis != 4) return tmh::fail(TM_ERR_UNSUPPORTED, "contribution order is for P1 cells");
+  const int64_t nb = rows->n_blocks;
+  if (nb == 0) return TM_OK;
+  if (!rows->row_ptr || !rows->inc_ptr || !rows->colpos || !dest || !ent_start || (!rows->blk_ptr && !rows->blk_info))
+    return tmh::fail(TM_ERR_ARG, "NULL buffer");
+  rows_dest_kernel<<<nb, kBlock, 0, reinterpret_cast<hipStream_t>(stream_)>>>(
+      n_basis, rows->row_ptr, rows->inc_ptr, rows->colpos, rows->blk_ptr, rows->blk_info, nb, dest, ent_start);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) {
+    tmh::set_error("contribution order launch: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  return TM_OK;
   TM_TRY_END
 }

@@ -70,6 +70,62 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 constexpr int kPer = 8;
 constexpr int kChunk = kPer * kBlock;
 
+// The loads of rows_spmv that do not depend on x, for the first chunk:
+// issued early (before a kernel's scalar reductions) so that their latency
+// overlaps; spmv_finish completes the product.
+template <typename T>
+struct SpmvHead {
+  int32_t k0, k1, ra, rb;
+  T a[kPer];
+  int32_t c[kPer];
+};
+
+template <typename T>
+__device__ __forceinline__ void spmv_begin(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                           const T* __restrict__ av, int32_t row0, int32_t row1, SpmvHead<T>& h) {
+  const int32_t i = row0 + threadIdx.x;
+  h.k0 = rp[row0];
+  h.k1 = rp[row1];
+  h.ra = i < row1 ? rp[i] : 0;
+  h.rb = i < row1 ? rp[i + 1] : 0;
+  const int32_t n = h.k1 - h.k0 < kChunk ? h.k1 - h.k0 : kChunk;
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int32_t e = threadIdx.x + j * kBlock;
+    h.a[j] = e < n ? av[h.k0 + e] : T(0);
+    h.c[j] = e < n ? ci[h.k0 + e] : 0;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T spmv_finish(const int32_t* __restrict__ ci, const T* __restrict__ av,
+                                         const T* __restrict__ x, SpmvHead<T>& h, T* s_prod) {
+  T acc = T(0);
+  bool first = true;
+  for (int32_t c0 = h.k0; c0 < h.k1; c0 += kChunk) {
+    const int32_t n = h.k1 - c0 < kChunk ? h.k1 - c0 : kChunk;
+    if (!first) {
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        const int32_t e = threadIdx.x + j * kBlock;
+        h.a[j] = e < n ? av[c0 + e] : T(0);
+        h.c[j] = e < n ? ci[c0 + e] : 0;
+      }
+    }
+    first = false;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int32_t e = threadIdx.x + j * kBlock;
+      if (e < n) s_prod[e] = h.a[j] * x[h.c[j]];
+    }
+    __syncthreads();
+    const int32_t lo = h.ra > c0 ? h.ra : c0, hi = h.rb < c0 + n ? h.rb : c0 + n;
+    for (int32_t e = lo; e < hi; ++e) acc += s_prod[e - c0];
+    __syncthreads();
+  }
+  return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ T rows_spmv(const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                        const T* __restrict__ av, const T* __restrict__ x, int32_t row0,
@@ -368,6 +424,10 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t sy = blk_sys[blk];
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
   double* c = sc + sy * kNS2;
+  // the matrix loads go out before the |r|^2 reduction
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  SpmvHead<T> head;
+  spmv_begin<T>(rp, ci, av, row0, row1, head);
   double rr, bb;
   if (first) {
     system_sum2(part + 2 * nb, part + 4 * nb, b0, b1, sh, rr, bb);
@@ -383,8 +443,7 @@ __global__ void __launch_bounds__(kBlock)
   }
   if (!active) return;  // uniform per workgroup
   __shared__ T s_prod[kChunk];
-  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
-  const T wv = rows_spmv<T>(rp, ci, av, u, row0, row1, s_prod);
+  const T wv = spmv_finish<T>(ci, av, u, head, s_prod);
   const int32_t i = row0 + threadIdx.x;
   double d = 0.0;
   if (i < row1) {
@@ -408,9 +467,22 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t blk = blockIdx.x;
   const int32_t sy = blk_sys[blk];
   double* c = sc + sy * kNS2;
-  if (c[6] == 0.0) return;  // cleared by the preceding S launch; uniform per system
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
   const int P = parity & 1, Q = P ^ 1;
+  // the row's vector entries go out with the active flag and the partials
+  const int32_t i = blk_row0[blk] + threadIdx.x;
+  const bool has = i < blk_row1[blk];
+  T u0 = T(0), p0 = T(0), w0 = T(0), s0 = T(0), x0 = T(0), r0 = T(0), d0 = T(0);
+  if (has) {
+    u0 = u[i];
+    p0 = p[i];
+    w0 = w[i];
+    s0 = sv[i];
+    x0 = x[i];
+    r0 = r[i];
+    d0 = dinv[i];
+  }
+  if (c[6] == 0.0) return;  // cleared by the preceding S launch; uniform per system
   double gamma, delta;
   system_sum2(part + Q * nb, part + 3 * nb, b0, b1, sh, gamma, delta);
   const double* prev = c + 8 + 4 * Q;
@@ -432,16 +504,15 @@ __global__ void __launch_bounds__(kBlock)
     c[7] = iter + 1.0;
   }
   const T al = (T)alpha, be = (T)beta;
-  const int32_t i = blk_row0[blk] + threadIdx.x;
   double g = 0.0, rr = 0.0;
-  if (i < blk_row1[blk]) {
-    const T pi = u[i] + be * p[i];
-    const T si = w[i] + be * sv[i];
+  if (has) {
+    const T pi = u0 + be * p0;
+    const T si = w0 + be * s0;
     p[i] = pi;
     sv[i] = si;
-    x[i] += al * pi;
-    const T ri = r[i] - al * si;
-    const T ui = dinv[i] * ri;
+    x[i] = x0 + al * pi;
+    const T ri = r0 - al * si;
+    const T ui = d0 * ri;
     r[i] = ri;
     u[i] = ui;
     g = (double)ri * (double)ui;

@@ -132,15 +132,34 @@ class Pattern:
             self._inc_inv = inv
         return self._inc_inv
 
-    def rows_view(self, conn: torch.Tensor) -> Optional[nat.RowsView]:
+    def rows_view(self, conn: torch.Tensor, order: bool = True) -> Optional[nat.RowsView]:
+        """ctypes view of the fused P1 path, or None when it cannot serve this
+        pattern.  ``order``: with the contribution order (``rows_order``) on
+        a device pattern, so every entry is a contiguous sum in the kernel."""
         cp = self.colpos(conn)
         if cp is None:
             return None
         bp = self.plan(self.ROWS_CAP, node=True, max_rows=self.ROWS_MAX)
         info = self.block_info(self.ROWS_CAP)
-        return nat.RowsView(self.nnz, self.row_ptr.data_ptr(), self.inc_ptr.data_ptr(), self.inc.data_ptr(),
-                            cp.data_ptr(), self._entry_mask.data_ptr(), bp.numel() - 1, bp.data_ptr(),
-                            info.data_ptr())
+        rv = nat.RowsView(self.nnz, self.row_ptr.data_ptr(), self.inc_ptr.data_ptr(), self.inc.data_ptr(),
+                          cp.data_ptr(), self._entry_mask.data_ptr(), bp.numel() - 1, bp.data_ptr(),
+                          info.data_ptr(), None, None)
+        if order and self.row_ptr.device.type == "cuda":
+            dest, ent_start = self.rows_order(rv)
+            rv.dest, rv.ent_start = dest.data_ptr(), ent_start.data_ptr()
+        return rv
+
+    def rows_order(self, rv: nat.RowsView):
+        """(dest [E*B, 4] int16, ent_start [nnz] int16) of tm_rows_dest_build for
+        the fused path's plan, built once on the pattern's device."""
+        if getattr(self, "_rows_order", None) is None:
+            dev = self.row_ptr.device
+            dest = torch.zeros(self.inc.numel() * 4, dtype=torch.int16, device=dev)
+            ent_start = torch.zeros(max(self.nnz, 1), dtype=torch.int16, device=dev)
+            nat.check(nat.lib().tm_rows_dest_build(self.n_basis, ctypes.byref(rv), dest.data_ptr(),
+                                                   ent_start.data_ptr(), nat.stream_of(dev)), "tm_rows_dest_build")
+            self._rows_order = (dest, ent_start)
+        return self._rows_order
 
     def block_info(self, cap: int) -> torch.Tensor:
         """[n_blocks + 1, 4] int32: first row, first incidence and first CSR
@@ -289,10 +308,37 @@ class Pattern:
             self._sym_slot = None   # only the paired list is read from here on
         return self._sym_pairs
 
+    #: symmetric element blocks are written in contribution order (slot_dest)
+    #: so that the paired reduction streams contiguous slots; False keeps the
+    #: element-ordered packed blocks and gathers them through the segment list
+    contiguous_slots: bool = True
+
+    def slot_dest(self) -> Optional[torch.Tensor]:
+        """Contribution-ordered slot of every packed upper block (int32 [E*P],
+        bit 31 = stored transposed), the inverse of the paired segment list;
+        None when an element repeats a vertex.  Every packed block is listed
+        at least once among the upper entries' contributions, so a list of
+        exactly E*P contributions lists each once (a repeated vertex lists
+        some twice, on the diagonal).  Built once on the pattern's device."""
+        if getattr(self, "_slot_dest", None) is None:
+            _, _, _, seg_slot = self.sym_pairs()
+            B = self.n_basis
+            n = self.n_elem * (B * (B + 1) // 2)
+            if seg_slot.numel() != n:
+                self._slot_dest = False
+            else:
+                dest = torch.empty(n, dtype=torch.int32, device=seg_slot.device)
+                nat.check(nat.lib().tm_slot_dest_build(n, seg_slot.data_ptr(), dest.data_ptr(),
+                                                       nat.stream_of(seg_slot.device)), "tm_slot_dest_build")
+                self._slot_dest = dest
+        return self._slot_dest if self._slot_dest is not False else None
+
     def view(self, dtype: torch.dtype = torch.float64, hh: int = 1, symmetric: bool = False) -> nat.PatternView:
         """ctypes view for the reductions; ``symmetric`` selects the paired
         upper-entry list of ``sym_pairs`` (element kernels launched with
-        TM_FLAG_UPPER): each upper entry is summed once and mirrored."""
+        TM_FLAG_UPPER): each upper entry is summed once and mirrored, and
+        (``contiguous_slots``) its contributions are consecutive slots that
+        the element kernels write through ``slot_dest``."""
         if not symmetric:
             bp = self.plan(nat.reduce_cap(dtype, hh))
             return nat.PatternView(self.nnz, self.row_ptr.data_ptr(), self.col_idx.data_ptr(),
@@ -302,6 +348,7 @@ class Pattern:
         if cap not in self._sym_plans:
             self._sym_plans[cap] = torch.from_numpy(nat.reduce_plan(seg_ptr.cpu().numpy(), cap)).to(seg_ptr.device)
         bp = self._sym_plans[cap]
+        sd = self.slot_dest() if self.contiguous_slots else None
         return nat.PatternView(entry.numel(), self.row_ptr.data_ptr(), self.col_idx.data_ptr(), seg_ptr.data_ptr(),
                                seg_slot.data_ptr(), bp.numel() - 1, bp.data_ptr(), entry.data_ptr(),
-                               entry_t.data_ptr())
+                               entry_t.data_ptr(), None if sd is None else sd.data_ptr())

@@ -165,3 +165,37 @@ def test_symmetric_flag_is_rejected_where_blocks_are_stored_whole():
     st = L.tm_assemble_neohookean(ctypes.byref(mv), ctypes.byref(form), u.data_ptr(), ctypes.byref(pv), None,
                                   tslot.data_ptr(), tan.data_ptr(), None, None, nat.stream_of(torch.device(DEV)))
     assert st == -1 and "TM_FLAG_UPPER" in L.tm_last_error().decode()
+
+
+@pytest.mark.parametrize("etype", ["tetra10", "hexahedron", "triangle6", "tetra20", "tetra"])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_contribution_ordered_slots_equal_gathered_slots(etype, dtype):
+    """Symmetric blocks written in contribution order (slot_dest, contiguous
+    segment sums) against the same blocks gathered through the segment list:
+    bit-identical linear-elasticity, Neo-Hookean and J2 tangents."""
+    from tensormesh_amd.pattern import Pattern
+    say(f"case contiguous slots {etype} {dtype}")
+    m = _mesh(etype, seed=3)
+    md = tm.Mesh(m.points.to(dtype), m.cells, m.boundary_mask).to(DEV)
+    D = md.points.shape[1]
+    P = md.points
+    u = 0.03 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, -1], 0.5 * P[:, 0] ** 2][:D], 1)
+    try:
+        res = {}
+        for contiguous in (True, False):
+            Pattern.contiguous_slots = contiguous
+            L = tm.LinearElasticityElementAssembler.from_mesh(md, quadrature_order=2, E=1.0, nu=0.3)
+            L.symmetric_slots = True
+            nh = tm.NeoHookeanModel.from_mesh(md, quadrature_order=2, E=1.0, nu=0.3)
+            nh.symmetric_slots = True
+            j2 = tm.J2Plasticity.from_mesh(md, quadrature_order=2, E=200.0, nu=0.3, sig0=0.25, H=1.0)
+            j2.symmetric_slots = True
+            res[contiguous] = (L().values, nh.tangent(u), j2.tangent(0.1 * u))
+            if contiguous:
+                assert L._pattern_on(torch.device(DEV)).slot_dest() is not None
+    finally:
+        Pattern.contiguous_slots = True
+    for a, b, what in zip(res[True], res[False], ("elasticity", "neo-hookean", "j2")):
+        a = a.values if hasattr(a, "values") else a
+        b = b.values if hasattr(b, "values") else b
+        _same(a, b, what)

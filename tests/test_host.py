@@ -2,6 +2,7 @@
 form recognition, the generic torch form path and the API containers."""
 import ctypes
 import math
+import os
 import re
 from pathlib import Path
 
@@ -617,6 +618,25 @@ def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
             out[entry_t[k]] = acc[tr]
     assert not np.isnan(out).any(), "every entry is written"
     assert np.array_equal(out, full)
+    # contribution order (slot_dest, tm_slot_dest_build restated): the element
+    # kernels write packed block s at slot dest[s] (transposed when flagged);
+    # every segment then sums consecutive slots -- the same values in the same order
+    assert ss.size == E * P
+    dest = np.empty(E * P, np.int64)
+    dest[ss & 0x7FFFFFFF] = np.arange(ss.size)
+    flip = ss < 0
+    contig = np.empty((E * P, H * H))
+    contig[dest] = packed_slot
+    contig[dest[ss[flip] & 0x7FFFFFFF]] = packed_slot[ss[flip] & 0x7FFFFFFF][:, tr]
+    out2 = np.full((pat.nnz, H * H), np.nan)
+    for k in range(entry.size):
+        acc = np.zeros(H * H)
+        for t in range(sp[k], sp[k + 1]):
+            acc += contig[t]
+        out2[entry[k]] = acc
+        if entry_t[k] != entry[k]:
+            out2[entry_t[k]] = acc[tr]
+    assert np.array_equal(out2, full)
     rows = np.repeat(np.arange(m.n_points), np.diff(pat.row_ptr.numpy()))
     cols = pat.col_idx.numpy()
     assert (rows[entry] <= cols[entry]).all()
@@ -650,3 +670,29 @@ def test_symmetric_slots_default_policy():
     assert A._symmetric_blocks(True) is False
     A.symmetric_slots = True
     assert A._symmetric_blocks(False) is True
+
+
+def test_host_pattern_builder_under_address_and_ub_sanitizers(tmp_path):
+    """The host pattern builder (tm_pattern.cpp, tm_common.cpp) built with
+    AddressSanitizer and UndefinedBehaviorSanitizer (g++, host code only) and
+    driven over small meshes, a repeated vertex, an empty mesh, out-of-range
+    node ids, negative sizes and an over-long row (tests/native)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ is not available")
+    exe = tmp_path / "pattern_asan"
+    src = ROOT / "tensormesh_amd" / "csrc"
+    cmd = [gxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fno-omit-frame-pointer", f"-I{ROOT / 'include'}", f"-I{src}",
+           str(ROOT / "tests" / "native" / "pattern_driver.cpp"), str(src / "tm_pattern.cpp"),
+           str(src / "tm_common.cpp"), "-o", str(exe), "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    # verify_asan_link_order=0: the process environment may preload other
+    # libraries ahead of the sanitizer runtime
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.stdout + r.stderr)[-3000:]

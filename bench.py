@@ -353,6 +353,36 @@ def grad_step_bench(n=512, steps=2, rtol=1e-10):
            "speedup_vs_restated_torch_solver": res["plain"]["total_ms"] / res["ours"]["total_ms"],
            "solve_share_ours": (res["ours"]["solve_ms"] + res["ours"]["backward_ms"]) / res["ours"]["total_ms"],
            "gradient_agreement_max_rel": agree}
+    with torch.no_grad():
+        Ki, fi = cond(A(element_data={"kappa": kap}), b)
+    out["pcg_us_per_iteration"] = pcg_iteration_bench(Ki, fi)
+    return out
+
+
+def pcg_iteration_bench(Ki, fi, iters=512, chunk=64):
+    """Device time per Jacobi-PCG iteration on one condensed system, per
+    variant (launches per iteration 1 / 2 / 5), from HIP events around graph
+    replays of ``chunk`` iterations with a tolerance no iterate reaches."""
+    from tensormesh_amd.solve import BatchedPCG
+    out = {"rows": int(Ki.shape[0]), "nnz": int(Ki.nnz)}
+    saved = BatchedPCG.launches
+    try:
+        for launches in (2, 1, 5):
+            BatchedPCG.launches = launches
+            s = BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-300)
+            s.solve(fi, max_iter=chunk, chunk=chunk)      # captures the graph
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            s.residual(fi, torch.zeros_like(fi))   # restart from x = 0, parity 0
+            g = s._graphs[chunk]
+            ev[0].record()
+            for _ in range(iters // chunk):
+                g.replay()
+            ev[1].record()
+            torch.cuda.synchronize()
+            out[f"launches_{launches}"] = ev[0].elapsed_time(ev[1]) * 1e3 / iters
+    finally:
+        BatchedPCG.launches = saved
     return out
 
 
@@ -419,12 +449,15 @@ def hyperelastic_bench(n=32, reps=20):
 
 
 def neohookean_energy_bench(m, model, reps=20):
-    """Neo-Hookean energy(u).backward() at u = 0.1 x, float64: the HIP energy
+    """Neo-Hookean energy(u).backward() at u = (0.1 x, 0.01 sin(pi x) sin(pi z), 0),
+    float64: the HIP energy
     with its residual backward against the reference algorithm (vmapped torch
     energy + autograd) on the same GPU; device time via HIP events."""
     pts = m.points
     u0 = torch.zeros_like(pts)
     u0[:, 0] = 0.1 * pts[:, 0]
+    # a non-uniform part, so that elements differ
+    u0[:, 1] = 0.01 * torch.sin(np.pi * pts[:, 0]) * torch.sin(np.pi * pts[:, 2])
 
     def run(reference):
         model.reference_path = reference

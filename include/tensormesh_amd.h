@@ -97,7 +97,11 @@ typedef struct tm_form_desc {
   double young;              /* LINEAR_ELASTICITY: E  */
   double poisson;            /* LINEAR_ELASTICITY: nu */
   int32_t flags;             /* TM_FLAG_* */
-  int32_t reserved;
+  int32_t epw;               /* elements per wavefront of the wave kernels, 0 = the kernel's own choice:
+                                element matrices 1 or 2, Neo-Hookean 1, 2 or 4 (VALU tangent), J2 1, 4
+                                or 8 (cells of <= 4 nodes with rules of <= 4 points); a count the cell,
+                                rule or LDS budget does not admit is TM_ERR_ARG.  Results do not depend
+                                on it (one lane group per element, the same arithmetic). */
 } tm_form_desc;
 
 /* tm_form_desc.flags */
@@ -224,9 +228,12 @@ typedef struct tm_pcg_view {
   const int32_t* blk_row1;   /* [n_blocks] */
   const int32_t* sys_blk;    /* [n_sys + 1] */
   double rtol;               /* stop system s once |r_s| <= rtol |b_s| */
-  void* s;                   /* [n_rows] A p (two-launch variant only) */
-  int32_t two_launch;        /* 1: two launches per iteration (partials [5 n_blocks], scalars [16 n_sys]) */
-  int32_t parity;            /* two-launch: number of iterations run since tm_pcg_init, mod 2 */
+  void* s;                   /* [n_rows] A p (two- and one-launch variants) */
+  int32_t two_launch;        /* launches per iteration: 0 five (partials [3 n_blocks], scalars [8 n_sys]),
+                                1 two (partials [5 n_blocks], scalars [16 n_sys]),
+                                2 one (partials [7 n_blocks], scalars [16 n_sys], aux) */
+  int32_t parity;            /* two- and one-launch: iterations run since tm_pcg_init, mod 2 */
+  void* aux;                 /* one-launch: [2 n_rows], the second copies of A u and A p */
 } tm_pcg_view;
 
 int tm_abi_version(void);
@@ -607,8 +614,11 @@ int tm_assemble_facet_linear(const tm_mesh_view* mesh, const tm_facet_view* face
  *                   recurrences; every workgroup re-sums its system's
  *                   partials in workgroup order, so a step costs
  *                   O(workgroups per system) extra loads per workgroup:
- *                   meant for systems of up to ~2048 workgroups); otherwise
- *                   five.  Host-visible scalars per system s (stride 8 for
+ *                   meant for systems of up to ~2048 workgroups); with
+ *                   two_launch = 2 one launch per step (the same
+ *                   recurrences: each workgroup forms the next u = M r of
+ *                   its SpMV columns itself from the previous step's r,
+ *                   A u and A p, double-buffered by parity); otherwise five.  Host-visible scalars per system s (stride 8 for
  *                   five launches, 16 for two): [4] |r|^2, [5] |b|^2,
  *                   [6] active, [7] iterations.
  * ------------------------------------------------------------------- */

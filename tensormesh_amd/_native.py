@@ -38,7 +38,7 @@ class FormDesc(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("n_quad", ctypes.c_int32),
                 ("tab_host", ctypes.c_void_p), ("tab_dev", ctypes.c_void_p),
                 ("coeff", ctypes.c_void_p), ("young", ctypes.c_double),
-                ("poisson", ctypes.c_double), ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("poisson", ctypes.c_double), ("flags", ctypes.c_int32), ("epw", ctypes.c_int32)]
 
 
 TM_FLAG_MFMA = 1
@@ -77,7 +77,7 @@ class PcgView(ctypes.Structure):
                                                  "partials", "scalars", "blk_sys", "blk_row0", "blk_row1",
                                                  "sys_blk")]
     _fields_ += [("rtol", ctypes.c_double), ("s", ctypes.c_void_p), ("two_launch", ctypes.c_int32),
-                 ("parity", ctypes.c_int32)]
+                 ("parity", ctypes.c_int32), ("aux", ctypes.c_void_p)]
 
 
 class J2Material(ctypes.Structure):

@@ -158,6 +158,33 @@ class ElementAssembler(_AssemblerBase):
     #: path (torch ops on the same device) -- the "plain" side of a
     #: measurement, never chosen implicitly.
     reference_path: bool = False
+    #: Elements per wavefront of the wave kernels (element matrices of P2 and
+    #: block forms, Neo-Hookean, J2): None = each kernel's own choice, an int =
+    #: that count (the kernel rejects one the cell, rule or LDS budget does not
+    #: admit), "auto" = the fastest admitted count, timed once per device,
+    #: kernel, cell, form, dtype and rule and cached (tensormesh_amd.tune).
+    #: The results do not depend on it.
+    epw: Union[int, str, None] = None
+
+    def _launch_epw(self, kernel: str, dev, dtype, tag: str, run: Callable[[int], object], options) -> int:
+        """tm_form_desc.epw for a launch: 0 (the kernel's own choice), the
+        fixed ``epw``, or the tuned fastest of ``options`` (``run(e)`` makes the
+        call with e elements per wavefront)."""
+        if self.epw is None:
+            return 0
+        if self.epw != "auto":
+            return int(self.epw)
+        from .tune import device_tag, tuner
+        key = f"{device_tag(dev)}|epw|{kernel}|{self.etype}|{tag}|{dtype}|q{self.quadrature_order}"
+
+        def fixed(e):
+            saved = self.epw
+            self.epw = e
+            try:
+                return run(e)
+            finally:
+                self.epw = saved
+        return int(tuner().choose(key, {str(e): (lambda e=e: fixed(e)) for e in options}, device=dev))
 
     def __init__(self, elements: Dict[str, torch.Tensor], points: torch.Tensor, pattern: Pattern,
                  quadrature_order: int = 2, *args, **kwargs):
@@ -304,6 +331,9 @@ class ElementAssembler(_AssemblerBase):
         sym = not p1_scalar and self._symmetric_blocks(H > 1)
         if sym:
             form.flags |= nat.TM_FLAG_UPPER
+        if not p1_scalar:
+            form.epw = self._launch_epw("wave", dev, dtype, f"{match.form}|{int(sym)}",
+                                        lambda e: self._native_values(match, points, coeff), (1, 2))
         pairs = B * (B + 1) // 2 if sym else B * B      # packed upper blocks, or all
         slot = torch.empty(E * pairs * H * H, dtype=dtype, device=dev)
         values = torch.empty(pat.nnz * H * H, dtype=dtype, device=dev)
@@ -399,6 +429,10 @@ class ElementAssembler(_AssemblerBase):
         sym = want_tangent and not (form.flags & nat.TM_FLAG_MFMA) and self._symmetric_blocks(True)
         if sym:
             form.flags |= nat.TM_FLAG_UPPER
+        form.epw = self._launch_epw("neohookean", dev, dtype, f"{int(form.flags)}|{int(want_residual)}|"
+                                    f"{int(want_tangent)}",
+                                    lambda e: self._native_hyperelastic(u, points, young, poisson, want_residual,
+                                                                        want_tangent, "slots"), (1, 2, 4))
         pv = pat.view(dtype, D * D, symmetric=sym)
         node_plan = pat.plan(nat.reduce_cap(dtype, D), node=True)
         iv = nat.IncidenceView(pat.inc_ptr.data_ptr(), pat.inc.data_ptr(), node_plan.numel() - 1, node_plan.data_ptr())
@@ -697,7 +731,7 @@ class NodeAssembler(_AssemblerBase):
         return cache.get(fn, layout, lambda: match_linear(fn, point_data, scalar_data))
 
     def _linear_args(self, points: torch.Tensor, f: Optional[torch.Tensor], const: float):
-        """ctypes arguments of tm_assemble_linear(_noalloc) for a device call:
+        """ctypes arguments of tm_assemble_linear for a device call:
         (mesh view, form descriptor, constant or None, incidence view, the
         tensors the views point into).  ``points`` and ``f`` must be
         contiguous device tensors of one dtype."""

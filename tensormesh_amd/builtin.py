@@ -124,6 +124,8 @@ class NeoHookeanModel(ElementAssembler):
         mesh = self._mesh_view(points)
         form = nat.FormDesc(0, _n_quad(self.etype, self.quadrature_order), tab.ctypes.data, tab_dev.data_ptr(),
                             None, float(self.E), float(self.nu), 0)
+        form.epw = self._launch_epw("neohookean_energy", dev, dtype, str(int(want_residual)),
+                                    lambda e: self._native_energy(u, want_residual, points), (1, 2, 4))
         ews = torch.empty(E + nat.J2_ENERGY_PARTS, dtype=torch.float64, device=dev)
         energy = torch.empty((), dtype=dtype, device=dev)
         res = rslot = None
@@ -349,6 +351,11 @@ class J2Plasticity(ElementAssembler):
         sym = want_tangent and self._symmetric_blocks(True)
         form = nat.FormDesc(0, nq, tab.ctypes.data, tab_dev.data_ptr(), None, 0.0, 0.0,
                             nat.TM_FLAG_UPPER if sym else 0)
+        form.epw = self._launch_epw("j2", dev, dtype, f"{int(sym)}|{int(want_energy)}|{int(want_residual)}|"
+                                    f"{int(want_tangent)}",
+                                    lambda e: self._native_j2(u, eps_p, alpha, points, want_energy=want_energy,
+                                                              want_residual=want_residual,
+                                                              want_tangent=want_tangent), (1, 4, 8))
         st = nat.J2State(eps_p.data_ptr(), alpha.data_ptr())
         pv = pat.view(dtype, D * D, symmetric=sym)
         node_plan = pat.plan(nat.reduce_cap(dtype, D), node=True)

@@ -303,38 +303,49 @@ template <typename T, int D, int B>
 int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, const void* disp,
               const tm_j2_state* st, int update, double* e_out, void* rslot, void* tslot, hipStream_t s,
               const int32_t* slot_dest) {
-  // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
-  // per workgroup, the BIG variant, with rules of up to 32 points
-  constexpr bool kBig = B * D > 32;
-  constexpr int kWpb = kBig ? 1 : kWavesPerBlock;
+  // more than 32 element dofs (tetra20: 60, hexahedron27: 81), or a rule of
+  // more than 16 points (the 24-point tetrahedron rule): one element per
+  // workgroup, the BIG variant, with rules of up to 32 points
+  constexpr bool kBigCell = B * D > 32;
   const int Q = f->n_quad;
-  if (Q <= 0 || Q > (kBig ? 32 : 16)) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
+  const bool big = kBigCell || Q > 16;
+  const int wpb = big ? 1 : kWavesPerBlock;
+  if (Q <= 0 || Q > 32) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   // four small elements (B <= 4) per wavefront when the rule has at most 4 points
   // small cells (B <= 4, rules of <= 4 points): eight elements per wavefront
   // without a tangent, four with one (its pair loop wants the lanes)
   const bool tan = tslot && !update;
-  const int epw = (B <= 4 && Q <= 4) ? (tan ? 4 : 8) : 1;
-  const int64_t per_block = (int64_t)kWpb * epw;
+  int epw = (B <= 4 && Q <= 4) ? (tan ? 4 : 8) : 1;
+  if (f->epw != 0) {
+    // form->epw: 1 everywhere; 4 or 8 for cells of <= 4 nodes with rules of <= 4 points
+    if (!(f->epw == 1 || ((f->epw == 4 || f->epw == 8) && B <= 4 && Q <= 4)))
+      return tmh::fail(TM_ERR_ARG, "elements per wavefront not admitted for this cell / rule");
+    epw = f->epw;
+  }
+  const int64_t per_block = (int64_t)wpb * epw;
   const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
-  const dim3 g((unsigned)grid), blk(kWave * kWpb);
+  const dim3 g((unsigned)grid), blk(kWave * wpb);
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
-#define TM_J2_TAN(MQ, EPW, TAN)                                                                        \
-  j2_wave_kernel<T, D, B, MQ, EPW, TAN, kBig><<<g, blk, 0, s>>>(                                       \
+#define TM_J2_TAN(MQ, EPW, TAN, BG)                                                                    \
+  j2_wave_kernel<T, D, B, MQ, EPW, TAN, BG><<<g, blk, 0, s>>>(                                         \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, k, m->n_elem, (T*)st->eps_p,       \
       (T*)st->alpha, update, upper, e_out, (T*)rslot, (T*)tslot, upper ? slot_dest : nullptr)
-#define TM_J2(MQ, EPW)            \
-  do {                            \
-    if (tslot && !update)         \
-      TM_J2_TAN(MQ, EPW, true);   \
-    else                          \
-      TM_J2_TAN(MQ, EPW, false);  \
+#define TM_J2_V(MQ, EPW, BG)          \
+  do {                                \
+    if (tslot && !update)             \
+      TM_J2_TAN(MQ, EPW, true, BG);   \
+    else                              \
+      TM_J2_TAN(MQ, EPW, false, BG);  \
   } while (0)
-  if constexpr (kBig) {
-    if (Q <= 16) TM_J2(16, 1); else TM_J2(32, 1);
+#define TM_J2(MQ, EPW) TM_J2_V(MQ, EPW, false)
+  if constexpr (kBigCell) {
+    if (Q <= 16) TM_J2_V(16, 1, true); else TM_J2_V(32, 1, true);
+  } else if (big) {
+    TM_J2_V(32, 1, true);
   } else if constexpr (B <= 4) {
-    if (epw == 8) TM_J2_TAN(4, 8, false);
+    if (epw == 8) TM_J2(4, 8);
     else if (epw == 4) TM_J2(4, 4);
     else if (Q <= 4) TM_J2(4, 1);
     else TM_J2(16, 1);
@@ -342,6 +353,7 @@ int launch_j2(const tm_mesh_view* m, const tm_form_desc* f, const J2Consts& k, c
     if (Q <= 4) TM_J2(4, 1); else TM_J2(16, 1);
   }
 #undef TM_J2
+#undef TM_J2_V
 #undef TM_J2_TAN
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {

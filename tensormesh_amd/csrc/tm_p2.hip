@@ -564,44 +564,60 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
 template <typename T, int D, int B>
 int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp, void* tslot, void* rslot,
                  const int32_t* inc_inv, double* energy_e, const int32_t* slot_dest, hipStream_t stream) {
-  // more than 32 element dofs (tetra20: 60, hexahedron27: 81): one element
-  // per workgroup, the BIG variant
-  constexpr bool kBig = B * D > 32;
-  constexpr int kWpb = kBig ? 1 : kWavesPerBlock;
+  // more than 32 element dofs (tetra20: 60, hexahedron27: 81), or a rule of
+  // more than 16 points (the 24-point tetrahedron rule): one element per
+  // workgroup, the BIG variant, with rules of up to 32 points
+  constexpr bool kBigCell = B * D > 32;
   const int Q = f->n_quad;
-  if (Q <= 0 || Q > (kBig ? kMaxQWave : kMaxQ))
-    return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
+  const bool big = kBigCell || Q > kMaxQ;
+  const int wpb = big ? 1 : kWavesPerBlock;
+  if (Q <= 0 || Q > kMaxQWave) return tmh::fail(TM_ERR_UNSUPPORTED, "quadrature rule has too many points");
   if (!f->tab_dev) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   const double mu = f->young / (2.0 * (1.0 + f->poisson));
   const double lam = f->young * f->poisson / ((1.0 + f->poisson) * (1.0 - 2.0 * f->poisson));
   const bool mfma = (f->flags & TM_FLAG_MFMA) != 0;
   if (mfma && !(std::is_same<T, float>::value && B * D <= 32))
     return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path is float32 only");
+  if (mfma && Q > kMaxQ) return tmh::fail(TM_ERR_UNSUPPORTED, "the MFMA tangent path takes rules of <= 16 points");
   if (mfma && (Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
-  // two elements per wavefront (32 lanes each) on the VALU path: the
-  // per-quadrature-point stages, which use Q lanes, then serve two elements
-  const int epw = (mfma || kBig) ? 1 : 2;
-  const int64_t per_block = (int64_t)kWpb * epw;
+  // two elements per wavefront (32 lanes each) on the VALU path by default:
+  // the per-quadrature-point stages, which use Q lanes, then serve two
+  // elements; form->epw asks for 1, 2 or 4 (4: 16 lanes, rules of <= 4 points)
+  int epw = (mfma || big) ? 1 : 2;
+  if (f->epw != 0) {
+    const bool ok = f->epw == 1 || (!mfma && !big && (f->epw == 2 || (f->epw == 4 && Q <= 4 && B <= 16)));
+    if (!ok) return tmh::fail(TM_ERR_ARG, "elements per wavefront not admitted for this cell / rule / tangent");
+    epw = f->epw;
+  }
+  const int64_t per_block = (int64_t)wpb * epw;
   const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
-  const dim3 g((unsigned)grid), blk(kWave * kWpb);
+  const dim3 g((unsigned)grid), blk(kWave * wpb);
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
   if (upper && (mfma || inc_inv))
     return tmh::fail(TM_ERR_ARG, "TM_FLAG_UPPER needs the element-ordered VALU tangent (no MFMA, no block rows)");
-#define TM_HYPER(MF, MQ, EPW)                                                                       \
-  hyper_wave_kernel<T, D, B, MF, MQ, kBig, EPW><<<g, blk, 0, stream>>>(                             \
+#define TM_HYPER_V(MF, MQ, BG, EPW)                                                                 \
+  hyper_wave_kernel<T, D, B, MF, MQ, BG, EPW><<<g, blk, 0, stream>>>(                               \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \
       (T*)tslot, (T*)rslot, inc_inv, energy_e, upper ? slot_dest : nullptr)
-  if constexpr (kBig) {
-    if (Q <= 16) TM_HYPER(false, 16, 1); else TM_HYPER(false, 32, 1);
+#define TM_HYPER(MF, MQ, EPW) TM_HYPER_V(MF, MQ, false, EPW)
+  if constexpr (kBigCell) {
+    if (Q <= 16) TM_HYPER_V(false, 16, true, 1); else TM_HYPER_V(false, 32, true, 1);
+  } else if (big) {
+    TM_HYPER_V(false, 32, true, 1);
   } else if (mfma) {
     if constexpr (std::is_same<T, float>::value) {  // float64 was rejected above
       if (Q <= 4) TM_HYPER(true, 4, 1); else TM_HYPER(true, 16, 1);
     }
+  } else if (Q <= 4) {
+    if (epw == 1) TM_HYPER(false, 4, 1);
+    else if (epw == 2) TM_HYPER(false, 4, 2);
+    else if constexpr (B <= 16) TM_HYPER(false, 4, 4);
   } else {
-    if (Q <= 4) TM_HYPER(false, 4, 2); else TM_HYPER(false, 16, 2);
+    if (epw == 1) TM_HYPER(false, 16, 1); else TM_HYPER(false, 16, 2);
   }
 #undef TM_HYPER
+#undef TM_HYPER_V
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     tmh::set_error("neo-hookean kernel launch: %s", hipGetErrorString(err));
@@ -634,17 +650,25 @@ int launch_wave(const tm_mesh_view* m, const tm_form_desc* f, void* slot_ws, hip
   // serves two elements
   constexpr int kElemBytes = 8 * (B * D + MQ * (D * D + 2 + B * D)) + 4 * B;
   constexpr int kEpw = (2 * kWavesPerBlock * kElemBytes <= 40960 && MQ <= kWave / 2 && B <= kWave / 2) ? 2 : 1;
-  const int64_t per_block = (int64_t)kWavesPerBlock * kEpw;
+  // form->epw: 1, or 2 where two elements' arrays fit (0: the largest that fits)
+  const int epw = f->epw == 0 ? kEpw : f->epw;
+  if (epw != 1 && epw != kEpw) return tmh::fail(TM_ERR_ARG, "elements per wavefront not admitted for this cell / rule");
+  const int64_t per_block = (int64_t)kWavesPerBlock * epw;
   const int64_t grid = (m->n_elem + per_block - 1) / per_block;
   if (grid == 0) return TM_OK;
   const dim3 block(kWave * kWavesPerBlock);
   double lam = 0.0, mu = 0.0;
   *hh_out = 1;
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
-#define TM_WAVE(H, FORM)                                                                                       \
-  wave_element_kernel<T, D, B, H, FORM, MQ, kEpw><<<grid, block, 0, stream>>>(m->conn, pts, coeff, f->tab_dev, Q, \
-                                                                              lam, mu, m->n_elem, upper, slot,  \
-                                                                              upper ? slot_dest : nullptr)
+#define TM_WAVE(H, FORM)                                                                                          \
+  do {                                                                                                            \
+    if (epw == kEpw)                                                                                              \
+      wave_element_kernel<T, D, B, H, FORM, MQ, kEpw><<<grid, block, 0, stream>>>(                               \
+          m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, upper, slot, upper ? slot_dest : nullptr);   \
+    else                                                                                                          \
+      wave_element_kernel<T, D, B, H, FORM, MQ, 1><<<grid, block, 0, stream>>>(                                  \
+          m->conn, pts, coeff, f->tab_dev, Q, lam, mu, m->n_elem, upper, slot, upper ? slot_dest : nullptr);   \
+  } while (0)
   switch (f->form) {
     case TM_FORM_LAPLACE:
       TM_WAVE(1, TM_FORM_LAPLACE);

@@ -22,6 +22,12 @@
 
 #include "tm_internal.h"
 
+// No implicit contraction of a * b + c in this file: a rounding step should
+// not depend on how the compiler schedules an expression, so that the two-
+// and one-launch iterations, which form the same values in different
+// kernels, agree bit for bit.  Fused operations are written out (fma_t).
+#pragma clang fp contract(off)
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -319,6 +325,19 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
+// a * b + c rounded once.  The two- and one-launch iterations write their
+// vector updates with it so that no update depends on which products the
+// compiler contracts: the one-launch step forms u = M r in registers where the
+// two-launch one loads it, and a contraction of that product into p = u + beta p
+// would round differently.
+template <typename T>
+__device__ __forceinline__ T fma_t(T a, T b, T c) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_fma(a, b, c);
+  else
+    return __builtin_fmaf(a, b, c);
+}
+
 // ---------------------------------------------------------------------------
 // Two-launch PCG iteration (Chronopoulos-Gear recurrences, u = M r, w = A u,
 // s = A p):
@@ -506,12 +525,12 @@ __global__ void __launch_bounds__(kBlock)
   const T al = (T)alpha, be = (T)beta;
   double g = 0.0, rr = 0.0;
   if (has) {
-    const T pi = u0 + be * p0;
-    const T si = w0 + be * s0;
+    const T pi = fma_t(be, p0, u0);
+    const T si = fma_t(be, s0, w0);
     p[i] = pi;
     sv[i] = si;
-    x[i] = x0 + al * pi;
-    const T ri = r0 - al * si;
+    x[i] = fma_t(al, pi, x0);
+    const T ri = fma_t(-al, si, r0);
     const T ui = d0 * ri;
     r[i] = ri;
     u[i] = ui;
@@ -527,6 +546,280 @@ __global__ void __launch_bounds__(kBlock)
   if (threadIdx.x == 0) {
     part[P * nb + blk] = g;
     part[2 * nb + blk] = rr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One-launch PCG iteration: the recurrences and the arithmetic of the
+// two-launch one, with the vector update folded into the SpMV launch.  Step k
+// (parity P, reading slot Q = P ^ 1 of the double-buffered r, A u, A p):
+//   scalars: |r_Q|^2 test, gamma = (r_Q, u_Q), delta = (A u_Q, u_Q) from the
+//            workgroup partials of step k-1, alpha / beta as the vec kernel;
+//   own rows: p = u_Q + beta p, s_P = w_Q + beta s_Q, x += alpha p,
+//            r_P = r_Q - alpha s_P, u_P = M r_P;
+//   w_P = A u_P, where the u_P of every column is formed from slot Q by the
+//            same inline arithmetic (bit-identical to the owner's);
+//   partials of gamma, delta and |r_P|^2 for step k+1.
+// Partials: part[(0|1) nb] gamma, [(2|3) nb] delta, [(4|5) nb] |r|^2, [6 nb]
+// |b|^2, by slot; scalars as the two-launch variant (the active flag, once
+// cleared, stops every later step).  Every iterate is bit-identical to the
+// two-launch one.
+// ---------------------------------------------------------------------------
+
+// next u = M (r - alpha (w + beta s)) of one row, with s and r written back
+template <typename T>
+__device__ __forceinline__ T cg1_next(T r, T w, T s, T d, T al, T be, T& s_new, T& r_new) {
+  s_new = fma_t(be, s, w);
+  r_new = fma_t(-al, s_new, r);
+  return d * r_new;
+}
+
+// r = b - A x, s = 0, p = 0 (slot 0); partials gamma_0, |r|^2, |b|^2
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg1_init_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
+                    const int32_t* __restrict__ blk_row1, const int32_t* __restrict__ sys_blk,
+                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ av,
+                    const T* __restrict__ b, const T* __restrict__ x, const T* __restrict__ dinv, T* __restrict__ r,
+                    T* __restrict__ p, T* __restrict__ sv, double* __restrict__ part, double* __restrict__ sc,
+                    int32_t nb) {
+  __shared__ double sh[3 * (kBlock / 64)];
+  __shared__ T s_prod[kChunk];
+  const int32_t blk = blockIdx.x;
+  const int32_t sy = blk_sys[blk];
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const T ax = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
+  const int32_t i = row0 + threadIdx.x;
+  double g = 0.0, bb = 0.0, rr = 0.0;
+  if (i < row1) {
+    const T ri = b[i] - ax;
+    const T ui = dinv[i] * ri;
+    r[i] = ri;
+    p[i] = T(0);
+    sv[i] = T(0);
+    g = (double)ri * (double)ui;
+    bb = (double)b[i] * (double)b[i];
+    rr = (double)ri * (double)ri;
+  }
+  double v[3] = {g, bb, rr};
+  block_sum_n<3>(v, sh);
+  if (threadIdx.x == 0) {
+    part[0 * nb + blk] = v[0];
+    part[4 * nb + blk] = v[2];
+    part[6 * nb + blk] = v[1];
+    if (blk == sys_blk[sy]) {
+      sc[sy * kNS2 + 8 + 0 + 2] = 0.0;  // iterations in slot 0 (read by step 0)
+      sc[sy * kNS2 + 7] = 0.0;
+    }
+  }
+}
+
+// w = A u of slot 0 (u = M r formed per column), partials delta_0; publishes
+// |b|^2, |r|^2 and the active flag
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg1_spmv0_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
+                     const int32_t* __restrict__ blk_row1, const int32_t* __restrict__ sys_blk,
+                     const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ av,
+                     const T* __restrict__ dinv, const T* __restrict__ r, T* __restrict__ w,
+                     double* __restrict__ part, double* __restrict__ sc, int32_t nb, double tol2) {
+  __shared__ double sh[2 * (kBlock / 64)];
+  __shared__ T s_prod[kChunk];
+  const int32_t blk = blockIdx.x;
+  const int32_t sy = blk_sys[blk];
+  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
+  double* c = sc + sy * kNS2;
+  double rr, bb;
+  system_sum2(part + 4 * nb, part + 6 * nb, b0, b1, sh, rr, bb);
+  const bool active = rr > tol2 * bb && bb > 0.0;
+  if (threadIdx.x == 0 && blk == b0) {
+    c[4] = rr;
+    c[5] = bb;
+    c[6] = active ? 1.0 : 0.0;
+  }
+  if (!active) return;  // uniform per workgroup
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const int32_t i = row0 + threadIdx.x;
+  const int32_t k0 = rp[row0], k1 = rp[row1];
+  const int32_t ra = i < row1 ? rp[i] : 0, rb = i < row1 ? rp[i + 1] : 0;
+  T acc = T(0);
+  for (int32_t c0 = k0; c0 < k1; c0 += kChunk) {
+    const int32_t n = k1 - c0 < kChunk ? k1 - c0 : kChunk;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int32_t e = threadIdx.x + j * kBlock;
+      if (e < n) {
+        const int32_t col = ci[c0 + e];
+        s_prod[e] = av[c0 + e] * (dinv[col] * r[col]);
+      }
+    }
+    __syncthreads();
+    const int32_t lo = ra > c0 ? ra : c0, hi = rb < c0 + n ? rb : c0 + n;
+    for (int32_t e = lo; e < hi; ++e) acc += s_prod[e - c0];
+    __syncthreads();
+  }
+  double d = 0.0;
+  if (i < row1) {
+    w[i] = acc;
+    d = (double)acc * (double)(dinv[i] * r[i]);
+  }
+  d = block_sum<T>(d, sh);
+  if (threadIdx.x == 0) part[2 * nb + blk] = d;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    cg1_step_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
+                    const int32_t* __restrict__ blk_row1, const int32_t* __restrict__ sys_blk,
+                    const int32_t* __restrict__ rp, const int32_t* __restrict__ ci, const T* __restrict__ av,
+                    const T* __restrict__ dinv, const T* __restrict__ rQ, const T* __restrict__ wQ,
+                    const T* __restrict__ sQ, T* __restrict__ rP, T* __restrict__ wP, T* __restrict__ sP,
+                    T* __restrict__ p, T* __restrict__ x, double* __restrict__ part, double* __restrict__ sc,
+                    int32_t nb, double tol2, int parity) {
+  __shared__ double sh[3 * (kBlock / 64)];
+  __shared__ T s_prod[kChunk];
+  __shared__ int s_run;
+  const int32_t blk = blockIdx.x;
+  const int32_t sy = blk_sys[blk];
+  const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
+  double* c = sc + sy * kNS2;
+  const int P = parity & 1, Q = P ^ 1;
+  // a system that stopped stays stopped: once it is inactive its slots are no
+  // longer written, so the parity would otherwise bring back an older |r|^2.
+  // One thread reads the flag (the system's first workgroup may be clearing it
+  // in this very step) and the workgroup follows that one read.
+  if (threadIdx.x == 0) s_run = c[6] != 0.0;
+  const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const int32_t i = row0 + threadIdx.x;
+  const bool has = i < row1;
+  // own-row state and the first SpMV chunk go out before the scalar reductions
+  T rq = T(0), wq = T(0), sq = T(0), pq = T(0), xq = T(0), dq = T(0);
+  if (has) {
+    rq = rQ[i];
+    wq = wQ[i];
+    sq = sQ[i];
+    pq = p[i];
+    xq = x[i];
+    dq = dinv[i];
+  }
+  const int32_t k0 = rp[row0], k1 = rp[row1];
+  const int32_t ra = has ? rp[i] : 0, rb = has ? rp[i + 1] : 0;
+  T a[kPer], gr[kPer], gw[kPer], gs[kPer], gd[kPer];
+  {
+    const int32_t n = k1 - k0 < kChunk ? k1 - k0 : kChunk;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int32_t e = threadIdx.x + j * kBlock;
+      a[j] = T(0);
+      gr[j] = gw[j] = gs[j] = gd[j] = T(0);
+      if (e < n) {
+        const int32_t col = ci[k0 + e];
+        a[j] = av[k0 + e];
+        gr[j] = rQ[col];
+        gw[j] = wQ[col];
+        gs[j] = sQ[col];
+        gd[j] = dinv[col];
+      }
+    }
+  }
+  __syncthreads();
+  if (!s_run) return;  // uniform per workgroup
+  double sums[3] = {0.0, 0.0, 0.0};
+  {
+    // gamma_Q, delta_Q, |r_Q|^2 over the system's workgroups, in workgroup order
+    for (int32_t wgi = b0 + threadIdx.x; wgi < b1; wgi += kBlock) {
+      sums[0] += part[Q * nb + wgi];
+      sums[1] += part[(2 + Q) * nb + wgi];
+      sums[2] += part[(4 + Q) * nb + wgi];
+    }
+    block_sum_n<3>(sums, sh);
+  }
+  const double gamma = sums[0], delta = sums[1], rr = sums[2];
+  const double bb = c[5];
+  const bool active = rr > tol2 * bb && bb > 0.0;
+  if (threadIdx.x == 0 && blk == b0) {
+    c[4] = rr;
+    c[6] = active ? 1.0 : 0.0;
+  }
+  if (!active) return;  // uniform per workgroup
+  const double* prev = c + 8 + 4 * Q;
+  const double iter = prev[2];
+  double alpha, beta;
+  if (iter == 0.0) {
+    beta = 0.0;
+    alpha = delta != 0.0 ? gamma / delta : 0.0;
+  } else {
+    beta = prev[1] != 0.0 ? gamma / prev[1] : 0.0;
+    const double den = delta - beta * gamma / prev[0];
+    alpha = den != 0.0 ? gamma / den : 0.0;
+  }
+  if (threadIdx.x == 0 && blk == b0) {
+    double* cur = c + 8 + 4 * P;
+    cur[0] = alpha;
+    cur[1] = gamma;
+    cur[2] = iter + 1.0;
+    c[7] = iter + 1.0;
+  }
+  const T al = (T)alpha, be = (T)beta;
+  double g = 0.0, r2 = 0.0;
+  T ui = T(0);
+  if (has) {
+    const T uq = dq * rq;
+    const T pi = fma_t(be, pq, uq);
+    T si, ri;
+    ui = cg1_next<T>(rq, wq, sq, dq, al, be, si, ri);
+    p[i] = pi;
+    sP[i] = si;
+    x[i] = fma_t(al, pi, xq);
+    rP[i] = ri;
+    g = (double)ri * (double)ui;
+    r2 = (double)ri * (double)ri;
+  }
+  // w_P = A u_P, rows summed in ascending entry order (CSR-stream through LDS)
+  T acc = T(0);
+  bool first = true;
+  for (int32_t c0 = k0; c0 < k1; c0 += kChunk) {
+    const int32_t n = k1 - c0 < kChunk ? k1 - c0 : kChunk;
+    if (!first) {
+#pragma unroll
+      for (int j = 0; j < kPer; ++j) {
+        const int32_t e = threadIdx.x + j * kBlock;
+        if (e < n) {
+          const int32_t col = ci[c0 + e];
+          a[j] = av[c0 + e];
+          gr[j] = rQ[col];
+          gw[j] = wQ[col];
+          gs[j] = sQ[col];
+          gd[j] = dinv[col];
+        }
+      }
+    }
+    first = false;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      const int32_t e = threadIdx.x + j * kBlock;
+      if (e < n) {
+        T sn, rn;
+        const T uc = cg1_next<T>(gr[j], gw[j], gs[j], gd[j], al, be, sn, rn);
+        s_prod[e] = a[j] * uc;
+      }
+    }
+    __syncthreads();
+    const int32_t lo = ra > c0 ? ra : c0, hi = rb < c0 + n ? rb : c0 + n;
+    for (int32_t e = lo; e < hi; ++e) acc += s_prod[e - c0];
+    __syncthreads();
+  }
+  double d = 0.0;
+  if (has) {
+    wP[i] = acc;
+    d = (double)acc * (double)ui;
+  }
+  double v[3] = {g, d, r2};
+  block_sum_n<3>(v, sh);
+  if (threadIdx.x == 0) {
+    part[P * nb + blk] = v[0];
+    part[(2 + P) * nb + blk] = v[1];
+    part[(4 + P) * nb + blk] = v[2];
   }
 }
 
@@ -621,6 +914,37 @@ int cg2_iterate(const tm_pcg_view* v, void* x, int32_t n_iter, hipStream_t s) {
   return hip_status("pcg iterate");
 }
 
+template <typename T>
+int cg1_init(const tm_pcg_view* v, const void* b, const void* x, hipStream_t s) {
+  const int32_t nb = (int32_t)v->n_blocks;
+  const double tol2 = v->rtol * v->rtol;
+  cg1_init_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, v->row_ptr, v->col_idx,
+                                           (const T*)v->values, (const T*)b, (const T*)x, (const T*)v->dinv,
+                                           (T*)v->r, (T*)v->p, (T*)v->s, v->partials, v->scalars, nb);
+  cg1_spmv0_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, v->row_ptr,
+                                            v->col_idx, (const T*)v->values, (const T*)v->dinv, (const T*)v->r,
+                                            (T*)v->q, v->partials, v->scalars, nb, tol2);
+  return hip_status("pcg init");
+}
+
+// slot 0: r, q (A u), s (A p); slot 1: z, aux[0, n), aux[n, 2n)
+template <typename T>
+int cg1_iterate(const tm_pcg_view* v, void* x, int32_t n_iter, hipStream_t s) {
+  const int32_t nb = (int32_t)v->n_blocks;
+  const double tol2 = v->rtol * v->rtol;
+  T* rb[2] = {(T*)v->r, (T*)v->z};
+  T* wb[2] = {(T*)v->q, (T*)v->aux};
+  T* sb[2] = {(T*)v->s, (T*)v->aux + v->n_rows};
+  for (int32_t it = 0; it < n_iter; ++it) {
+    const int Q = (v->parity + it) & 1, P = Q ^ 1;
+    cg1_step_kernel<T><<<nb, kBlock, 0, s>>>(v->blk_sys, v->blk_row0, v->blk_row1, v->sys_blk, v->row_ptr,
+                                             v->col_idx, (const T*)v->values, (const T*)v->dinv, rb[Q], wb[Q],
+                                             sb[Q], rb[P], wb[P], sb[P], (T*)v->p, (T*)x, v->partials, v->scalars,
+                                             nb, tol2, P);
+  }
+  return hip_status("pcg iterate");
+}
+
 int check_view(const tm_pcg_view* v) {
   if (!v) return tmh::fail(TM_ERR_ARG, "NULL view");
   if (v->n_rows < 0 || v->n_rows >= tmh::kMaxIndex || v->n_blocks <= 0 || v->n_sys <= 0)
@@ -628,6 +952,9 @@ int check_view(const tm_pcg_view* v) {
   if (!v->row_ptr || !v->col_idx || !v->values || !v->dinv || !v->r || !v->z || !v->p || !v->q || !v->partials ||
       !v->scalars || !v->blk_sys || !v->blk_row0 || !v->blk_row1 || !v->sys_blk)
     return tmh::fail(TM_ERR_ARG, "NULL buffer in pcg view");
+  if (v->two_launch < 0 || v->two_launch > 2) return tmh::fail(TM_ERR_ARG, "unknown PCG variant");
+  if (v->two_launch >= 1 && !v->s) return tmh::fail(TM_ERR_ARG, "the two- and one-launch PCG need s");
+  if (v->two_launch == 2 && !v->aux) return tmh::fail(TM_ERR_ARG, "the one-launch PCG needs aux [2 n_rows]");
   return TM_OK;
 }
 
@@ -675,9 +1002,13 @@ extern "C" int tm_pcg_init(const tm_pcg_view* view, const void* b, const void* x
   if (st) return st;
   if (!b || !x) return tmh::fail(TM_ERR_ARG, "NULL b / x");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
-  const bool two = view->two_launch != 0;
-  if (view->dtype == TM_F64) return two ? cg2_init<double>(view, b, x, s) : pcg_init<double>(view, b, x, s);
-  if (view->dtype == TM_F32) return two ? cg2_init<float>(view, b, x, s) : pcg_init<float>(view, b, x, s);
+  const int var = view->two_launch;
+  if (view->dtype == TM_F64)
+    return var == 2 ? cg1_init<double>(view, b, x, s) : var ? cg2_init<double>(view, b, x, s)
+                                                           : pcg_init<double>(view, b, x, s);
+  if (view->dtype == TM_F32)
+    return var == 2 ? cg1_init<float>(view, b, x, s) : var ? cg2_init<float>(view, b, x, s)
+                                                          : pcg_init<float>(view, b, x, s);
   return tmh::fail(TM_ERR_ARG, "unknown dtype");
   TM_TRY_END
 }
@@ -688,9 +1019,13 @@ extern "C" int tm_pcg_iterate(const tm_pcg_view* view, void* x, int32_t n_iter, 
   if (st) return st;
   if (!x || n_iter < 0) return tmh::fail(TM_ERR_ARG, "bad x / n_iter");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream_);
-  const bool two = view->two_launch != 0;
-  if (view->dtype == TM_F64) return two ? cg2_iterate<double>(view, x, n_iter, s) : pcg_iterate<double>(view, x, n_iter, s);
-  if (view->dtype == TM_F32) return two ? cg2_iterate<float>(view, x, n_iter, s) : pcg_iterate<float>(view, x, n_iter, s);
+  const int var = view->two_launch;
+  if (view->dtype == TM_F64)
+    return var == 2 ? cg1_iterate<double>(view, x, n_iter, s) : var ? cg2_iterate<double>(view, x, n_iter, s)
+                                                                   : pcg_iterate<double>(view, x, n_iter, s);
+  if (view->dtype == TM_F32)
+    return var == 2 ? cg1_iterate<float>(view, x, n_iter, s) : var ? cg2_iterate<float>(view, x, n_iter, s)
+                                                                  : pcg_iterate<float>(view, x, n_iter, s);
   return tmh::fail(TM_ERR_ARG, "unknown dtype");
   TM_TRY_END
 }

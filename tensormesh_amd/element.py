@@ -64,6 +64,14 @@ TRI_QUAD = {
          [0.636502499121399, 0.053145049844817], [0.310352451033784, 0.053145049844817],
          [0.053145049844817, 0.310352451033784], [0.053145049844817, 0.636502499121399],
          [0.310352451033784, 0.636502499121399], [0.636502499121399, 0.310352451033784]]),
+    7: ([-0.074785022233841] + [0.087807628716604, 0.026673617804419] * 3 + [0.0385568804451285] * 6,
+        [[0.333333333333333, 0.333333333333333], [0.26034596607904, 0.47930806784192],
+         [0.065130102902216, 0.869739794195568], [0.47930806784192, 0.26034596607904],
+         [0.869739794195568, 0.065130102902216], [0.26034596607904, 0.26034596607904],
+         [0.065130102902216, 0.065130102902216], [0.63844418856981, 0.048690315425316],
+         [0.312865496004874, 0.048690315425316], [0.048690315425316, 0.312865496004874],
+         [0.048690315425316, 0.63844418856981], [0.312865496004874, 0.63844418856981],
+         [0.63844418856981, 0.312865496004874]]),
 }
 
 TET_QUAD = {
@@ -115,6 +123,32 @@ TET_QUAD = {
          [0.0665501535736643, 0.4334498464263357, 0.4334498464263357],
          [0.4334498464263357, 0.0665501535736643, 0.4334498464263357],
          [0.4334498464263357, 0.4334498464263357, 0.0665501535736643]]),
+    7: ([0.0066537917096946494] * 4 + [0.0016795351758867834] * 4 + [0.009226196923942399] * 4
+        + [0.008035714285714283] * 12,
+        [[0.3561913862225449, 0.2146028712591517, 0.2146028712591517],
+         [0.2146028712591517, 0.2146028712591517, 0.2146028712591517],
+         [0.2146028712591517, 0.2146028712591517, 0.3561913862225449],
+         [0.2146028712591517, 0.3561913862225449, 0.2146028712591517],
+         [0.877978124396166, 0.0406739585346113, 0.0406739585346113],
+         [0.0406739585346113, 0.0406739585346113, 0.0406739585346113],
+         [0.0406739585346113, 0.0406739585346113, 0.877978124396166],
+         [0.0406739585346113, 0.877978124396166, 0.0406739585346113],
+         [0.0329863295731731, 0.3223378901422757, 0.3223378901422757],
+         [0.3223378901422757, 0.3223378901422757, 0.3223378901422757],
+         [0.3223378901422757, 0.3223378901422757, 0.0329863295731731],
+         [0.3223378901422757, 0.0329863295731731, 0.3223378901422757],
+         [0.2696723314583159, 0.0636610018750175, 0.0636610018750175],
+         [0.0636610018750175, 0.2696723314583159, 0.0636610018750175],
+         [0.0636610018750175, 0.0636610018750175, 0.2696723314583159],
+         [0.6030056647916491, 0.0636610018750175, 0.0636610018750175],
+         [0.0636610018750175, 0.6030056647916491, 0.0636610018750175],
+         [0.0636610018750175, 0.0636610018750175, 0.6030056647916491],
+         [0.0636610018750175, 0.2696723314583159, 0.6030056647916491],
+         [0.2696723314583159, 0.6030056647916491, 0.0636610018750175],
+         [0.6030056647916491, 0.0636610018750175, 0.2696723314583159],
+         [0.0636610018750175, 0.6030056647916491, 0.2696723314583159],
+         [0.2696723314583159, 0.0636610018750175, 0.6030056647916491],
+         [0.6030056647916491, 0.2696723314583159, 0.0636610018750175]]),
 }
 
 # n-point Gauss-Legendre on [-1, 1] (quadrature.py:36-54), mapped to [0, 1] with

@@ -64,10 +64,14 @@ def _col32(A) -> torch.Tensor:
 class BatchedPCG:
     """Jacobi-PCG over a block-diagonal stack of CSR systems on the device.
 
-    ``two_launch`` (default True) uses the two-launch Chronopoulos-Gear
-    iteration for stacks whose largest system spans at most
-    ``TWO_LAUNCH_MAX_BLOCKS`` workgroups (every workgroup re-sums its
-    system's partials); otherwise, or with False, the five-launch textbook PCG.
+    ``launches`` picks the iteration: 2 (default) the two-launch
+    Chronopoulos-Gear iteration, 1 the same iteration in one launch per step
+    (the SpMV recomputes the updated vector of every column it reads: four
+    gathers per matrix entry instead of one, measured 2.5x slower per
+    iteration on a 261k-row system) -- both for stacks whose largest system
+    spans at most ``TWO_LAUNCH_MAX_BLOCKS`` workgroups (every workgroup
+    re-sums its system's partials) and with bit-identical iterates -- or 5,
+    the textbook PCG (used as well for larger systems).
 
     Parameters
     ----------
@@ -77,7 +81,7 @@ class BatchedPCG:
     rtol : stop a system once |r| <= rtol |b|
     """
 
-    two_launch: bool = True
+    launches: int = 2
     TWO_LAUNCH_MAX_BLOCKS = 2048
 
     def __init__(self, row_ptr: torch.Tensor, col_idx: torch.Tensor, values: torch.Tensor,
@@ -102,12 +106,16 @@ class BatchedPCG:
         self.blk_sys, self.blk_row0, self.blk_row1, self.sys_blk = t(blk_sys), t(blk0), t(blk1), t(sys_blk)
         self.n_blocks = len(blk_sys)
         max_blk = max((sys_blk[i + 1] - sys_blk[i] for i in range(self.n_sys)), default=0)
-        self.two = bool(self.two_launch) and max_blk <= self.TWO_LAUNCH_MAX_BLOCKS
+        small = max_blk <= self.TWO_LAUNCH_MAX_BLOCKS
+        self.variant = {1: 2, 2: 1}.get(int(self.launches), 0) if small else 0   # tm_pcg_view.two_launch
+        self.two = self.variant != 0
         self.stride = 16 if self.two else 8
         self._parity = 0
         e = lambda: torch.empty(self.n, dtype=self.dtype, device=self.dev)
         self.dinv, self.r, self.z, self.p, self.q, self.x, self.s = e(), e(), e(), e(), e(), e(), e()
-        self.partials = torch.zeros(self.n_blocks * (5 if self.two else 3), dtype=torch.float64, device=self.dev)
+        self.aux = torch.empty(2 * self.n if self.variant == 2 else 1, dtype=self.dtype, device=self.dev)
+        n_part = {0: 3, 1: 5, 2: 7}[self.variant]
+        self.partials = torch.zeros(self.n_blocks * n_part, dtype=torch.float64, device=self.dev)
         self.scalars = torch.zeros(self.n_sys * self.stride, dtype=torch.float64, device=self.dev)
         self.rtol = rtol
         self._graphs = {}
@@ -132,7 +140,8 @@ class BatchedPCG:
             setattr(v, name, getattr(self, name).data_ptr())
         v.rtol = self.rtol
         if self.two:
-            v.s, v.two_launch, v.parity = self.s.data_ptr(), 1, self._parity
+            v.s, v.two_launch, v.parity = self.s.data_ptr(), self.variant, self._parity
+            v.aux = self.aux.data_ptr()
         return v
 
     def residual(self, b: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
@@ -226,18 +235,42 @@ def _solver_for(A, rtol: float) -> BatchedPCG:
 
 
 def cg(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, rtol: float = 1e-10, atol: float = 0.0,
-       max_iter: int = 100000, check_every: int = 25):
+       max_iter: int = 100000, check_every: int = 25, refine: int = 0):
     """Jacobi-preconditioned conjugate gradients (torch_solve.py:54-128).
 
     Returns (x, iterations, relative residual).  Device matrices in CSR order
     use the native batched PCG with one system; others run in torch.
+
+    ``refine`` (not in the reference): that many steps of iterative
+    refinement after the solve, each solving A d = b - A x to ``rtol``
+    relative to its own right-hand side and adding d.  The CG recurrence
+    residual drifts from the true one, so a solve stopped at ``rtol`` leaves
+    an error up to cond(A) * rtol; one refinement step takes it to the
+    rounding level of the residual b - A x.  The reported residual is then
+    the true |b - A x| / |b| and the iterations are the total.
     """
+    x, it, rel = _cg(A, b, x0, rtol, atol, max_iter, check_every)
+    _warn_unconverged(it, rel, rtol, atol, b)
+    if refine <= 0:
+        return x, it, rel
+    b = b.detach().to(A.values.dtype)
+    with torch.no_grad():
+        for _ in range(refine):
+            r = b - spmv(A, x)
+            d, it_d, rel_d = _cg(A, r, None, rtol, 0.0, max_iter, check_every)
+            _warn_unconverged(it_d, rel_d, rtol, 0.0, r)
+            x = x + d
+            it += it_d
+        bnorm = torch.linalg.vector_norm(b).item()
+        rel = torch.linalg.vector_norm(b - spmv(A, x)).item() / max(bnorm, 1e-300)
+    return x, it, rel
+
+
+def _cg(A, b, x0, rtol, atol, max_iter, check_every):
     if A.device.type == "cuda" and A.row_ptr is not None and b.dim() == 1:
         solver = _solver_for(A, rtol)
         x, it, rel = solver.solve(b.detach(), x0=None if x0 is None else x0.detach(), max_iter=max_iter)
-        it, rel = int(it[0].item()), float(rel[0].item())
-        _warn_unconverged(it, rel, rtol, atol, b)
-        return x, it, rel
+        return x, int(it[0].item()), float(rel[0].item())
     vals = A.values.detach()
     row, col = A.row_indices, A.col_indices
     n = A.shape[0]
@@ -271,7 +304,6 @@ def cg(A, b: torch.Tensor, x0: Optional[torch.Tensor] = None, rtol: float = 1e-1
         p = z + (rz_new / rz) * p
         rz = rz_new
     rnorm = torch.linalg.vector_norm(b - _spmv_raw(vals, row, col, n, x)).item()
-    _warn_unconverged(it, rnorm / max(bnorm, 1e-300), rtol, atol, b)
     return x, it, rnorm / max(bnorm, 1e-300)
 
 

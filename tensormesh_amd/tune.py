@@ -21,6 +21,10 @@ computation):
   paths' in the last bits (within 1e-15 relative) -- the choice is opt-in.
 * ``NeoHookeanModel.nh_path = "auto"``: slot-buffer segment reduction or
   incidence-ordered block rows; same element values, same order.
+* ``epw = "auto"`` on the element-matrix, Neo-Hookean and J2 assemblers:
+  elements per wavefront of the wave kernels (1/2, 1/2/4, 1/4/8; counts a
+  kernel does not admit for the case are skipped).  Each element keeps its
+  own lane group and arithmetic, so the results are bit-identical.
 * ``BatchedPCG.solve(chunk=None)``: iterations per graph replay between
   convergence polls, from the previous solve's iteration count on the same
   system (systems stop on the device, so the chunk changes only the number of
@@ -98,11 +102,15 @@ class Tuner:
         hit = self.get(key)
         if isinstance(hit, dict) and hit.get("choice") in candidates:
             return hit["choice"]
+        from ._native import NativeError
         dev = device or torch.device("cuda", torch.cuda.current_device())
         times = {}
         for name in names:
             fn = candidates[name]
-            fn()
+            try:
+                fn()
+            except NativeError:
+                continue   # a launch shape the kernel does not admit for this case
             torch.cuda.synchronize(dev)
             start = torch.cuda.Event(enable_timing=True)
             stop = torch.cuda.Event(enable_timing=True)
@@ -112,6 +120,8 @@ class Tuner:
             stop.record()
             torch.cuda.synchronize(dev)
             times[name] = start.elapsed_time(stop) / reps
+        if not times:
+            return names[0]
         best = min(times, key=times.get)
         self.put(key, {"choice": best, "ms": times})
         return best

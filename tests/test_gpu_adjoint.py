@@ -74,9 +74,12 @@ def test_adjoint_matches_oracle_autograd(kind, form):
 
 
 def _loss_chain(points, kappa, A, cond, b, u_target):
+    # one refinement step takes the solve from the CG stopping error
+    # (up to cond * rtol) to the rounding level of the residual, so the
+    # difference quotient sees the loss, not where the iteration stopped
     K = A(points, element_data={"kappa": kappa})
     Ki, fi = cond(K, b)
-    u = Ki.solve(fi, rtol=1e-13)
+    u = Ki.solve(fi, rtol=1e-13, refine=1)
     return ((u - u_target) ** 2).sum()
 
 
@@ -94,7 +97,7 @@ def test_gradient_step_finite_differences():
     with torch.no_grad():
         Kt = A(element_data={"kappa": kap_t})
         Kti, fti = cond(Kt, b)
-        u_target = Kti.solve(fti, rtol=1e-13)
+        u_target = Kti.solve(fti, rtol=1e-13, refine=1)
     kr = kap.clone().requires_grad_(True)
     pr = m.points.clone().requires_grad_(True)
     L = _loss_chain(pr, kr, A, cond, b, u_target)

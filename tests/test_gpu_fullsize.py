@@ -9,7 +9,8 @@
 * Adjoint gradient on the 512^2 x 2 triangle mesh (524,288 elements): the
   hand-written assembly adjoint against the oracle's autograd at full size,
   and the whole assemble -> condense -> solve -> loss gradient against central
-  finite differences (h = 1e-6) on 64 seeded directions.
+  finite differences on 64 seeded directions (h = 1e-6 to the loss's float64
+  rounding floor, the fourth-order stencil at h = 1e-4 to 1e-6).
 * Batched solves: all 256 uneven meshes in one assembly, eight of them
   against the per-mesh oracle, every system solved to 1e-8.
 """
@@ -192,7 +193,20 @@ def test_cfg3_assembly_adjoint_full_size_against_oracle_autograd():
 
 
 def test_cfg3_gradient_step_finite_differences_full_size():
-    say("case cfg3 512^2x2 gradient vs central finite differences, 64 directions, h = 1e-6")
+    """The whole assemble -> condense -> solve -> loss gradient on 512^2 x 2
+    against finite differences on 64 seeded directions (kappa and interior
+    node coordinates together).
+
+    The loss itself carries a rounding floor: cond(K) amplifies float64
+    rounding to about 2e-11 relative in L (measured with an exact sparse LU
+    of the same systems on the host: central differences at h = 1e-6 then
+    disagree with the exact gradient by up to 6e-4 per direction and 2e-5
+    normwise).  So the central differences at h = 1e-6 are held to that
+    floor (5e-5 normwise), and the fourth-order central stencil at h = 1e-4
+    (truncation O(h^4), rounding noise / 1e-4) to 1e-6 normwise.  Every
+    solve takes one refinement step, so the solver's stopping error stays
+    below the rounding floor."""
+    say("case cfg3 512^2x2 gradient vs central finite differences, 64 directions")
     m0, kap0, kapt0 = _cfg3()
     m = m0.to(DEV)
     kap, kap_t = kap0.to(DEV), kapt0.to(DEV)
@@ -202,42 +216,46 @@ def test_cfg3_gradient_step_finite_differences_full_size():
     x, y = m.points[:, 0], m.points[:, 1]
     f = 2 * np.pi ** 2 * torch.sin(np.pi * x) * torch.sin(np.pi * y)
     b = tm.SourceNodeAssembler.from_mesh(m)(point_data={"f": f})
-    rtol = 1e-13
+    kw = dict(rtol=1e-13, max_iter=20000, refine=1)
 
     def loss(points, kappa):
         K = A(points, element_data={"kappa": kappa})
         Ki, fi = cond(K, b)
-        u = Ki.solve(fi, rtol=rtol, max_iter=20000)
+        u = Ki.solve(fi, **kw)
         return ((u - u_t) ** 2).sum()
 
     with torch.no_grad():
         Kt = A(element_data={"kappa": kap_t})
         Kti, fti = cond(Kt, b)
-        u_t = Kti.solve(fti, rtol=rtol, max_iter=20000)
-        say(f"  target solve: {Kti.last_solve_info[0]} iterations, relres {Kti.last_solve_info[1]:.2e}")
+        u_t = Kti.solve(fti, **kw)
+        say(f"  target solve: {Kti.last_solve_info[0]} iterations, true relres {Kti.last_solve_info[1]:.2e}")
     kr = kap.clone().requires_grad_(True)
     pr = m.points.clone().requires_grad_(True)
     loss(pr, kr).backward()
     interior = (~m.boundary_mask).to(torch.float64)[:, None]
     g = torch.Generator().manual_seed(11)
-    h = 1e-6
-    fds, ans = [], []
+    c2, c4, ans = [], [], []
     for t in range(64):
         dk = torch.randn(E, generator=g, dtype=torch.float64).to(DEV)
         dx = torch.randn(m.points.shape, generator=g, dtype=torch.float64).to(DEV) * interior * 1e-3
         with torch.no_grad():
-            lp = loss(m.points + h * dx, kap + h * dk)
-            lm = loss(m.points - h * dx, kap - h * dk)
-        fds.append(((lp - lm) / (2 * h)).item())
+            at = lambda s: loss(m.points + s * dx, kap + s * dk).item()
+            h = 1e-6
+            c2.append((at(h) - at(-h)) / (2 * h))
+            h = 1e-4
+            c4.append((at(-2 * h) - 8 * at(-h) + 8 * at(h) - at(2 * h)) / (12 * h))
         ans.append(((kr.grad * dk).sum() + (pr.grad * dx).sum()).item())
         if t % 16 == 15:
             say(f"  {t + 1} directions done")
-    fds, ans = np.array(fds), np.array(ans)
-    rel = np.abs(fds - ans) / np.abs(fds)
-    worst = np.abs(fds - ans).max() / np.abs(fds).max()
-    say(f"  max |fd - adjoint| / max |fd| = {worst:.2e}; per-direction median {np.median(rel):.2e}, "
-        f"max {rel.max():.2e}")
-    assert worst <= 1e-6
+    c2, c4, ans = np.array(c2), np.array(c4), np.array(ans)
+    w2 = np.abs(c2 - ans).max() / np.abs(ans).max()
+    w4 = np.abs(c4 - ans).max() / np.abs(ans).max()
+    say(f"  central h=1e-6: max |fd - adjoint| / max |adjoint| = {w2:.2e} "
+        f"(per-direction median {np.median(np.abs(c2 - ans) / np.abs(ans)):.2e})")
+    say(f"  fourth-order h=1e-4: max |fd - adjoint| / max |adjoint| = {w4:.2e} "
+        f"(per-direction median {np.median(np.abs(c4 - ans) / np.abs(ans)):.2e})")
+    assert w2 <= 5e-5
+    assert w4 <= 1e-6
 
 
 # ------------------------------------------------------------ batched, cfg4

@@ -196,6 +196,6 @@ def test_contribution_ordered_slots_equal_gathered_slots(etype, dtype):
     finally:
         Pattern.contiguous_slots = True
     for a, b, what in zip(res[True], res[False], ("elasticity", "neo-hookean", "j2")):
-        a = a.values if hasattr(a, "values") else a
-        b = b.values if hasattr(b, "values") else b
+        a = a.values if isinstance(a, tm.SparseMatrix) else a
+        b = b.values if isinstance(b, tm.SparseMatrix) else b
         _same(a, b, what)

@@ -79,3 +79,55 @@ def test_adaptive_pcg_chunk_leaves_solution_unchanged():
     assert s._last_iters == int(it_a.max()) and torch.equal(it32, it_a)
     assert torch.equal(x32, x_a)
     say(f"  iterations {int(it_a.max())}, adaptive chunk {tune.pcg_chunk(int(it32.max()))}")
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_elements_per_wavefront_choices_are_bit_identical(dtype, tmp_path, monkeypatch):
+    """Every admitted elements-per-wavefront count of the element-matrix,
+    Neo-Hookean and J2 wave kernels gives the same bits; "auto" times them
+    once and keeps one; a count the kernel does not admit is rejected."""
+    import importlib
+    from tensormesh_amd import _native as nat
+    from tensormesh_amd.mesh import unit_cube_hex, unit_cube_tet
+    monkeypatch.setenv("TENSORMESH_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    tune = importlib.import_module("tensormesh_amd.tune")
+    monkeypatch.setattr(tune, "_TUNER", None)
+    m10 = unit_cube_tet(3, 0.1, 2, order=2)
+    m10 = tm.Mesh(m10.points.to(dtype), m10.cells).to(DEV)
+    m4 = unit_cube_tet(4, 0.1, 2)
+    m4 = tm.Mesh(m4.points.to(dtype), m4.cells).to(DEV)
+    mh = unit_cube_hex(3, 0.1, 2)
+    mh = tm.Mesh(mh.points.to(dtype), mh.cells).to(DEV)
+    P = m10.points
+    u10 = 0.03 * torch.stack([torch.sin(P[:, 1]), P[:, 0] * P[:, 2], 0.5 * P[:, 0] ** 2], 1)
+    P4 = m4.points
+    u4 = torch.stack([3e-3 * P4[:, 0], -1e-3 * P4[:, 1], 5e-4 * P4[:, 2]], 1)
+    cases = [
+        ("elasticity tetra10", tm.LinearElasticityElementAssembler.from_mesh(m10, quadrature_order=2, E=1.0, nu=0.3),
+         lambda A: A().values, (1, 2)),
+        ("elasticity hexahedron", tm.LinearElasticityElementAssembler.from_mesh(mh, quadrature_order=2, E=1.0, nu=0.3),
+         lambda A: A().values, (1, 2)),
+        ("neo-hookean tetra10", tm.NeoHookeanModel.from_mesh(m10, quadrature_order=2, E=1.0, nu=0.3),
+         lambda A: torch.cat([t.reshape(-1) for t in (A.residual_and_tangent(u10)[0],
+                                                     A.residual_and_tangent(u10)[1].values)]), (1, 2, 4)),
+        ("neo-hookean energy", tm.NeoHookeanModel.from_mesh(m10, quadrature_order=2, E=1.0, nu=0.3),
+         lambda A: torch.cat([A._native_energy(u10, True)[0].reshape(1), A._native_energy(u10, True)[1]]),
+         (1, 2, 4)),
+        ("j2 tetra", tm.J2Plasticity.from_mesh(m4, quadrature_order=2, E=200.0, nu=0.3, sig0=0.25, H=1.0),
+         lambda A: torch.cat([A.residual_and_tangent(u4)[0], A.residual_and_tangent(u4)[1].values]), (1, 4, 8)),
+    ]
+    for name, A, run, options in cases:
+        outs = {}
+        for e in options:
+            A.epw = e
+            outs[e] = run(A)
+        for e in options[1:]:
+            assert torch.equal(outs[options[0]], outs[e]), f"{name}: epw {e} differs from {options[0]}"
+        A.epw = "auto"
+        assert torch.equal(run(A), outs[options[0]]), name
+        A.epw = 3
+        with pytest.raises(nat.NativeError):
+            run(A)
+        A.epw = None
+    data = json.loads((tmp_path / "tune.json").read_text())
+    assert any("|epw|neohookean|" in k for k in data) and any("|epw|j2|" in k for k in data)

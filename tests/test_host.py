@@ -572,6 +572,36 @@ def test_cg_warns_when_the_budget_runs_out():
         cg(Ki, fi, rtol=1e-12, max_iter=2, check_every=1)
 
 
+def test_cg_refinement_reaches_the_residual_rounding_level():
+    """refine=1 solves A d = b - A x once more: on a loosely stopped solve the
+    error against the dense solution drops by orders of magnitude, the
+    reported residual is the true one and the iterations are the total."""
+    from tensormesh_amd.solve import cg
+    m = unit_square_tri(12, 0.2, 3)
+    A = tm.DiffusionElementAssembler.from_mesh(m)
+    A.reference_path = True
+    E = m.elements().shape[0]
+    K = A(element_data={"kappa": torch.from_numpy(lognormal_coefficient(E, 3))})
+    Ki, fi = tm.Condenser(m.boundary_mask)(K, torch.ones(m.n_points, dtype=torch.float64))
+    ref = torch.linalg.solve(Ki.to_dense(), fi)
+    x0, it0, _ = cg(Ki, fi, rtol=1e-6, check_every=1)
+    x1, it1, rel1 = cg(Ki, fi, rtol=1e-6, check_every=1, refine=1)
+    e0 = ((x0 - ref).norm() / ref.norm()).item()
+    e1 = ((x1 - ref).norm() / ref.norm()).item()
+    assert e1 < 1e-3 * e0
+    assert it1 > it0
+    true_rel = ((fi - Ki.to_dense() @ x1).norm() / fi.norm()).item()
+    assert abs(rel1 - true_rel) <= 1e-3 * true_rel + 1e-18
+    # differentiable: the adjoint solve is refined as well
+    v = Ki.values.clone().requires_grad_(True)
+    Kg = tm.SparseMatrix(v, Ki.row_indices, Ki.col_indices, Ki.shape)
+    u = Kg.solve(fi, rtol=1e-6, refine=2)
+    u.sum().backward()
+    lam = torch.linalg.solve(Ki.to_dense(), torch.ones_like(fi))
+    g_ref = -lam[Ki.row_indices] * ref[Ki.col_indices]
+    assert torch.allclose(v.grad, g_ref, rtol=1e-8, atol=1e-12 * g_ref.abs().max().item())
+
+
 @pytest.mark.parametrize("etype,H", [("tetra10", 3), ("triangle6", 2), ("tetra10", 1)])
 def test_symmetric_pair_lists_reproduce_the_full_reduction(etype, H):
     """Pattern.sym_pairs on the CPU: summing each upper entry once from the

@@ -100,13 +100,14 @@ def test_layout_hash_block_and_condensed_are_int64():
 
 # ------------------------------------------------------ reference element
 # degree of exactness of each rule of the reference's tables; its tetrahedron
-# "order 5" (14 points) and "order 6" (15 points) rules reach degrees 4 and 5
-_EXACT = {("triangle", o): o for o in range(1, 7)}
-_EXACT.update({("tetra", o): d for o, d in ((1, 1), (2, 2), (3, 3), (4, 4), (5, 4), (6, 5))})
+# "order 5" (14 points), "order 6" (15 points) and "order 7" (24 points)
+# rules reach degrees 4, 5 and 6
+_EXACT = {("triangle", o): o for o in range(1, 8)}
+_EXACT.update({("tetra", o): d for o, d in ((1, 1), (2, 2), (3, 3), (4, 4), (5, 4), (6, 5), (7, 6))})
 
 
-@pytest.mark.parametrize("etype,order,vol", [("triangle", o, 0.5) for o in (1, 2, 3, 4, 5, 6)] +
-                         [("tetra", o, 1 / 6) for o in (1, 2, 3, 4, 5, 6)])
+@pytest.mark.parametrize("etype,order,vol", [("triangle", o, 0.5) for o in range(1, 8)] +
+                         [("tetra", o, 1 / 6) for o in range(1, 8)])
 def test_quadrature_weights_and_exactness(etype, order, vol):
     w, q = fem.quadrature(etype, order)
     assert abs(w.sum().item() - vol) < 1e-12

@@ -571,6 +571,24 @@ int tm_segmented_reduce_pairs(int32_t dtype, int64_t n_seg, int32_t hh, const in
 int tm_assemble_linear(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
                        const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream);
 
+/* Allocation-free variant of tm_assemble_linear.  No counterpart in the
+ * reference: it exists for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and summation order as
+ * tm_assemble_linear without slot_ws: the n_elem*B slot values live in a
+ * library-owned device array of TM_LINEAR_SCRATCH_BYTES, one per GPU the
+ * library runs on, held until the process ends (its address is looked up on
+ * the first call on each device).  Calls whose slot values exceed it
+ * (n_elem*B*sizeof(value) > TM_LINEAR_SCRATCH_BYTES, e.g. more than 2,097,152
+ * float64 tetrahedra) fail with TM_ERR_SIZE before any launch.  The stream
+ * must belong to the current device.  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU (two streams, or two host threads)
+ * race; calls in one stream are ordered and safe.  Nothing needs clearing:
+ * each call writes every slot value it reads. */
+#define TM_LINEAR_SCRATCH_BYTES (64LL << 20)
+int tm_assemble_linear_noalloc(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
+                               const tm_incidence_view* inc, void* out, tm_stream_t stream);
+
 /* ---------------------------------------------------------------------
  * Condensation values.  Replaces the per-call part of Condenser.__call__
  * (tensormesh/operator/condense.py:278-296).  Topology arrays come from the

@@ -90,6 +90,7 @@ class J2State(ctypes.Structure):
 
 
 J2_ENERGY_PARTS = 1024
+LINEAR_SCRATCH_BYTES = 64 << 20   # TM_LINEAR_SCRATCH_BYTES: slot array of tm_assemble_linear_noalloc
 
 
 class ContactPlane(ctypes.Structure):
@@ -131,6 +132,9 @@ SIGNATURES = {
     "tm_assemble_linear": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
                                           ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p]),
+    "tm_assemble_linear_noalloc": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),
+                                                  ctypes.c_void_p, ctypes.POINTER(IncidenceView), ctypes.c_void_p,
+                                                  ctypes.c_void_p]),
     "tm_colpos_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
                         + [ctypes.c_void_p] * 6),
     "tm_assemble_p1_rows": (ctypes.c_int, [ctypes.POINTER(MeshView), ctypes.POINTER(FormDesc),

@@ -731,7 +731,7 @@ class NodeAssembler(_AssemblerBase):
         return cache.get(fn, layout, lambda: match_linear(fn, point_data, scalar_data))
 
     def _linear_args(self, points: torch.Tensor, f: Optional[torch.Tensor], const: float):
-        """ctypes arguments of tm_assemble_linear for a device call:
+        """ctypes arguments of tm_assemble_linear(_noalloc) for a device call:
         (mesh view, form descriptor, constant or None, incidence view, the
         tensors the views point into).  ``points`` and ``f`` must be
         contiguous device tensors of one dtype."""

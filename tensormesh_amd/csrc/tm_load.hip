@@ -11,13 +11,52 @@
 // (ReduceProjector over conn.flatten(), node_assembler.py:665-670) becomes a
 // gather over the node incidence lists (ascending e*B + i), done by the
 // segmented reduction of tm_reduce.hip with seg_ptr = inc_ptr, seg_slot = inc.
+//
+// tm_assemble_linear_noalloc runs the same two stages with the slot values in
+// a file-scope device array (`scratch`) instead of a caller workspace, for
+// callers that cannot allocate.  Its device address is looked up once per GPU
+// with hipGetSymbolAddress (the host-side name of a __device__ array is not
+// a device address) and kept per device.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 
 #include "tm_internal.h"
 
 namespace {
 
 constexpr int kBlock = 256;
+
+// Slot values of tm_assemble_linear_noalloc: n_elem*B values of the mesh dtype.
+static __device__ __align__(256) unsigned char scratch[TM_LINEAR_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device (with it current) and kept.
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) {
+    tmh::set_error("hipGetDevice: %s", hipGetErrorString(err));
+    return TM_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) return tmh::fail(TM_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || !p) {
+      tmh::set_error("hipGetSymbolAddress(scratch): %s", hipGetErrorString(err));
+      return TM_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TM_OK;
+}
 
 template <typename T, int D, int B, bool CONST>
 __global__ void __launch_bounds__(kBlock)
@@ -101,24 +140,70 @@ int dispatch(const tm_mesh_view* m, const tm_form_desc* f, const double* cval, v
   }
 }
 
-}  // namespace
-
-extern "C" int tm_assemble_linear(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
-                                  const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream_) {
-  TM_TRY_BEGIN
-  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
+int check_linear_args(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
+                      const tm_incidence_view* inc) {
   if (!mesh || !form || !inc) return tmh::fail(TM_ERR_ARG, "NULL argument");
   if (tmh::cell_basis(mesh->cell) < 0) return tmh::fail(TM_ERR_ARG, "unknown cell");
+  if (mesh->dtype != TM_F64 && mesh->dtype != TM_F32) return tmh::fail(TM_ERR_ARG, "unknown dtype");
   if (mesh->n_elem < 0 || mesh->n_elem * tmh::cell_basis(mesh->cell) >= tmh::kMaxIndex)
     return tmh::fail(TM_ERR_SIZE, "mesh too large for int32 indexing");
   if (!form->tab_dev || form->n_quad <= 0) return tmh::fail(TM_ERR_ARG, "device form tables missing");
   if (!const_value && !form->coeff) return tmh::fail(TM_ERR_ARG, "need a nodal field or a constant");
-  if (mesh->n_elem > 0 && (!slot_ws || !out || !mesh->conn || !mesh->points))
-    return tmh::fail(TM_ERR_ARG, "NULL buffer");
+  if (mesh->n_points < 0 || mesh->n_points >= tmh::kMaxIndex) return tmh::fail(TM_ERR_SIZE, "n_points out of range");
+  if (mesh->n_points > 0 && (!inc->inc_ptr || !inc->inc)) return tmh::fail(TM_ERR_ARG, "NULL incidence lists");
+  return TM_OK;
+}
+
+int run_linear(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
+               const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream_) {
+  hipStream_t stream = reinterpret_cast<hipStream_t>(stream_);
   int st = mesh->dtype == TM_F64 ? dispatch<double>(mesh, form, const_value, slot_ws, stream)
                                  : dispatch<float>(mesh, form, const_value, slot_ws, stream);
   if (st) return st;
   return tm_segmented_reduce(mesh->dtype, mesh->n_points, 1, inc->inc_ptr, inc->inc, inc->n_blocks, inc->blk_ptr,
                              slot_ws, out, stream_);
+}
+
+}  // namespace
+
+extern "C" int tm_assemble_linear(const tm_mesh_view* mesh, const tm_form_desc* form, const double* const_value,
+                                  const tm_incidence_view* inc, void* slot_ws, void* out, tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  int st = check_linear_args(mesh, form, const_value, inc);
+  if (st) return st;
+  if (mesh->n_elem > 0 && (!slot_ws || !mesh->conn || !mesh->points)) return tmh::fail(TM_ERR_ARG, "NULL buffer");
+  if (mesh->n_points > 0 && !out) return tmh::fail(TM_ERR_ARG, "NULL output");
+  return run_linear(mesh, form, const_value, inc, slot_ws, out, stream_);
+  TM_TRY_END
+}
+
+extern "C" int tm_assemble_linear_noalloc(const tm_mesh_view* mesh, const tm_form_desc* form,
+                                          const double* const_value, const tm_incidence_view* inc, void* out,
+                                          tm_stream_t stream_) {
+  TM_TRY_BEGIN
+  int st = check_linear_args(mesh, form, const_value, inc);
+  if (st) return st;
+  const int64_t value_bytes = mesh->dtype == TM_F64 ? 8 : 4;
+  const int64_t need = mesh->n_elem * tmh::cell_basis(mesh->cell) * value_bytes;
+  if (need > (int64_t)TM_LINEAR_SCRATCH_BYTES) {
+    tmh::set_error("tm_assemble_linear_noalloc: %lld slot bytes exceed the %lld-byte scratch array "
+                   "(use tm_assemble_linear with a workspace)",
+                   (long long)need, (long long)TM_LINEAR_SCRATCH_BYTES);
+    return TM_ERR_SIZE;
+  }
+  if (mesh->n_elem > 0 && (!mesh->conn || !mesh->points)) return tmh::fail(TM_ERR_ARG, "NULL buffer");
+  if (mesh->n_points > 0 && !out) return tmh::fail(TM_ERR_ARG, "NULL output");
+  if (stream_) {
+    // the scratch address is per device: the stream must belong to the current one
+    int cur = -1, sdev = -1;
+    if (hipGetDevice(&cur) != hipSuccess ||
+        hipStreamGetDevice(reinterpret_cast<hipStream_t>(stream_), &sdev) != hipSuccess)
+      return tmh::fail(TM_ERR_HIP, "tm_assemble_linear_noalloc: device query failed");
+    if (cur != sdev) return tmh::fail(TM_ERR_ARG, "tm_assemble_linear_noalloc: stream is not on the current device");
+  }
+  void* slot = nullptr;
+  st = scratch_on_current_device(&slot);
+  if (st) return st;
+  return run_linear(mesh, form, const_value, inc, slot, out, stream_);
   TM_TRY_END
 }

@@ -31,6 +31,25 @@ def test_library_exports_every_header_symbol():
     assert L.tm_abi_version() == nat.ABI_VERSION
 
 
+def test_noalloc_load_rejects_sizes_beyond_its_array():
+    """tm_assemble_linear_noalloc refuses slot values beyond its fixed device
+    array with TM_ERR_SIZE before touching the device (host arithmetic only)."""
+    L = nat.lib()
+    B = 4
+    n_elem = nat.LINEAR_SCRATCH_BYTES // (8 * B) + 1           # one float64 tetrahedron too many
+    dummy = ctypes.c_int32(0)
+    mesh = nat.MeshView(nat.CELL["tetra"], nat.TM_F64, n_elem, 8, ctypes.addressof(dummy), ctypes.addressof(dummy))
+    tab = ctypes.c_double(0.0)
+    form = nat.FormDesc(0, 1, ctypes.addressof(tab), ctypes.addressof(tab), None, 0.0, 0.0)
+    iv = nat.IncidenceView(ctypes.addressof(dummy), ctypes.addressof(dummy), 0, None)
+    c = ctypes.c_double(1.0)
+    st = L.tm_assemble_linear_noalloc(ctypes.byref(mesh), ctypes.byref(form), ctypes.byref(c), ctypes.byref(iv),
+                                      ctypes.addressof(dummy), None)
+    assert st == -2 and b"scratch" in L.tm_last_error()
+    mesh.dtype = nat.TM_F32                                      # the same count fits in float32
+    assert nat.LINEAR_SCRATCH_BYTES // (4 * B) >= n_elem
+
+
 def test_errors_cross_the_abi_as_status():
     L = nat.lib()
     conn = np.array([[0, 1, 5]], dtype=np.int32)

@@ -733,21 +733,29 @@ def batched_bench(n_meshes=256, seed=0):
             "loop_same_api_s_extrapolated": t_loop, "speedup_vs_loop": t_loop / t_b}
 
 
-def source_digest():
-    """SHA-256 of the kernel sources (tensormesh_amd/csrc): ties counter data to
-    the code it was measured on."""
+# the sources that decide what the headline kernel (p1_rows_kernel) reads and
+# writes: the kernel, its headers and the builders of the plans it walks
+ROOFLINE_SOURCES = ("tensormesh_amd/csrc/tm_rows.hip", "tensormesh_amd/csrc/tm_device.h",
+                    "tensormesh_amd/csrc/tm_internal.h", "include/tensormesh_amd.h",
+                    "tensormesh_amd/csrc/tm_pattern.cpp", "tensormesh_amd/csrc/tm_pattern_gpu.hip",
+                    "tensormesh_amd/pattern.py")
+
+
+def source_digest(paths=ROOFLINE_SOURCES):
+    """SHA-256 of the given sources: ties counter data to the code it was
+    measured on."""
     import hashlib
     h = hashlib.sha256()
-    csrc = os.path.join(ROOT, "tensormesh_amd", "csrc")
-    for name in sorted(os.listdir(csrc)):
-        with open(os.path.join(csrc, name), "rb") as f:
-            h.update(name.encode() + f.read())
+    for rel in paths:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(rel.encode() + f.read())
     return h.hexdigest()
 
 
 def fp32_bench(steps=30, warmup=5):
     """The headline workload in float32 (coordinates, kappa and values float32;
-    the fused kernel accumulates in float32): elements/s, the fraction of
+    the fused kernel evaluates element geometry in float32, redoing slivers in
+    float64, and accumulates in float32): elements/s, the fraction of
     the HBM roofline on the float32 two-stage pipeline bytes, and the error
     against the float64 result of the same call on the same (float32)
     inputs, which the GPU tests hold to the float64 oracle within 1e-12."""

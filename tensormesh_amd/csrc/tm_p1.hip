@@ -22,6 +22,9 @@
 // tm_reduce.hip.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
+#include "tm_device.h"
 #include "tm_internal.h"
 
 namespace {
@@ -32,6 +35,7 @@ struct P1Consts {
   double W;           // sum_q w_q
   double wphi[4];     // sum_q w_q phi_b(q)
   double mref[16];    // sum_q w_q phi_i(q) phi_j(q), row-major B x B
+  int32_t redo;       // float32 geometry: redo slivers in float64 (not TM_FLAG_F32_NO_REDO)
 };
 
 template <int D>
@@ -39,10 +43,11 @@ struct Simplex {
   static constexpr int B = D + 1;
 };
 
-// Gather the B vertex coordinates of element e.
-template <typename T, int D>
+// Gather the B vertex coordinates of element e as values of the geometry
+// type G (double, or float for float meshes).
+template <typename T, int D, typename G>
 __device__ __forceinline__ void gather(const int32_t* __restrict__ conn, const T* __restrict__ pts,
-                                       int64_t e, int (&v)[D + 1], double (&x)[D + 1][D]) {
+                                       int64_t e, int (&v)[D + 1], G (&x)[D + 1][D]) {
   constexpr int B = D + 1;
   if constexpr (D == 3) {
     const int4 c = reinterpret_cast<const int4*>(conn)[e];
@@ -54,21 +59,22 @@ __device__ __forceinline__ void gather(const int32_t* __restrict__ conn, const T
 #pragma unroll
   for (int b = 0; b < B; ++b) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[b][d] = (double)pts[(int64_t)v[b] * D + d];
+    for (int d = 0; d < D; ++d) x[b][d] = (G)pts[(int64_t)v[b] * D + d];
   }
 }
 
-// Cofactor vectors n[k] (columns of adj(J)) and det J for the P1 simplex.
-template <int D>
-__device__ __forceinline__ double cofactors(const double (&x)[D + 1][D], double (&n)[D][D]) {
-  if constexpr (D == 3) {
-    double a[3], b[3], c[3];
+// Edge vectors e[k] = x[k+1] - x[0], cofactor vectors n[k] (columns of
+// adj(J)) and det J for the P1 simplex.
+template <int D, typename G>
+__device__ __forceinline__ G cofactors(const G (&x)[D + 1][D], G (&e)[D][D], G (&n)[D][D]) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      a[d] = x[1][d] - x[0][d];
-      b[d] = x[2][d] - x[0][d];
-      c[d] = x[3][d] - x[0][d];
-    }
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int d = 0; d < D; ++d) e[k][d] = x[k + 1][d] - x[0][d];
+  if constexpr (D == 3) {
+    const G(&a)[3] = e[0];
+    const G(&b)[3] = e[1];
+    const G(&c)[3] = e[2];
     // columns of adj(J) for J with rows a, b, c
     n[0][0] = b[1] * c[2] - b[2] * c[1];
     n[0][1] = b[2] * c[0] - b[0] * c[2];
@@ -81,36 +87,44 @@ __device__ __forceinline__ double cofactors(const double (&x)[D + 1][D], double 
     n[2][2] = a[0] * b[1] - a[1] * b[0];
     return a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
   } else {
-    const double a0 = x[1][0] - x[0][0], a1 = x[1][1] - x[0][1];
-    const double b0 = x[2][0] - x[0][0], b1 = x[2][1] - x[0][1];
+    const G a0 = e[0][0], a1 = e[0][1];
+    const G b0 = e[1][0], b1 = e[1][1];
     n[0][0] = b1;  n[0][1] = -b0;
     n[1][0] = -a1; n[1][1] = a0;
     return a0 * b1 - a1 * b0;
   }
 }
 
-// Local matrix of one element in registers (double), B*B entries.
-template <typename T, int D, int FORM>
+// Local matrix of one element in registers, B*B entries (returned as
+// double).  G is the geometry type: float meshes evaluate in float and redo
+// elements that tmh::sliver_f32 flags in double.
+template <typename T, int D, int FORM, typename G = typename std::conditional<sizeof(T) == 4, float, double>::type>
 __device__ __forceinline__ void p1_local(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                                          const T* __restrict__ coeff, const P1Consts& c, int64_t e,
                                          double (&K)[(D + 1) * (D + 1)]) {
   constexpr int B = D + 1;
   int v[B];
-  double x[B][D];
-  gather<T, D>(conn, pts, e, v, x);
-  double n[D][D];
-  const double det = cofactors<D>(x, n);
-  const double adet = fabs(det);
+  G x[B][D];
+  gather<T, D, G>(conn, pts, e, v, x);
+  G ed[D][D], n[D][D];
+  const G det = cofactors<D, G>(x, ed, n);
+  if constexpr (sizeof(G) == 4) {
+    if (c.redo && tmh::sliver_f32<D>(ed, det)) {
+      p1_local<T, D, FORM, double>(conn, pts, coeff, c, e, K);
+      return;
+    }
+  }
+  const G adet = det < G(0) ? -det : det;
   if constexpr (FORM == TM_FORM_MASS) {
 #pragma unroll
-    for (int k = 0; k < B * B; ++k) K[k] = adet * c.mref[k];
+    for (int k = 0; k < B * B; ++k) K[k] = (double)(adet * (G)c.mref[k]);
     return;
   } else {
     // grad(phi) * det: g[0] = -sum n[k], g[k+1] = n[k]
-    double g[B][D];
+    G g[B][D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      double s = 0.0;
+      G s = G(0);
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         g[k + 1][d] = n[k][d];
@@ -118,26 +132,26 @@ __device__ __forceinline__ void p1_local(const int32_t* __restrict__ conn, const
       }
       g[0][d] = -s;
     }
-    double coef = c.W;
+    G coef = (G)c.W;
     if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
-      coef *= (double)coeff[e];
+      coef *= (G)coeff[e];
     } else if constexpr (FORM == TM_FORM_DIFFUSION_NODE) {
-      double kq = 0.0;
+      G kq = G(0);
 #pragma unroll
-      for (int b = 0; b < B; ++b) kq += (double)coeff[v[b]] * c.wphi[b];
+      for (int b = 0; b < B; ++b) kq += (G)coeff[v[b]] * (G)c.wphi[b];
       coef = kq;
     }
     // c * W * |det| * (n_i/det).(n_j/det) = c * W * (n_i . n_j) / |det|
-    const double scale = coef / adet;
+    const G scale = coef / adet;
 #pragma unroll
     for (int i = 0; i < B; ++i) {
 #pragma unroll
       for (int j = i; j < B; ++j) {
-        double s = 0.0;
+        G s = G(0);
 #pragma unroll
         for (int d = 0; d < D; ++d) s += g[i][d] * g[j][d];
-        K[i * B + j] = scale * s;
-        K[j * B + i] = scale * s;
+        K[i * B + j] = (double)(scale * s);
+        K[j * B + i] = (double)(scale * s);
       }
     }
   }
@@ -227,6 +241,7 @@ int fill_consts(const tm_form_desc* f, int B, int D, P1Consts& c) {
     for (int i = 0; i < B; ++i)
       for (int j = 0; j < B; ++j) c.mref[i * B + j] += w[q] * phi[q * B + i] * phi[q * B + j];
   }
+  c.redo = (f->flags & TM_FLAG_F32_NO_REDO) ? 0 : 1;
   (void)D;
   return TM_OK;
 }

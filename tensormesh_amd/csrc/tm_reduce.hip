@@ -184,16 +184,65 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
   }
   __syncthreads();
-  for (int32_t k = kb + threadIdx.x; k < ke; k += kBlock) {
-    const int32_t a = seg_ptr[k] - cb, z = seg_ptr[k + 1] - cb;
-    T acc[HH];
+  constexpr int kD = HH == 9 ? 3 : HH == 4 ? 2 : 1;
+  constexpr int L = kD > 1 ? kD : HH;  // lanes per entry below
+  constexpr int kPerPass = kBlock / L;
+  // more than one pass per lane group measured slower for float32 slots
+  // (whose 16 KB runs hold twice the entries) than a thread per entry
+  constexpr int kMaxPass = sizeof(T) == 8 ? 4 : 1;
+  if (HH == 1 || ke - kb > kMaxPass * kPerPass) {
+    for (int32_t k = kb + threadIdx.x; k < ke; k += kBlock) {
+      const int32_t a = seg_ptr[k] - cb, z = seg_ptr[k + 1] - cb;
+      T acc[HH];
 #pragma unroll
-    for (int h = 0; h < HH; ++h) acc[h] = T(0);
-    for (int32_t u = a; u < z; ++u) {
+      for (int h = 0; h < HH; ++h) acc[h] = T(0);
+      for (int32_t u = a; u < z; ++u) {
 #pragma unroll
-      for (int h = 0; h < HH; ++h) acc[h] += lds[u * HH + h];
+        for (int h = 0; h < HH; ++h) acc[h] += lds[u * HH + h];
+      }
+      write_entry<T, HH>(out, k, acc, entry, entry_t);
     }
-    write_entry<T, HH>(out, k, acc, entry, entry_t);
+  } else {
+    // L lanes per entry (one per block row for D x D blocks), lane l summing
+    // values [l*V, l*V + V), up to kMaxPass entries per lane group with all
+    // their bounds and destinations loaded first.  The lanes of an entry read
+    // adjacent LDS words and write its HH values (and their transposed copy)
+    // as one contiguous piece, where a thread per entry stores HH-strided
+    // words from every lane.  Each value is still summed over the entry's
+    // contributions in order.
+    constexpr int V = HH / L;
+    const int32_t le = (int32_t)threadIdx.x / L, l = (int32_t)threadIdx.x - le * L;
+    if (le < kPerPass) {
+      int32_t ra[kMaxPass], rz[kMaxPass];
+      int64_t d0[kMaxPass], d1[kMaxPass];
+#pragma unroll
+      for (int ps = 0; ps < kMaxPass; ++ps) {
+        const int32_t k = kb + le + ps * kPerPass;
+        const bool has = k < ke;
+        ra[ps] = has ? seg_ptr[k] - cb : 0;
+        rz[ps] = has ? seg_ptr[k + 1] - cb : 0;
+        d0[ps] = !has ? -1 : entry ? (int64_t)entry[k] : (int64_t)k;
+        d1[ps] = (has && entry) ? (int64_t)entry_t[k] : -1;
+      }
+#pragma unroll
+      for (int ps = 0; ps < kMaxPass; ++ps) {
+        if (d0[ps] < 0) break;
+        T acc[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] = T(0);
+        for (int32_t u = ra[ps]; u < rz[ps]; ++u) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) acc[v] += lds[u * HH + l * V + v];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[d0[ps] * HH + l * V + v] = acc[v];
+        if (d1[ps] >= 0 && d1[ps] != d0[ps]) {
+          // value (l, v) of the block lands at (v, l) of the mirror
+#pragma unroll
+          for (int v = 0; v < V; ++v) out[d1[ps] * HH + (kD > 1 ? v * kD + l : l * V + v)] = acc[v];
+        }
+      }
+    }
   }
 }
 

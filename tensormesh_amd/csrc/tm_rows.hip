@@ -18,6 +18,9 @@
 // read back from a B*B slot buffer.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
+#include "tm_device.h"
 #include "tm_internal.h"
 
 namespace {
@@ -31,6 +34,7 @@ struct RowConsts {
   double W;        // sum_q w_q
   double wphi[4];  // sum_q w_q phi_b(q)
   double mref[16]; // sum_q w_q phi_i phi_j
+  int32_t redo;    // float32 geometry: redo slivers in float64 (not TM_FLAG_F32_NO_REDO)
 };
 
 __device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
@@ -39,42 +43,48 @@ __device__ __forceinline__ int64_t xcd_logical(int64_t b, int64_t nb) {
   return x < r ? x * (q + 1) + y : r * (q + 1) + (x - r) * q + y;
 }
 
-// Coordinates of node v as D doubles: for 3-D double points one 16-byte and
-// one 8-byte load (8-byte aligned), for float points one 12-byte load.
-template <typename T, int D>
-__device__ __forceinline__ void load_node(const T* __restrict__ pts, int32_t v, double (&x)[D]) {
+// Coordinates of node v as D values of the geometry type G (double, or
+// float for float meshes): for 3-D double points one 16-byte and one 8-byte
+// load (8-byte aligned), for float points one 12-byte load.
+template <typename T, int D, typename G>
+__device__ __forceinline__ void load_node(const T* __restrict__ pts, int32_t v, G (&x)[D]) {
   const T* p = pts + (int64_t)v * D;
   if constexpr (D == 3 && sizeof(T) == 8) {
     typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
     const d2u a = *reinterpret_cast<const d2u*>(p);
-    x[0] = a.x;
-    x[1] = a.y;
-    x[2] = p[2];
+    x[0] = (G)a.x;
+    x[1] = (G)a.y;
+    x[2] = (G)p[2];
   } else if constexpr (D == 3) {
     typedef float f3u __attribute__((ext_vector_type(3), aligned(4)));
     const f3u a = *reinterpret_cast<const f3u*>(p);
-    x[0] = a.x;
-    x[1] = a.y;
-    x[2] = a.z;
+    x[0] = (G)a.x;
+    x[1] = (G)a.y;
+    x[2] = (G)a.z;
   } else {
 #pragma unroll
-    for (int j = 0; j < D; ++j) x[j] = (double)p[j];
+    for (int j = 0; j < D; ++j) x[j] = (G)p[j];
   }
 }
+
+// Geometry type of the row kernels: float meshes evaluate element geometry
+// in float and redo the slivers in double (tmh::sliver_f32).
+template <typename T>
+using Geom = typename std::conditional<sizeof(T) == 4, float, double>::type;
 
 // Element data a local row needs, gathered in three dependent steps
 // (incidence -> connectivity -> coordinates / coefficients) so that a thread
 // can issue the loads of several incidences before it computes any of them.
-template <int D>
+template <int D, typename G>
 struct ElemData {
   static constexpr int B = D + 1;
   int v[B];
-  double x[B][D];
-  double k[B];  // element coefficient in k[0], or the nodal coefficients
+  G x[B][D];
+  G k[B];  // element coefficient in k[0], or the nodal coefficients
 };
 
-template <int D>
-__device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int64_t e, ElemData<D>& d) {
+template <int D, typename G>
+__device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int64_t e, ElemData<D, G>& d) {
   constexpr int B = D + 1;
   if constexpr (D == 3) {
     const int4 q = reinterpret_cast<const int4*>(conn)[e];
@@ -85,17 +95,17 @@ __device__ __forceinline__ void load_conn(const int32_t* __restrict__ conn, int6
   }
 }
 
-template <typename T, int D, int FORM>
+template <typename T, int D, int FORM, typename G>
 __device__ __forceinline__ void load_geometry(const T* __restrict__ pts, const T* __restrict__ coeff, int64_t e,
-                                              ElemData<D>& d) {
+                                              ElemData<D, G>& d) {
   constexpr int B = D + 1;
 #pragma unroll
-  for (int b = 0; b < B; ++b) load_node<T, D>(pts, d.v[b], d.x[b]);
+  for (int b = 0; b < B; ++b) load_node<T, D, G>(pts, d.v[b], d.x[b]);
   if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
-    d.k[0] = (double)coeff[e];
+    d.k[0] = (G)coeff[e];
   } else if constexpr (FORM == TM_FORM_DIFFUSION_NODE) {
 #pragma unroll
-    for (int b = 0; b < B; ++b) d.k[b] = (double)coeff[d.v[b]];
+    for (int b = 0; b < B; ++b) d.k[b] = (G)coeff[d.v[b]];
   }
 }
 
@@ -105,17 +115,17 @@ __device__ __forceinline__ void load_geometry(const T* __restrict__ pts, const T
 // runtime i in registers (selects on the connectivity word only).  The row
 // node's coordinates come from LDS (staged per row); the other B - 1
 // vertices are gathered.
-template <int D>
+template <int D, typename G>
 struct RotData {
   static constexpr int B = D + 1;
-  int32_t c[B];     // rotated connectivity
-  double x[B][D];   // rotated coordinates (x[0] filled from LDS)
-  double k;         // element coefficient
+  int32_t c[B];  // rotated connectivity
+  G x[B][D];     // rotated coordinates (x[0] filled from LDS)
+  G k;           // element coefficient
 };
 
-template <typename T, int D, int FORM>
+template <typename T, int D, int FORM, typename G>
 __device__ __forceinline__ void load_rot(const int32_t* __restrict__ conn, const T* __restrict__ coeff, int64_t e,
-                                         int i, RotData<D>& d) {
+                                         int i, RotData<D, G>& d) {
   if constexpr (D == 3) {
     const int4 q = reinterpret_cast<const int4*>(conn)[e];
     d.c[0] = i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w;
@@ -128,29 +138,27 @@ __device__ __forceinline__ void load_rot(const int32_t* __restrict__ conn, const
     d.c[1] = i == 0 ? a1 : i == 1 ? a2 : a0;
     d.c[2] = i == 0 ? a2 : i == 1 ? a0 : a1;
   }
-  if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) d.k = (double)coeff[e];
+  if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) d.k = (G)coeff[e];
 }
 
-template <typename T, int D>
-__device__ __forceinline__ void gather_rot(const T* __restrict__ pts, RotData<D>& d) {
+template <typename T, int D, typename G>
+__device__ __forceinline__ void gather_rot(const T* __restrict__ pts, RotData<D, G>& d) {
 #pragma unroll
-  for (int p = 1; p < D + 1; ++p) load_node<T, D>(pts, d.c[p], d.x[p]);
+  for (int p = 1; p < D + 1; ++p) load_node<T, D, G>(pts, d.c[p], d.x[p]);
 }
 
-// Row 0 of the rotated element's local matrix: K[0][p] = scale g_0 . g_p
-template <int D, int FORM>
-__device__ __forceinline__ void rot_row(const RotData<D>& d, const RowConsts& c, double (&Kr)[D + 1]) {
-  constexpr int B = D + 1;
-  double n[D][D];
-  double det;
+// Edge vectors from vertex 0, cofactor vectors n[k] (columns of adj J) and
+// det J of the simplex with vertices x.
+template <int D, typename G>
+__device__ __forceinline__ G simplex_cofactors(const G (&x)[D + 1][D], G (&e)[D][D], G (&n)[D][D]) {
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+#pragma unroll
+    for (int j = 0; j < D; ++j) e[k][j] = x[k + 1][j] - x[0][j];
   if constexpr (D == 3) {
-    double a[3], bb[3], cc[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      a[j] = d.x[1][j] - d.x[0][j];
-      bb[j] = d.x[2][j] - d.x[0][j];
-      cc[j] = d.x[3][j] - d.x[0][j];
-    }
+    const G(&a)[3] = e[0];
+    const G(&bb)[3] = e[1];
+    const G(&cc)[3] = e[2];
     n[0][0] = bb[1] * cc[2] - bb[2] * cc[1];
     n[0][1] = bb[2] * cc[0] - bb[0] * cc[2];
     n[0][2] = bb[0] * cc[1] - bb[1] * cc[0];
@@ -160,79 +168,92 @@ __device__ __forceinline__ void rot_row(const RotData<D>& d, const RowConsts& c,
     n[2][0] = a[1] * bb[2] - a[2] * bb[1];
     n[2][1] = a[2] * bb[0] - a[0] * bb[2];
     n[2][2] = a[0] * bb[1] - a[1] * bb[0];
-    det = a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
+    return a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
   } else {
-    const double a0 = d.x[1][0] - d.x[0][0], a1 = d.x[1][1] - d.x[0][1];
-    const double b0 = d.x[2][0] - d.x[0][0], b1 = d.x[2][1] - d.x[0][1];
+    const G a0 = e[0][0], a1 = e[0][1];
+    const G b0 = e[1][0], b1 = e[1][1];
     n[0][0] = b1;  n[0][1] = -b0;
     n[1][0] = -a1; n[1][1] = a0;
-    det = a0 * b1 - a1 * b0;
+    return a0 * b1 - a1 * b0;
   }
-  double coef = c.W;
+}
+
+template <typename G>
+__device__ __forceinline__ G abs_g(G v) {
+  return v < G(0) ? -v : v;
+}
+
+// Row 0 of the rotated element's local matrix: K[0][p] = scale g_0 . g_p
+template <int D, int FORM, typename G>
+__device__ __forceinline__ void rot_row(const RotData<D, G>& d, const RowConsts& c, double (&Kr)[D + 1]) {
+  constexpr int B = D + 1;
+  G e[D][D], n[D][D];
+  const G det = simplex_cofactors<D, G>(d.x, e, n);
+  if constexpr (sizeof(G) == 4) {
+    if (c.redo && tmh::sliver_f32<D>(e, det)) {  // redo this element in double
+      RotData<D, double> w;
+#pragma unroll
+      for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int j = 0; j < D; ++j) w.x[b][j] = (double)d.x[b][j];
+      w.k = (double)d.k;
+      rot_row<D, FORM, double>(w, c, Kr);
+      return;
+    }
+  }
+  G coef = (G)c.W;
   if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) coef *= d.k;
-  const double scale = coef / fabs(det);
-  double g0[D];
+  const G scale = coef / abs_g(det);
+  G g0[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    double s = 0.0;
+    G s = G(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) s += n[k][j];
     g0[j] = -s;
   }
-  double s0 = 0.0;
+  G s0 = G(0);
 #pragma unroll
   for (int j = 0; j < D; ++j) s0 += g0[j] * g0[j];
-  Kr[0] = scale * s0;
+  Kr[0] = (double)(scale * s0);
 #pragma unroll
   for (int p = 1; p < B; ++p) {
-    double s = 0.0;
+    G s = G(0);
 #pragma unroll
     for (int j = 0; j < D; ++j) s += g0[j] * n[p - 1][j];
-    Kr[p] = scale * s;
+    Kr[p] = (double)(scale * s);
   }
 }
 
-// Row i of the local matrix of the element held in d, in double.
-template <int D, int FORM>
-__device__ __forceinline__ void row_from(const ElemData<D>& d, const RowConsts& c, int i, double (&Kr)[D + 1]) {
+// Row i of the local matrix of the element held in d.
+template <int D, int FORM, typename G>
+__device__ __forceinline__ void row_from(const ElemData<D, G>& d, const RowConsts& c, int i, double (&Kr)[D + 1]) {
   constexpr int B = D + 1;
-  double n[D][D];
-  double det;
-  if constexpr (D == 3) {
-    double a[3], bb[3], cc[3];
+  G e[D][D], n[D][D];
+  const G det = simplex_cofactors<D, G>(d.x, e, n);
+  if constexpr (sizeof(G) == 4) {
+    if (c.redo && tmh::sliver_f32<D>(e, det)) {  // redo this element in double
+      ElemData<D, double> w;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      a[j] = d.x[1][j] - d.x[0][j];
-      bb[j] = d.x[2][j] - d.x[0][j];
-      cc[j] = d.x[3][j] - d.x[0][j];
+      for (int b = 0; b < B; ++b) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) w.x[b][j] = (double)d.x[b][j];
+        w.k[b] = (double)d.k[b];
+      }
+      row_from<D, FORM, double>(w, c, i, Kr);
+      return;
     }
-    n[0][0] = bb[1] * cc[2] - bb[2] * cc[1];
-    n[0][1] = bb[2] * cc[0] - bb[0] * cc[2];
-    n[0][2] = bb[0] * cc[1] - bb[1] * cc[0];
-    n[1][0] = cc[1] * a[2] - cc[2] * a[1];
-    n[1][1] = cc[2] * a[0] - cc[0] * a[2];
-    n[1][2] = cc[0] * a[1] - cc[1] * a[0];
-    n[2][0] = a[1] * bb[2] - a[2] * bb[1];
-    n[2][1] = a[2] * bb[0] - a[0] * bb[2];
-    n[2][2] = a[0] * bb[1] - a[1] * bb[0];
-    det = a[0] * n[0][0] + a[1] * n[0][1] + a[2] * n[0][2];
-  } else {
-    const double a0 = d.x[1][0] - d.x[0][0], a1 = d.x[1][1] - d.x[0][1];
-    const double b0 = d.x[2][0] - d.x[0][0], b1 = d.x[2][1] - d.x[0][1];
-    n[0][0] = b1;  n[0][1] = -b0;
-    n[1][0] = -a1; n[1][1] = a0;
-    det = a0 * b1 - a1 * b0;
   }
-  const double adet = fabs(det);
+  const G adet = abs_g(det);
   if constexpr (FORM == TM_FORM_MASS) {
 #pragma unroll
-    for (int j = 0; j < B; ++j) Kr[j] = adet * c.mref[i * B + j];
+    for (int j = 0; j < B; ++j) Kr[j] = (double)(adet * (G)c.mref[i * B + j]);
     return;
   } else {
-    double g[B][D];
+    G g[B][D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      double s = 0.0;
+      G s = G(0);
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         g[k + 1][j] = n[k][j];
@@ -240,30 +261,30 @@ __device__ __forceinline__ void row_from(const ElemData<D>& d, const RowConsts& 
       }
       g[0][j] = -s;
     }
-    double coef = c.W;
+    G coef = (G)c.W;
     if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
       coef *= d.k[0];
     } else if constexpr (FORM == TM_FORM_DIFFUSION_NODE) {
-      double kq = 0.0;
+      G kq = G(0);
 #pragma unroll
-      for (int b = 0; b < B; ++b) kq += d.k[b] * c.wphi[b];
+      for (int b = 0; b < B; ++b) kq += d.k[b] * (G)c.wphi[b];
       coef = kq;
     }
-    const double scale = coef / adet;
-    double gi[D];
+    const G scale = coef / adet;
+    G gi[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      double s = g[0][j];
+      G s = g[0][j];
 #pragma unroll
       for (int b = 1; b < B; ++b) s = (i == b) ? g[b][j] : s;
       gi[j] = s;
     }
 #pragma unroll
     for (int j = 0; j < B; ++j) {
-      double s = 0.0;
+      G s = G(0);
 #pragma unroll
       for (int k = 0; k < D; ++k) s += gi[k] * g[j][k];
-      Kr[j] = scale * s;
+      Kr[j] = (double)(scale * s);
     }
   }
 }
@@ -273,10 +294,11 @@ template <typename T, int D, int FORM>
 __device__ __forceinline__ void local_row(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                                           const T* __restrict__ coeff, const RowConsts& c, int64_t e, int i,
                                           double (&Kr)[D + 1]) {
-  ElemData<D> d;
-  load_conn<D>(conn, e, d);
-  load_geometry<T, D, FORM>(pts, coeff, e, d);
-  row_from<D, FORM>(d, c, i, Kr);
+  using G = Geom<T>;
+  ElemData<D, G> d;
+  load_conn<D, G>(conn, e, d);
+  load_geometry<T, D, FORM, G>(pts, coeff, e, d);
+  row_from<D, FORM, G>(d, c, i, Kr);
 }
 
 // Block bounds: {first row, first incidence, first CSR entry} of blocks blk and
@@ -439,7 +461,8 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ int32_t s_rp[kMaxRows + 1];    // row starts, relative to k0
   __shared__ int32_t s_ip[kMaxRows + 1];    // incidence starts, relative to t0
   __shared__ uint16_t s_erow[kMaxEnt];      // entry -> local row
-  __shared__ double s_xr[kMaxRows * D];     // coordinates of the staged rows' nodes
+  using G = Geom<T>;
+  __shared__ G s_xr[kMaxRows * D];          // coordinates of the staged rows' nodes
   const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
   const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
   const int32_t r0 = bb.r0, r1 = bb.r1, t0 = bb.t0, k0 = bb.k0, k1 = bb.k1;
@@ -452,8 +475,8 @@ __global__ void __launch_bounds__(kBlock)
     // entry's contiguous range
     T* s_c = s_val;
     if (kRot && (int32_t)threadIdx.x < n_rows) {
-      double xr[D];
-      load_node<T, D>(pts, r0 + (int32_t)threadIdx.x, xr);
+      G xr[D];
+      load_node<T, D, G>(pts, r0 + (int32_t)threadIdx.x, xr);
 #pragma unroll
       for (int j = 0; j < D; ++j) s_xr[threadIdx.x * D + j] = xr[j];
     }
@@ -474,17 +497,17 @@ __global__ void __launch_bounds__(kBlock)
     }
     const int ia = ea % B, ib = eb % B;
     if constexpr (kRot) {
-      RotData<D> da, db;
-      if (ha) load_rot<T, D, FORM>(conn, coeff, ea / B, ia, da);
-      if (hb) load_rot<T, D, FORM>(conn, coeff, eb / B, ib, db);
-      if (ha) gather_rot<T, D>(pts, da);
-      if (hb) gather_rot<T, D>(pts, db);
+      RotData<D, G> da, db;
+      if (ha) load_rot<T, D, FORM, G>(conn, coeff, ea / B, ia, da);
+      if (hb) load_rot<T, D, FORM, G>(conn, coeff, eb / B, ib, db);
+      if (ha) gather_rot<T, D, G>(pts, da);
+      if (hb) gather_rot<T, D, G>(pts, db);
       __syncthreads();  // s_xr
       if (ha) {
 #pragma unroll
         for (int j = 0; j < D; ++j) da.x[0][j] = s_xr[(da.c[0] - r0) * D + j];
         double Kr[B];
-        rot_row<D, FORM>(da, c, Kr);
+        rot_row<D, FORM, G>(da, c, Kr);
         const uint64_t w = rotate_dest<B>(wa, ia);
 #pragma unroll
         for (int p = 0; p < B; ++p) s_c[(w >> (16 * p)) & 0xffffu] = (T)Kr[p];
@@ -493,26 +516,26 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int j = 0; j < D; ++j) db.x[0][j] = s_xr[(db.c[0] - r0) * D + j];
         double Kr[B];
-        rot_row<D, FORM>(db, c, Kr);
+        rot_row<D, FORM, G>(db, c, Kr);
         const uint64_t w = rotate_dest<B>(wb, ib);
 #pragma unroll
         for (int p = 0; p < B; ++p) s_c[(w >> (16 * p)) & 0xffffu] = (T)Kr[p];
       }
     } else {
-      ElemData<D> da, db;
-      if (ha) load_conn<D>(conn, ea / B, da);
-      if (hb) load_conn<D>(conn, eb / B, db);
-      if (ha) load_geometry<T, D, FORM>(pts, coeff, ea / B, da);
-      if (hb) load_geometry<T, D, FORM>(pts, coeff, eb / B, db);
+      ElemData<D, G> da, db;
+      if (ha) load_conn<D, G>(conn, ea / B, da);
+      if (hb) load_conn<D, G>(conn, eb / B, db);
+      if (ha) load_geometry<T, D, FORM, G>(pts, coeff, ea / B, da);
+      if (hb) load_geometry<T, D, FORM, G>(pts, coeff, eb / B, db);
       if (ha) {
         double Kr[B];
-        row_from<D, FORM>(da, c, ia, Kr);
+        row_from<D, FORM, G>(da, c, ia, Kr);
 #pragma unroll
         for (int j = 0; j < B; ++j) s_c[(wa >> (16 * j)) & 0xffffu] = (T)Kr[j];
       }
       if (hb) {
         double Kr[B];
-        row_from<D, FORM>(db, c, ib, Kr);
+        row_from<D, FORM, G>(db, c, ib, Kr);
 #pragma unroll
         for (int j = 0; j < B; ++j) s_c[(wb >> (16 * j)) & 0xffffu] = (T)Kr[j];
       }
@@ -534,8 +557,8 @@ __global__ void __launch_bounds__(kBlock)
     // loads that depend only on the block bounds go out first
     stage_row_maps(row_ptr, inc_ptr, bb, s_rp, s_ip);
     if (kRot && (int32_t)threadIdx.x < n_rows) {
-      double xr[D];
-      load_node<T, D>(pts, r0 + (int32_t)threadIdx.x, xr);
+      G xr[D];
+      load_node<T, D, G>(pts, r0 + (int32_t)threadIdx.x, xr);
 #pragma unroll
       for (int j = 0; j < D; ++j) s_xr[threadIdx.x * D + j] = xr[j];
     }
@@ -553,18 +576,18 @@ __global__ void __launch_bounds__(kBlock)
     const uint32_t pb = hb ? colpos[t0 + tb] : 0u;
     const int ia = ea % B, ib = eb % B;
     if constexpr (kRot) {
-      RotData<D> da, db;
-      if (ha) load_rot<T, D, FORM>(conn, coeff, ea / B, ia, da);
-      if (hb) load_rot<T, D, FORM>(conn, coeff, eb / B, ib, db);
-      if (ha) gather_rot<T, D>(pts, da);
-      if (hb) gather_rot<T, D>(pts, db);
+      RotData<D, G> da, db;
+      if (ha) load_rot<T, D, FORM, G>(conn, coeff, ea / B, ia, da);
+      if (hb) load_rot<T, D, FORM, G>(conn, coeff, eb / B, ib, db);
+      if (ha) gather_rot<T, D, G>(pts, da);
+      if (hb) gather_rot<T, D, G>(pts, db);
       __syncthreads();  // s_xr, s_rp, s_ip
       build_entry_rows(s_rp, n_rows, s_erow);
       if (ha) {
 #pragma unroll
         for (int j = 0; j < D; ++j) da.x[0][j] = s_xr[(da.c[0] - r0) * D + j];
         double Kr[B];
-        rot_row<D, FORM>(da, c, Kr);
+        rot_row<D, FORM, G>(da, c, Kr);
 #pragma unroll
         for (int p = 0; p < B; ++p) s_val[ta * B + p] = (T)Kr[p];
         s_pos[ta] = rotate_pos<B>(pa, ia);
@@ -573,29 +596,29 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
         for (int j = 0; j < D; ++j) db.x[0][j] = s_xr[(db.c[0] - r0) * D + j];
         double Kr[B];
-        rot_row<D, FORM>(db, c, Kr);
+        rot_row<D, FORM, G>(db, c, Kr);
 #pragma unroll
         for (int p = 0; p < B; ++p) s_val[tb * B + p] = (T)Kr[p];
         s_pos[tb] = rotate_pos<B>(pb, ib);
       }
     } else {
-      ElemData<D> da, db;
-      if (ha) load_conn<D>(conn, ea / B, da);
-      if (hb) load_conn<D>(conn, eb / B, db);
-      if (ha) load_geometry<T, D, FORM>(pts, coeff, ea / B, da);
-      if (hb) load_geometry<T, D, FORM>(pts, coeff, eb / B, db);
+      ElemData<D, G> da, db;
+      if (ha) load_conn<D, G>(conn, ea / B, da);
+      if (hb) load_conn<D, G>(conn, eb / B, db);
+      if (ha) load_geometry<T, D, FORM, G>(pts, coeff, ea / B, da);
+      if (hb) load_geometry<T, D, FORM, G>(pts, coeff, eb / B, db);
       __syncthreads();  // s_rp, s_ip
       build_entry_rows(s_rp, n_rows, s_erow);
       if (ha) {
         double Kr[B];
-        row_from<D, FORM>(da, c, ia, Kr);
+        row_from<D, FORM, G>(da, c, ia, Kr);
 #pragma unroll
         for (int j = 0; j < B; ++j) s_val[ta * B + j] = (T)Kr[j];
         s_pos[ta] = pa;
       }
       if (hb) {
         double Kr[B];
-        row_from<D, FORM>(db, c, ib, Kr);
+        row_from<D, FORM, G>(db, c, ib, Kr);
 #pragma unroll
         for (int j = 0; j < B; ++j) s_val[tb * B + j] = (T)Kr[j];
         s_pos[tb] = pb;
@@ -718,6 +741,7 @@ int launch(const tm_mesh_view* m, const tm_form_desc* f, const tm_rows_view* rv,
     for (int i = 0; i < B; ++i)
       for (int j = 0; j < B; ++j) c.mref[i * B + j] += w * phi[i] * phi[j];
   }
+  c.redo = (f->flags & TM_FLAG_F32_NO_REDO) ? 0 : 1;
   const T* pts = static_cast<const T*>(m->points);
   const T* coeff = static_cast<const T*>(f->coeff);
   T* out = static_cast<T*>(values);

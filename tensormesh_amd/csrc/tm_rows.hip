@@ -141,6 +141,35 @@ __device__ __forceinline__ void load_rot(const int32_t* __restrict__ conn, const
   if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) d.k = (G)coeff[e];
 }
 
+// Connectivity word(s) of element e: one int4 for tetrahedra, three ints
+// for triangles (loads only, so that a thread can issue those of several
+// elements before it selects from any of them).  Named fields rather than an
+// array: the rotation below selects among them, and an array indexed by the
+// runtime i would be placed in scratch memory.
+template <int D>
+__device__ __forceinline__ int4 load_conn_word(const int32_t* __restrict__ conn, int64_t e) {
+  if constexpr (D == 3) {
+    return reinterpret_cast<const int4*>(conn)[e];
+  } else {
+    return make_int4(conn[e * 3 + 0], conn[e * 3 + 1], conn[e * 3 + 2], 0);
+  }
+}
+
+// The rotated view of incidence (e, i) from its loaded connectivity word.
+template <int D, typename G>
+__device__ __forceinline__ void rotate_conn(const int4 q, int i, RotData<D, G>& d) {
+  if constexpr (D == 3) {
+    d.c[0] = i == 0 ? q.x : i == 1 ? q.y : i == 2 ? q.z : q.w;
+    d.c[1] = i == 0 ? q.y : i == 1 ? q.z : i == 2 ? q.w : q.x;
+    d.c[2] = i == 0 ? q.z : i == 1 ? q.w : i == 2 ? q.x : q.y;
+    d.c[3] = i == 0 ? q.w : i == 1 ? q.x : i == 2 ? q.y : q.z;
+  } else {
+    d.c[0] = i == 0 ? q.x : i == 1 ? q.y : q.z;
+    d.c[1] = i == 0 ? q.y : i == 1 ? q.z : q.x;
+    d.c[2] = i == 0 ? q.z : i == 1 ? q.x : q.y;
+  }
+}
+
 template <typename T, int D, typename G>
 __device__ __forceinline__ void gather_rot(const T* __restrict__ pts, RotData<D, G>& d) {
 #pragma unroll
@@ -480,28 +509,43 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
       for (int j = 0; j < D; ++j) s_xr[threadIdx.x * D + j] = xr[j];
     }
+    if (n_inc == 0 || n_ent == 0) return;  // rows of unused nodes only (uniform per workgroup)
     const int32_t ta = threadIdx.x, tb = threadIdx.x + kBlock;
     const bool ha = ta < n_inc, hb = tb < n_inc;
-    const int32_t ea = ha ? inc[t0 + ta] : 0;
-    const int32_t eb = hb ? inc[t0 + tb] : 0;
-    const uint64_t wa = ha ? dest[t0 + ta] : 0ull;
-    const uint64_t wb = hb ? dest[t0 + tb] : 0ull;
+    // every load below is unconditional (indices clamped into the run, a
+    // run has at least one incidence and one entry), so that they issue
+    // together instead of one exec-masked branch and wait each
+    const int32_t ea = inc[t0 + min(ta, n_inc - 1)];
+    const int32_t eb = inc[t0 + min(tb, n_inc - 1)];
+    const uint64_t wa = dest[t0 + min(ta, n_inc - 1)];
+    const uint64_t wb = dest[t0 + min(tb, n_inc - 1)];
     // the thread's entry ranges, loaded ahead of phase 1
     constexpr int kEpt = kMaxEnt / kBlock;
     int32_t cs[kEpt], ce[kEpt];
 #pragma unroll
     for (int u = 0; u < kEpt; ++u) {
       const int32_t kl = (int32_t)threadIdx.x + u * kBlock;
-      cs[u] = kl < n_ent ? (int32_t)ent_start[k0 + kl] : 0;
-      ce[u] = kl + 1 < n_ent ? (int32_t)ent_start[k0 + kl + 1] : n_inc * B;
+      const int32_t s0 = (int32_t)ent_start[k0 + min(kl, n_ent - 1)];
+      const int32_t s1 = (int32_t)ent_start[k0 + min(kl + 1, n_ent - 1)];
+      cs[u] = s0;
+      ce[u] = kl + 1 < n_ent ? s1 : n_inc * B;
     }
     const int ia = ea % B, ib = eb % B;
+    // both incidences' loads are issued unconditionally (element 0 stands in
+    // for a missing incidence), so that neither waits for the other's
+    const int64_t ga = ea / B, gb = eb / B;
     if constexpr (kRot) {
       RotData<D, G> da, db;
-      if (ha) load_rot<T, D, FORM, G>(conn, coeff, ea / B, ia, da);
-      if (hb) load_rot<T, D, FORM, G>(conn, coeff, eb / B, ib, db);
-      if (ha) gather_rot<T, D, G>(pts, da);
-      if (hb) gather_rot<T, D, G>(pts, db);
+      const int4 wca = load_conn_word<D>(conn, ga);
+      const int4 wcb = load_conn_word<D>(conn, gb);
+      if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) {
+        da.k = (G)coeff[ga];
+        db.k = (G)coeff[gb];
+      }
+      rotate_conn<D, G>(wca, ia, da);
+      rotate_conn<D, G>(wcb, ib, db);
+      gather_rot<T, D, G>(pts, da);
+      gather_rot<T, D, G>(pts, db);
       __syncthreads();  // s_xr
       if (ha) {
 #pragma unroll
@@ -523,10 +567,10 @@ __global__ void __launch_bounds__(kBlock)
       }
     } else {
       ElemData<D, G> da, db;
-      if (ha) load_conn<D, G>(conn, ea / B, da);
-      if (hb) load_conn<D, G>(conn, eb / B, db);
-      if (ha) load_geometry<T, D, FORM, G>(pts, coeff, ea / B, da);
-      if (hb) load_geometry<T, D, FORM, G>(pts, coeff, eb / B, db);
+      load_conn<D, G>(conn, ga, da);
+      load_conn<D, G>(conn, gb, db);
+      load_geometry<T, D, FORM, G>(pts, coeff, ga, da);
+      load_geometry<T, D, FORM, G>(pts, coeff, gb, db);
       if (ha) {
         double Kr[B];
         row_from<D, FORM, G>(da, c, ia, Kr);
@@ -541,13 +585,22 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
     __syncthreads();
-    // phase 2: entry kl sums its contributions in list order (ascending element)
+    // phase 2: entry kl sums its contributions in list order (ascending
+    // element), four LDS reads in flight, the adds in order
 #pragma unroll
     for (int u = 0; u < kEpt; ++u) {
       const int32_t kl = (int32_t)threadIdx.x + u * kBlock;
       if (kl < n_ent) {
         T acc = T(0);
-        for (int32_t t = cs[u]; t < ce[u]; ++t) acc += s_c[t];
+        int32_t t = cs[u];
+        for (; t + 3 < ce[u]; t += 4) {
+          const T v0 = s_c[t], v1 = s_c[t + 1], v2 = s_c[t + 2], v3 = s_c[t + 3];
+          acc += v0;
+          acc += v1;
+          acc += v2;
+          acc += v3;
+        }
+        for (; t < ce[u]; ++t) acc += s_c[t];
         values[k0 + kl] = acc;
       }
     }

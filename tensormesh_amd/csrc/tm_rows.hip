@@ -318,12 +318,14 @@ __device__ __forceinline__ void row_from(const ElemData<D, G>& d, const RowConst
   }
 }
 
-// Row i of the local matrix of element e (all three load steps in sequence).
+// Row i of the local matrix of element e (all three load steps in sequence),
+// geometry in double: only runs too large for LDS take this walk, and a
+// single geometry type keeps its registers out of the staged paths' budget.
 template <typename T, int D, int FORM>
 __device__ __forceinline__ void local_row(const int32_t* __restrict__ conn, const T* __restrict__ pts,
                                           const T* __restrict__ coeff, const RowConsts& c, int64_t e, int i,
                                           double (&Kr)[D + 1]) {
-  using G = Geom<T>;
+  using G = double;
   ElemData<D, G> d;
   load_conn<D, G>(conn, e, d);
   load_geometry<T, D, FORM, G>(pts, coeff, e, d);
@@ -475,29 +477,70 @@ __device__ __forceinline__ void stage_row_maps(const int32_t* __restrict__ row_p
   }
 }
 
+// Entries [k0, k1) of rows [r0, r1) straight from global memory, each entry
+// walking its row's incidences and recomputing the contributing local rows
+// (runs too large to stage in LDS).
 template <typename T, int D, int FORM>
-__global__ void __launch_bounds__(kBlock)
-    p1_rows_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
-                   RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
-                   const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
-                   const uint32_t* __restrict__ entry_mask, const int32_t* __restrict__ blk_ptr,
-                   const int32_t* __restrict__ blk_info, int64_t n_blocks, const uint64_t* __restrict__ dest,
-                   const uint16_t* __restrict__ ent_start, T* __restrict__ values) {
+__device__ void walk_entries(const int32_t* __restrict__ conn, const T* __restrict__ pts,
+                             const T* __restrict__ coeff, const RowConsts& c, const int32_t* __restrict__ row_ptr,
+                             const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+                             const uint32_t* __restrict__ colpos, int32_t r0, int32_t r1, int32_t k0, int32_t k1,
+                             T* __restrict__ values) {
+  constexpr int B = D + 1;
+  for (int32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
+    int32_t lo = r0, hi = r1 - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (row_ptr[mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    const int32_t r = lo;
+    const uint32_t p = (uint32_t)(k - row_ptr[r]);
+    T acc = T(0);
+    for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
+      const uint32_t cp = colpos[t];
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        if (((cp >> (8 * j)) & 0xffu) == p) {
+          const int32_t ei = inc[t];
+          double Kr[B];
+          local_row<T, D, FORM>(conn, pts, coeff, c, ei / B, ei % B, Kr);
+          acc += (T)Kr[j];
+        }
+      }
+    }
+    values[k] = acc;
+  }
+}
+
+// The fused kernel with the contribution order of tm_rows_dest_build: phase 1
+// writes each local-row value at its position in the workgroup's
+// contribution list (entry by entry, ascending element within an entry),
+// phase 2 sums every entry's contiguous range.  Its own kernel so that it
+// allocates only the LDS it uses (the staged values and the rows' node
+// coordinates), which lets more workgroups share a CU.
+// float32: registers capped for seven workgroups per CU (the float geometry
+// and its double redo fit in 72 VGPRs with a few spills; measured 52.7 ->
+// 48.0 us); float64 keeps the compiler's allocation (capped, it spills on
+// the hot path: 59.9 -> 71.5 us)
+template <typename T, int D, int FORM>
+__global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 7 : 1)
+    p1_rows_dest_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
+                        RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
+                        const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
+                        const int32_t* __restrict__ blk_ptr, const int32_t* __restrict__ blk_info, int64_t n_blocks,
+                        const uint64_t* __restrict__ dest, const uint16_t* __restrict__ ent_start,
+                        T* __restrict__ values) {
   constexpr int B = D + 1;
   static_assert(kCapInc == 2 * kBlock, "phase 1 stages two incidences per thread");
-  __shared__ T s_val[kCapInc * B];
-  __shared__ uint32_t s_pos[kCapInc];
-  __shared__ int32_t s_rp[kMaxRows + 1];    // row starts, relative to k0
-  __shared__ int32_t s_ip[kMaxRows + 1];    // incidence starts, relative to t0
-  __shared__ uint16_t s_erow[kMaxEnt];      // entry -> local row
   using G = Geom<T>;
-  __shared__ G s_xr[kMaxRows * D];          // coordinates of the staged rows' nodes
+  __shared__ T s_val[kCapInc * B];
+  __shared__ G s_xr[kMaxRows * D];  // coordinates of the staged rows' nodes
   const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
   const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
   const int32_t r0 = bb.r0, r1 = bb.r1, t0 = bb.t0, k0 = bb.k0, k1 = bb.k1;
   const int32_t n_inc = bb.t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
   constexpr bool kRot = FORM == TM_FORM_LAPLACE || FORM == TM_FORM_DIFFUSION_ELEM;
-  if (dest && n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
+  if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
     // contribution-ordered staging: s_val is the workgroup's contribution
     // list (entry by entry, ascending element within an entry); phase 1
     // writes each local-row value at its list position, phase 2 sums every
@@ -606,6 +649,31 @@ __global__ void __launch_bounds__(kBlock)
     }
     return;
   }
+  // oversized run: every entry walks global memory, recomputing local rows
+  walk_entries<T, D, FORM>(conn, pts, coeff, c, row_ptr, inc_ptr, inc, colpos, r0, r1, k0, k1, values);
+}
+
+template <typename T, int D, int FORM>
+__global__ void __launch_bounds__(kBlock)
+    p1_rows_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
+                   RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
+                   const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
+                   const uint32_t* __restrict__ entry_mask, const int32_t* __restrict__ blk_ptr,
+                   const int32_t* __restrict__ blk_info, int64_t n_blocks, T* __restrict__ values) {
+  constexpr int B = D + 1;
+  static_assert(kCapInc == 2 * kBlock, "phase 1 stages two incidences per thread");
+  __shared__ T s_val[kCapInc * B];
+  __shared__ uint32_t s_pos[kCapInc];
+  __shared__ int32_t s_rp[kMaxRows + 1];    // row starts, relative to k0
+  __shared__ int32_t s_ip[kMaxRows + 1];    // incidence starts, relative to t0
+  __shared__ uint16_t s_erow[kMaxEnt];      // entry -> local row
+  using G = Geom<T>;
+  __shared__ G s_xr[kMaxRows * D];          // coordinates of the staged rows' nodes
+  const int64_t blk = xcd_logical(blockIdx.x, n_blocks);
+  const BlockBounds bb = block_bounds(blk_info, blk_ptr, inc_ptr, row_ptr, blk);
+  const int32_t r0 = bb.r0, r1 = bb.r1, t0 = bb.t0, k0 = bb.k0, k1 = bb.k1;
+  const int32_t n_inc = bb.t1 - t0, n_rows = r1 - r0, n_ent = k1 - k0;
+  constexpr bool kRot = FORM == TM_FORM_LAPLACE || FORM == TM_FORM_DIFFUSION_ELEM;
   if (n_inc <= kCapInc && n_rows <= kMaxRows && n_ent <= kMaxEnt) {
     // loads that depend only on the block bounds go out first
     stage_row_maps(row_ptr, inc_ptr, bb, s_rp, s_ip);
@@ -683,29 +751,7 @@ __global__ void __launch_bounds__(kBlock)
     return;
   }
   // oversized run: every entry walks global memory, recomputing local rows
-  for (int32_t k = k0 + threadIdx.x; k < k1; k += kBlock) {
-    int32_t lo = r0, hi = r1 - 1;
-    while (lo < hi) {
-      const int32_t mid = (lo + hi + 1) >> 1;
-      if (row_ptr[mid] <= k) lo = mid; else hi = mid - 1;
-    }
-    const int32_t r = lo;
-    const uint32_t p = (uint32_t)(k - row_ptr[r]);
-    T acc = T(0);
-    for (int32_t t = inc_ptr[r]; t < inc_ptr[r + 1]; ++t) {
-      const uint32_t cp = colpos[t];
-#pragma unroll
-      for (int j = 0; j < B; ++j) {
-        if (((cp >> (8 * j)) & 0xffu) == p) {
-          const int32_t ei = inc[t];
-          double Kr[B];
-          local_row<T, D, FORM>(conn, pts, coeff, c, ei / B, ei % B, Kr);
-          acc += (T)Kr[j];
-        }
-      }
-    }
-    values[k] = acc;
-  }
+  walk_entries<T, D, FORM>(conn, pts, coeff, c, row_ptr, inc_ptr, inc, colpos, r0, r1, k0, k1, values);
 }
 
 // Row-centric stage S for P1 slot buffers: the same staging and per-entry walk
@@ -801,10 +847,17 @@ int launch(const tm_mesh_view* m, const tm_form_desc* f, const tm_rows_view* rv,
   const int64_t nb = rv->n_blocks;
   if (nb == 0) return TM_OK;
   const uint64_t* dest = (rv->dest && rv->ent_start) ? reinterpret_cast<const uint64_t*>(rv->dest) : nullptr;
-#define TM_ROWS_LAUNCH(FORM)                                                                              \
-  p1_rows_kernel<T, D, FORM><<<nb, kBlock, 0, s>>>(m->conn, pts, coeff, c, rv->row_ptr, rv->inc_ptr, rv->inc, \
-                                                   rv->colpos, rv->entry_mask, rv->blk_ptr, rv->blk_info, nb,     \
-                                                   dest, rv->ent_start, out)
+#define TM_ROWS_LAUNCH(FORM)                                                                                \
+  do {                                                                                                      \
+    if (dest)                                                                                               \
+      p1_rows_dest_kernel<T, D, FORM><<<nb, kBlock, 0, s>>>(m->conn, pts, coeff, c, rv->row_ptr, rv->inc_ptr, \
+                                                            rv->inc, rv->colpos, rv->blk_ptr, rv->blk_info, nb, \
+                                                            dest, rv->ent_start, out);                     \
+    else                                                                                                    \
+      p1_rows_kernel<T, D, FORM><<<nb, kBlock, 0, s>>>(m->conn, pts, coeff, c, rv->row_ptr, rv->inc_ptr,       \
+                                                       rv->inc, rv->colpos, rv->entry_mask, rv->blk_ptr,        \
+                                                       rv->blk_info, nb, out);                                  \
+  } while (0)
   switch (f->form) {
     case TM_FORM_LAPLACE: TM_ROWS_LAUNCH(TM_FORM_LAPLACE); break;
     case TM_FORM_DIFFUSION_ELEM:

@@ -518,12 +518,12 @@ __device__ void walk_entries(const int32_t* __restrict__ conn, const T* __restri
 // phase 2 sums every entry's contiguous range.  Its own kernel so that it
 // allocates only the LDS it uses (the staged values and the rows' node
 // coordinates), which lets more workgroups share a CU.
-// float32: registers capped for seven workgroups per CU (the float geometry
-// and its double redo fit in 72 VGPRs with a few spills; measured 52.7 ->
-// 48.0 us); float64 keeps the compiler's allocation (capped, it spills on
-// the hot path: 59.9 -> 71.5 us)
+// Built for seven workgroups per CU (72 VGPRs) except for the nodal
+// coefficient form, which would spill: the float32 instances went from 52.7
+// to 48.0 us with it; float64 went 59.9 -> 71.5 us while its hot path
+// spilled, and fits without spills since the entry ranges are packed.
 template <typename T, int D, int FORM>
-__global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 7 : 1)
+__global__ void __launch_bounds__(kBlock, FORM == TM_FORM_DIFFUSION_NODE ? 1 : 7)
     p1_rows_dest_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
                         RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
                         const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,
@@ -564,14 +564,16 @@ __global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 7 : 1)
     const uint64_t wb = dest[t0 + min(tb, n_inc - 1)];
     // the thread's entry ranges, loaded ahead of phase 1
     constexpr int kEpt = kMaxEnt / kBlock;
-    int32_t cs[kEpt], ce[kEpt];
+    // (the first two per thread, packed start | end << 16: runs average about
+    // 320 entries; the third and fourth are loaded in phase 2 when present)
+    constexpr int kEarly = 2;
+    uint32_t rng[kEarly];
 #pragma unroll
-    for (int u = 0; u < kEpt; ++u) {
+    for (int u = 0; u < kEarly; ++u) {
       const int32_t kl = (int32_t)threadIdx.x + u * kBlock;
-      const int32_t s0 = (int32_t)ent_start[k0 + min(kl, n_ent - 1)];
-      const int32_t s1 = (int32_t)ent_start[k0 + min(kl + 1, n_ent - 1)];
-      cs[u] = s0;
-      ce[u] = kl + 1 < n_ent ? s1 : n_inc * B;
+      const uint32_t s0 = ent_start[k0 + min(kl, n_ent - 1)];
+      const uint32_t s1 = ent_start[k0 + min(kl + 1, n_ent - 1)];
+      rng[u] = s0 | ((kl + 1 < n_ent ? s1 : (uint32_t)(n_inc * B)) << 16);
     }
     const int ia = ea % B, ib = eb % B;
     // both incidences' loads are issued unconditionally (element 0 stands in
@@ -634,16 +636,23 @@ __global__ void __launch_bounds__(kBlock, sizeof(T) == 4 ? 7 : 1)
     for (int u = 0; u < kEpt; ++u) {
       const int32_t kl = (int32_t)threadIdx.x + u * kBlock;
       if (kl < n_ent) {
+        int32_t t, te;
+        if (u < kEarly) {
+          t = (int32_t)(rng[u < kEarly ? u : 0] & 0xffffu);
+          te = (int32_t)(rng[u < kEarly ? u : 0] >> 16);
+        } else {
+          t = ent_start[k0 + kl];
+          te = kl + 1 < n_ent ? (int32_t)ent_start[k0 + kl + 1] : n_inc * B;
+        }
         T acc = T(0);
-        int32_t t = cs[u];
-        for (; t + 3 < ce[u]; t += 4) {
+        for (; t + 3 < te; t += 4) {
           const T v0 = s_c[t], v1 = s_c[t + 1], v2 = s_c[t + 2], v3 = s_c[t + 3];
           acc += v0;
           acc += v1;
           acc += v2;
           acc += v3;
         }
-        for (; t < ce[u]; ++t) acc += s_c[t];
+        for (; t < te; ++t) acc += s_c[t];
         values[k0 + kl] = acc;
       }
     }

@@ -27,6 +27,16 @@ __device__ __forceinline__ int fill_pairs(uint16_t* pairs, bool upper) {
   return B * (B + 1) / 2;
 }
 
+// Bijection from the hardware dispatch order (workgroup b of a launch runs on
+// XCD b mod 8) to a logical order in which every XCD owns one contiguous
+// range of [0, nb): neighbouring logical workgroups then share that XCD's L2,
+// and so do launches that map the same logical workgroup to the same rows.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t q = nb / 8, r = nb % 8;
+  const int64_t x = b % 8, y = b / 8;
+  return x < r ? x * (q + 1) + y : r * (q + 1) + (x - r) * q + y;
+}
+
 // Float32 geometry of a P1 simplex, and when it must be redone in float64.
 // With edges e_k = x_{k+1} - x_0 and det = det[e_0 .. e_{D-1}], the shape
 // quality q = |det| / prod_k |e_k| (the volume over that of the box on the

@@ -13,13 +13,18 @@
 //     iterations can be captured in one graph;
 //   * a system stops updating once |r| <= rtol |b| (its active flag clears),
 //     the host only polls between chunks;
-//   * SpMV rows are summed in ascending entry order (CSR-stream through LDS).
+//   * SpMV rows are summed in ascending entry order (CSR-stream through LDS);
+//   * logical workgroup b (rows, partial-sum slot) runs on XCD-contiguous
+//     hardware workgroups (tmh::xcd_block) in every kernel, so each XCD keeps
+//     one contiguous share of the vectors in its L2 from launch to launch
+//     and the SpMV's column gathers stay mostly within that share.
 // Two iteration variants: the textbook one, five launches per iteration with
 // scalars sc[s*8 + {0:rz, 1:pq, 2:alpha, 3:beta, 4:rr, 5:bb, 6:active,
 // 7:iterations}], and a two-launch Chronopoulos-Gear one (described where its
 // kernels are defined).
 #include <hip/hip_runtime.h>
 
+#include "tm_device.h"
 #include "tm_internal.h"
 
 // No implicit contraction of a * b + c in this file: a rounding step should
@@ -173,7 +178,7 @@ __global__ void __launch_bounds__(kBlock)
                     double* __restrict__ part) {
   __shared__ double sh[kBlock];
   __shared__ T s_prod[kChunk];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
   const T ax = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
   const int32_t i = row0 + threadIdx.x;
@@ -207,7 +212,7 @@ __global__ void __launch_bounds__(kBlock)
                     const T* __restrict__ av, const T* __restrict__ p, T* __restrict__ q,
                     const double* __restrict__ sc, double* __restrict__ part) {
   __shared__ double sh[kBlock];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t s = blk_sys[blk];
   if (sc[s * kNS + kActive] == 0.0) return;  // uniform per workgroup
   __shared__ T s_prod[kChunk];
@@ -232,7 +237,7 @@ __global__ void __launch_bounds__(kBlock)
                       const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                       const double* __restrict__ sc, double* __restrict__ part) {
   __shared__ double sh[kBlock];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t s = blk_sys[blk];
   if (sc[s * kNS + kActive] == 0.0) return;
   const T alpha = (T)sc[s * kNS + kAlpha];
@@ -261,7 +266,7 @@ __global__ void __launch_bounds__(kBlock)
     pcg_dir_kernel(const int32_t* __restrict__ blk_sys, const int32_t* __restrict__ blk_row0,
                    const int32_t* __restrict__ blk_row1, const T* __restrict__ z, T* __restrict__ p,
                    const double* __restrict__ sc) {
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t s = blk_sys[blk];
   if (sc[s * kNS + kActive] == 0.0) return;
   const T beta = (T)sc[s * kNS + kBeta];
@@ -394,7 +399,7 @@ __global__ void __launch_bounds__(kBlock)
                     double* __restrict__ sc, int32_t nb) {
   __shared__ double sh[kBlock];
   __shared__ T s_prod[kChunk];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t sy = blk_sys[blk];
   const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
   const T ax = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
@@ -439,7 +444,7 @@ __global__ void __launch_bounds__(kBlock)
                     const T* __restrict__ u, T* __restrict__ w, double* __restrict__ part, double* __restrict__ sc,
                     int32_t nb, double tol2, int first) {
   __shared__ double sh[kBlock];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t sy = blk_sys[blk];
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
   double* c = sc + sy * kNS2;
@@ -483,7 +488,7 @@ __global__ void __launch_bounds__(kBlock)
                    T* __restrict__ u, T* __restrict__ p, T* __restrict__ sv, double* __restrict__ part,
                    double* __restrict__ sc, int32_t nb, int parity) {
   __shared__ double sh[kBlock];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t sy = blk_sys[blk];
   double* c = sc + sy * kNS2;
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
@@ -585,7 +590,7 @@ __global__ void __launch_bounds__(kBlock)
                     int32_t nb) {
   __shared__ double sh[3 * (kBlock / 64)];
   __shared__ T s_prod[kChunk];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t sy = blk_sys[blk];
   const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
   const T ax = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
@@ -625,7 +630,7 @@ __global__ void __launch_bounds__(kBlock)
                      double* __restrict__ part, double* __restrict__ sc, int32_t nb, double tol2) {
   __shared__ double sh[2 * (kBlock / 64)];
   __shared__ T s_prod[kChunk];
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t sy = blk_sys[blk];
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
   double* c = sc + sy * kNS2;
@@ -679,7 +684,7 @@ __global__ void __launch_bounds__(kBlock)
   __shared__ double sh[3 * (kBlock / 64)];
   __shared__ T s_prod[kChunk];
   __shared__ int s_run;
-  const int32_t blk = blockIdx.x;
+  const int32_t blk = (int32_t)tmh::xcd_block(blockIdx.x, gridDim.x);
   const int32_t sy = blk_sys[blk];
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
   double* c = sc + sy * kNS2;
@@ -829,7 +834,7 @@ __global__ void __launch_bounds__(kBlock)
                 const T* __restrict__ x, T* __restrict__ y, int64_t n_rows) {
   // 256 rows per workgroup, CSR-stream
   __shared__ T s_prod[kChunk];
-  const int64_t r0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t r0 = tmh::xcd_block(blockIdx.x, gridDim.x) * kBlock;
   const int32_t row0 = (int32_t)r0, row1 = (int32_t)(r0 + kBlock < n_rows ? r0 + kBlock : n_rows);
   const T v = rows_spmv<T>(rp, ci, av, x, row0, row1, s_prod);
   const int32_t i = row0 + threadIdx.x;

@@ -18,8 +18,9 @@
  *    gives a thread-local message.  No C++ exception crosses the ABI.
  *  - Indices are int32; entry points reject sizes that do not fit.
  *  - dtype TM_F64: points, coefficients and values are double.  TM_F32: they
- *    are float; element geometry (edge vectors, cofactors, determinant) is
- *    still evaluated in double, contraction and accumulation in float.
+ *    are float; the P1 scalar kernels evaluate element geometry in float and
+ *    redo slivers in double (TM_FLAG_F32_NO_REDO), the other kernels evaluate
+ *    geometry in double; contraction and accumulation are in float.
  */
 #ifndef TENSORMESH_AMD_H
 #define TENSORMESH_AMD_H

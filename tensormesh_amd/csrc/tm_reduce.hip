@@ -129,19 +129,44 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
     return;
   }
+  // phase-2 lane mapping (below), decided and its loads issued up front:
+  // they depend only on the run's bounds, not on the staged values
+  constexpr int kD = HH == 9 ? 3 : HH == 4 ? 2 : 1;
+  constexpr int L = kD > 1 ? kD : HH;  // lanes per entry in the lane-group mode
+  constexpr int kPerPass = kBlock / L;
+  // more than one pass per lane group measured slower for float32 slots
+  // (whose 16 KB runs hold twice the entries) than a thread per entry
+  constexpr int kMaxPass = sizeof(T) == 8 ? 4 : 1;
+  const bool groups = HH > 1 && ke - kb <= kMaxPass * kPerPass;
+  const int32_t le = (int32_t)threadIdx.x / L, l = (int32_t)threadIdx.x - le * L;
+  int32_t ra[kMaxPass], rz[kMaxPass];
+  int64_t d0[kMaxPass], d1[kMaxPass];
+#pragma unroll
+  for (int ps = 0; ps < kMaxPass; ++ps) {
+    const int32_t k = kb + le + ps * kPerPass;
+    const bool has = groups && le < kPerPass && k < ke;
+    ra[ps] = has ? seg_ptr[k] - cb : 0;
+    rz[ps] = has ? seg_ptr[k + 1] - cb : 0;
+    d0[ps] = !has ? -1 : entry ? (int64_t)entry[k] : (int64_t)k;
+    d1[ps] = (has && entry) ? (int64_t)entry_t[k] : -1;
+  }
   if (!seg_slot) {
-    // contiguous contributions: the run's slots are one contiguous range
+    // contiguous contributions: the run's slots are one contiguous range,
+    // every value of this thread loaded before any is stored
     const T* src = slot + (int64_t)cb * HH;
     const int32_t nv = n * HH;
-    int32_t u = threadIdx.x;
-    for (; u + 3 * kBlock < nv; u += 4 * kBlock) {
-      const T v0 = src[u], v1 = src[u + kBlock], v2 = src[u + 2 * kBlock], v3 = src[u + 3 * kBlock];
-      lds[u] = v0;
-      lds[u + kBlock] = v1;
-      lds[u + 2 * kBlock] = v2;
-      lds[u + 3 * kBlock] = v3;
+    constexpr int kPerThread = (kCap * HH + kBlock - 1) / kBlock;
+    T v[kPerThread];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      const int32_t u = (int32_t)threadIdx.x + j * kBlock;
+      v[j] = u < nv ? src[u] : T(0);
     }
-    for (; u < nv; u += kBlock) lds[u] = src[u];
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+      const int32_t u = (int32_t)threadIdx.x + j * kBlock;
+      if (u < nv) lds[u] = v[j];
+    }
   } else if constexpr (HH == 1) {
     int32_t t = threadIdx.x;
     // four independent gathers per iteration keep the memory pipeline full
@@ -184,13 +209,7 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
   }
   __syncthreads();
-  constexpr int kD = HH == 9 ? 3 : HH == 4 ? 2 : 1;
-  constexpr int L = kD > 1 ? kD : HH;  // lanes per entry below
-  constexpr int kPerPass = kBlock / L;
-  // more than one pass per lane group measured slower for float32 slots
-  // (whose 16 KB runs hold twice the entries) than a thread per entry
-  constexpr int kMaxPass = sizeof(T) == 8 ? 4 : 1;
-  if (HH == 1 || ke - kb > kMaxPass * kPerPass) {
+  if (!groups) {
     for (int32_t k = kb + threadIdx.x; k < ke; k += kBlock) {
       const int32_t a = seg_ptr[k] - cb, z = seg_ptr[k + 1] - cb;
       T acc[HH];
@@ -204,43 +223,28 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
     }
   } else {
     // L lanes per entry (one per block row for D x D blocks), lane l summing
-    // values [l*V, l*V + V), up to kMaxPass entries per lane group with all
-    // their bounds and destinations loaded first.  The lanes of an entry read
-    // adjacent LDS words and write its HH values (and their transposed copy)
-    // as one contiguous piece, where a thread per entry stores HH-strided
-    // words from every lane.  Each value is still summed over the entry's
-    // contributions in order.
+    // values [l*V, l*V + V), up to kMaxPass entries per lane group.  The
+    // lanes of an entry read adjacent LDS words and write its HH values (and
+    // their transposed copy) as one contiguous piece, where a thread per
+    // entry stores HH-strided words from every lane.  Each value is still
+    // summed over the entry's contributions in order.
     constexpr int V = HH / L;
-    const int32_t le = (int32_t)threadIdx.x / L, l = (int32_t)threadIdx.x - le * L;
-    if (le < kPerPass) {
-      int32_t ra[kMaxPass], rz[kMaxPass];
-      int64_t d0[kMaxPass], d1[kMaxPass];
 #pragma unroll
-      for (int ps = 0; ps < kMaxPass; ++ps) {
-        const int32_t k = kb + le + ps * kPerPass;
-        const bool has = k < ke;
-        ra[ps] = has ? seg_ptr[k] - cb : 0;
-        rz[ps] = has ? seg_ptr[k + 1] - cb : 0;
-        d0[ps] = !has ? -1 : entry ? (int64_t)entry[k] : (int64_t)k;
-        d1[ps] = (has && entry) ? (int64_t)entry_t[k] : -1;
+    for (int ps = 0; ps < kMaxPass; ++ps) {
+      if (d0[ps] < 0) break;
+      T acc[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) acc[v] = T(0);
+      for (int32_t u = ra[ps]; u < rz[ps]; ++u) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) acc[v] += lds[u * HH + l * V + v];
       }
 #pragma unroll
-      for (int ps = 0; ps < kMaxPass; ++ps) {
-        if (d0[ps] < 0) break;
-        T acc[V];
+      for (int v = 0; v < V; ++v) out[d0[ps] * HH + l * V + v] = acc[v];
+      if (d1[ps] >= 0 && d1[ps] != d0[ps]) {
+        // value (l, v) of the block lands at (v, l) of the mirror
 #pragma unroll
-        for (int v = 0; v < V; ++v) acc[v] = T(0);
-        for (int32_t u = ra[ps]; u < rz[ps]; ++u) {
-#pragma unroll
-          for (int v = 0; v < V; ++v) acc[v] += lds[u * HH + l * V + v];
-        }
-#pragma unroll
-        for (int v = 0; v < V; ++v) out[d0[ps] * HH + l * V + v] = acc[v];
-        if (d1[ps] >= 0 && d1[ps] != d0[ps]) {
-          // value (l, v) of the block lands at (v, l) of the mirror
-#pragma unroll
-          for (int v = 0; v < V; ++v) out[d1[ps] * HH + (kD > 1 ? v * kD + l : l * V + v)] = acc[v];
-        }
+        for (int v = 0; v < V; ++v) out[d1[ps] * HH + (kD > 1 ? v * kD + l : l * V + v)] = acc[v];
       }
     }
   }

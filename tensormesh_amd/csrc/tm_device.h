@@ -71,4 +71,35 @@ __device__ __forceinline__ bool sliver_f32(const float (&e)[D][D], float det) {
   return !(lhs >= rhs);  // NaN (a zero edge) counts as a sliver
 }
 
+// Per-point D x D sums of one element, spread over the element's L lanes:
+//   out[q][r * D + c] = (diag && r == c) + sum_{b < B} a(q, r, b) * m(q, b, c)
+// One (q, r, c) per lane when the MQ D D entries (MQ >= Q, the instance's
+// largest rule) fit in one pass of the lanes, else one row (q, r) per lane.
+// Both take the nodes in order, so they give the same bits.
+template <int D, int B, int L, int MQ, typename A, typename M, typename Out>
+__device__ __forceinline__ void point_sums(int lane, int Q, bool diag, A a, M m, Out out) {
+  if constexpr (MQ * D * D <= L) {
+    if (lane < Q * D * D) {
+      const int q = lane / (D * D), r = (lane / D) % D, c = lane % D;
+      double s = (diag && r == c) ? 1.0 : 0.0;
+      for (int b = 0; b < B; ++b) s += a(q, r, b) * m(q, b, c);
+      out(q, r, c) = s;
+    }
+    return;
+  }
+  for (int t = lane; t < Q * D; t += L) {
+    const int q = t / D, r = t % D;
+    double s[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) s[c] = (diag && r == c) ? 1.0 : 0.0;
+    for (int b = 0; b < B; ++b) {
+      const double ab = a(q, r, b);
+#pragma unroll
+      for (int c = 0; c < D; ++c) s[c] += ab * m(q, b, c);
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) out(q, r, c) = s[c];
+  }
+}
+
 }  // namespace tmh

@@ -34,6 +34,7 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kMaxQ = 16;      // Neo-Hookean kernel
 constexpr int kMaxQWave = 32;  // element-matrix kernel (a 3 x 3 x 3 hexahedron rule has 27 points)
 
+
 // MQ: largest number of quadrature points the LDS arrays hold.  upper: only
 // the blocks i <= j (symmetric forms, TM_FLAG_UPPER); products are formed so
 // that block (j, i) would be the exact transpose of block (i, j).
@@ -78,17 +79,21 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
     }
   }
   __syncthreads();
+  // the Jacobians J[i][j] = sum_b dphi_b[i] x_b[j] (into s_inv, which
+  // point q's lane then overwrites with the inverse)
+  if (active)
+    tmh::point_sums<D, B, L, MQ>(
+        lane, Q, false, [&](int q, int i, int b) { return dphi[(q * D + i) * B + b]; },
+        [&](int, int b, int j) { return s_x[wv][b * D + j]; },
+        [&](int q, int i, int j) -> double& { return s_inv[wv][q][i * D + j]; });
+  __syncthreads();
   if (active && lane < Q) {
     const int q = lane;
     double J[D][D];
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-      for (int j = 0; j < D; ++j) {
-        double s = 0.0;
-        for (int b = 0; b < B; ++b) s += s_x[wv][b * D + j] * dphi[(q * D + i) * B + b];
-        J[i][j] = s;
-      }
+      for (int j = 0; j < D; ++j) J[i][j] = s_inv[wv][q][i * D + j];
     double inv[D][D];
     double det;
     if constexpr (D == 2) {
@@ -291,17 +296,21 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     }
   }
   __syncthreads();
+  // the Jacobians J[i][j] = sum_b dphi_b[i] x_b[j] (into s_inv, which
+  // point q's lane then overwrites with the inverse)
+  if (active)
+    tmh::point_sums<D, B, L, MQ>(
+        lane, Q, false, [&](int q, int i, int b) { return dphi[(q * D + i) * B + b]; },
+        [&](int, int b, int j) { return s_x[wv][b * D + j]; },
+        [&](int q, int i, int j) -> double& { return s_inv[wv][q][i * D + j]; });
+  __syncthreads();
   if (active && lane < Q) {
     const int q = lane;
     double J[D][D];
 #pragma unroll
     for (int i = 0; i < D; ++i)
 #pragma unroll
-      for (int j = 0; j < D; ++j) {
-        double s = 0.0;
-        for (int b = 0; b < B; ++b) s += s_x[wv][b * D + j] * dphi[(q * D + i) * B + b];
-        J[i][j] = s;
-      }
+      for (int j = 0; j < D; ++j) J[i][j] = s_inv[wv][q][i * D + j];
     double inv[D][D];
     double det;
     if constexpr (D == 2) {
@@ -343,18 +352,21 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     }
   }
   __syncthreads();
+  // the deformation gradients F[a][d] = delta_ad + sum_b u_b[a] g_b[d] (into
+  // s_Fi, which the per-point stage below overwrites with F^-1)
+  if (active)
+    tmh::point_sums<D, B, L, MQ>(
+        lane, Q, true, [&](int, int a, int b) { return s_u[wv][b * D + a]; },
+        [&](int q, int b, int d) { return s_g[wv][(q * B + b) * D + d]; },
+        [&](int q, int a, int d) -> double& { return s_Fi[wv][q][a * D + d]; });
+  __syncthreads();
   if (active && lane < Q) {
     const int q = lane;
-    const double* g = &s_g[wv][q * B * D];
     double F[D][D];
 #pragma unroll
     for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        double s = (a == d) ? 1.0 : 0.0;
-        for (int b = 0; b < B; ++b) s += s_u[wv][b * D + a] * g[b * D + d];
-        F[a][d] = s;
-      }
+      for (int d = 0; d < D; ++d) F[a][d] = s_Fi[wv][q][a * D + d];
     double Fi[D][D];
     double Jd;
     if constexpr (D == 2) {

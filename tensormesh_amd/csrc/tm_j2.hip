@@ -85,6 +85,15 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   const double* w = tb;
   const double* dphi = tb + Q + Q * B;
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
+  // the lane's tangent block destinations (contribution order), loaded now
+  // so that their latency hides behind the per-point stages
+  constexpr int kPairRounds = (B * B + L - 1) / L;
+  int32_t dst[kPairRounds];
+#pragma unroll
+  for (int r = 0; r < kPairRounds; ++r) {
+    const int p = lane + r * L;
+    dst[r] = (TAN && !update && slot_dest && active && p < n_pairs) ? slot_dest[e * n_pairs + p] : 0;
+  }
   __syncthreads();
   if (active) {
     for (int t = lane; t < B * D; t += L) {
@@ -246,8 +255,25 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __syncthreads();
   if (TAN && !update && active) {
     constexpr int DD = D * D;
-    for (int p = lane; p < n_pairs; p += L) {
-      const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
+#pragma unroll
+    for (int r = 0; r < kPairRounds; ++r) {
+      const int p = lane + r * L;
+      if (p >= n_pairs) break;
+      int i = s_pair[p] >> 8, j = s_pair[p] & 255;
+      // block p of the element's pairs (packed when upper): 72 contiguous bytes
+      // at f64; with slot_dest at its contribution-ordered slot, transposed
+      // when the id is negative -- formed as block (j, i), which is the exact
+      // transpose of (i, j) (the same products, commuted)
+      T* out = tslot + (e * n_pairs + p) * DD;
+      if (slot_dest) {
+        const int32_t d = dst[r];
+        out = tslot + (int64_t)(d & INT32_MAX) * DD;
+        if (d < 0) {
+          const int t = i;
+          i = j;
+          j = t;
+        }
+      }
       double acc[DD];
 #pragma unroll
       for (int c = 0; c < DD; ++c) acc[c] = 0.0;
@@ -272,29 +298,14 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
 #pragma unroll
         for (int a = 0; a < D; ++a)
 #pragma unroll
-          for (int b = 0; b < D; ++b)
-            acc[a * D + b] += lw * (vgi[a] * vgj[b]) + mw * (vgi[b] * vgj[a]) - cw * (vhi[a] * vhj[b]) +
-                              (a == b ? mw * dot : 0.0);
+          for (int b = 0; b < D; ++b) {
+            double t = lw * (vgi[a] * vgj[b]) + mw * (vgi[b] * vgj[a]) - cw * (vhi[a] * vhj[b]);
+            if (a == b) t += mw * dot;
+            acc[a * D + b] += t;
+          }
       }
-      // block p of the element's pairs (packed when upper): 72 contiguous bytes at
-      // f64; with slot_dest at its contribution-ordered slot, maybe transposed
-      if (slot_dest) {
-        const int32_t d = slot_dest[e * n_pairs + p];
-        T* out = tslot + (int64_t)(d & INT32_MAX) * DD;
-        if (d < 0) {
 #pragma unroll
-          for (int a = 0; a < D; ++a)
-#pragma unroll
-            for (int b = 0; b < D; ++b) out[a * D + b] = (T)acc[b * D + a];
-        } else {
-#pragma unroll
-          for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
-        }
-      } else {
-        T* out = tslot + (e * n_pairs + p) * DD;
-#pragma unroll
-        for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
-      }
+      for (int c = 0; c < DD; ++c) out[c] = (T)acc[c];
     }
   }
 }

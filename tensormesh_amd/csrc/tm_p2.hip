@@ -71,6 +71,16 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   if (active) {
     if (lane < B) s_v[wv][lane] = conn[e * B + lane];
   }
+  // the lane's block destinations (contribution order), loaded now so that
+  // their latency hides behind the per-point stages
+  constexpr int kPairRounds = (B * B + L - 1) / L;
+  int32_t dst[kPairRounds];
+#pragma unroll
+  for (int r = 0; r < kPairRounds; ++r) {
+    const int p = lane + r * L;
+    dst[r] = (FORM == TM_FORM_LINEAR_ELASTICITY && slot_dest && active && p < n_pairs)
+                 ? slot_dest[e * n_pairs + p] : 0;
+  }
   __syncthreads();
   if (active) {
     for (int t = lane; t < B * D; t += L) {
@@ -149,8 +159,23 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
   T* out = slot + e * (int64_t)(n_pairs * HH);
   if constexpr (FORM == TM_FORM_LINEAR_ELASTICITY) {
     // one lane per node pair: the H x H block from one read of g_i, g_j
-    for (int p = lane; p < n_pairs; p += L) {
-      const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
+#pragma unroll
+    for (int r = 0; r < kPairRounds; ++r) {
+      const int p = lane + r * L;
+      if (p >= n_pairs) break;
+      int i = s_pair[p] >> 8, j = s_pair[p] & 255;
+      // contribution-ordered slot, transposed where the pair is read as
+      // (j, i): formed as block (j, i), the exact transpose of (i, j)
+      T* o = out + p * HH;
+      if (slot_dest) {
+        const int32_t d = dst[r];
+        o = slot + (int64_t)(d & INT32_MAX) * HH;
+        if (d < 0) {
+          const int t = i;
+          i = j;
+          j = t;
+        }
+      }
       double acc[HH];
 #pragma unroll
       for (int k = 0; k < HH; ++k) acc[k] = 0.0;
@@ -169,26 +194,14 @@ __global__ void __launch_bounds__(kWave * kWavesPerBlock)
 #pragma unroll
         for (int a = 0; a < H; ++a)
 #pragma unroll
-          for (int c = 0; c < H; ++c)
-            acc[a * H + c] += (lam * (gi[a] * gj[c]) + mu * (gi[c] * gj[a]) + (a == c ? mu * dot : 0.0)) * w;
+          for (int c = 0; c < H; ++c) {
+            double t = lam * (gi[a] * gj[c]) + mu * (gi[c] * gj[a]);
+            if (a == c) t += mu * dot;
+            acc[a * H + c] += t * w;
+          }
       }
-      if (slot_dest) {
-        // contribution-ordered slot, transposed where the pair is read as (j, i)
-        const int32_t d = slot_dest[e * n_pairs + p];
-        T* o = slot + (int64_t)(d & INT32_MAX) * HH;
-        if (d < 0) {
 #pragma unroll
-          for (int a = 0; a < H; ++a)
-#pragma unroll
-            for (int c = 0; c < H; ++c) o[a * H + c] = (T)acc[c * H + a];
-        } else {
-#pragma unroll
-          for (int k = 0; k < HH; ++k) o[k] = (T)acc[k];
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < HH; ++k) out[p * HH + k] = (T)acc[k];
-      }
+      for (int k = 0; k < HH; ++k) o[k] = (T)acc[k];
     }
     return;
   }
@@ -287,6 +300,15 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   const double* w = tb;
   const double* dphi = tb + Q + Q * B;
   if (active && lane < B) s_v[wv][lane] = conn[e * B + lane];
+  // the lane's tangent block destinations (contribution order), loaded now
+  // so that their latency hides behind the per-point stages
+  constexpr int kPairRounds = (B * B + L - 1) / L;
+  int32_t dst[kPairRounds];
+#pragma unroll
+  for (int r = 0; r < kPairRounds; ++r) {
+    const int p = lane + r * L;
+    dst[r] = (!MFMA && slot_dest && tslot && active && p < n_pairs) ? slot_dest[e * n_pairs + p] : 0;
+  }
   __syncthreads();
   if (active) {
     for (int t = lane; t < B * D; t += L) {
@@ -495,18 +517,26 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     // one lane per node pair (i, j): the whole D x D block from one read of
     // g_i, g_j, h_i, h_j per quadrature point
     constexpr int DD = D * D;
-    for (int p = lane; p < n_pairs; p += L) {
-      const int i = s_pair[p] >> 8, j = s_pair[p] & 255;
+#pragma unroll
+    for (int r = 0; r < kPairRounds; ++r) {
+      const int p = lane + r * L;
+      if (p >= n_pairs) break;
+      int i = s_pair[p] >> 8, j = s_pair[p] & 255;
       T* out;
-      bool trans = false;
       if constexpr (!kStage) {
         // element order: block p of the element's pairs (packed when upper);
         // incidence order (full blocks only): slot row inc_inv[e*B + i];
-        // contribution order (upper): slot slot_dest[e*P + p], maybe transposed
+        // contribution order (upper): slot slot_dest[e*P + p], transposed
+        // when its id is negative -- formed as block (j, i), which is the
+        // exact transpose of (i, j) (the same products, commuted)
         if (slot_dest) {
-          const int32_t d = slot_dest[e * n_pairs + p];
+          const int32_t d = dst[r];
           out = tslot + (int64_t)(d & INT32_MAX) * DD;
-          trans = d < 0;
+          if (d < 0) {
+            const int t = i;
+            i = j;
+            j = t;
+          }
         } else {
           out = inc_inv ? tslot + (int64_t)inc_inv[e * B + i] * kRowVals + j * DD
                         : tslot + (e * n_pairs + p) * DD;
@@ -539,18 +569,14 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
 #pragma unroll
         for (int a = 0; a < D; ++a)
 #pragma unroll
-          for (int b = 0; b < D; ++b)
-            acc[a * D + b] += lw * (vhi[a] * vhj[b]) - cw * (vhi[b] * vhj[a]) + (a == b ? mw : 0.0);
+          for (int b = 0; b < D; ++b) {
+            double t = lw * (vhi[a] * vhj[b]) - cw * (vhi[b] * vhj[a]);
+            if (a == b) t += mw;
+            acc[a * D + b] += t;
+          }
       }
-      if (trans) {
 #pragma unroll
-        for (int a = 0; a < D; ++a)
-#pragma unroll
-          for (int b = 0; b < D; ++b) out[a * D + b] = (T)acc[b * D + a];
-      } else {
-#pragma unroll
-        for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
-      }
+      for (int k = 0; k < DD; ++k) out[k] = (T)acc[k];
     }
   }
   if constexpr (kStage) {

@@ -366,6 +366,7 @@ __device__ __forceinline__ T fma_t(T a, T b, T c) {
 // ---------------------------------------------------------------------------
 constexpr int kNS2 = 16;
 
+
 // sum over the system's workgroups [b0, b1) of part[w], in w order (every
 // thread of the workgroup gets the same value)
 __device__ __forceinline__ double system_sum(const double* __restrict__ part, int32_t b0, int32_t b1,
@@ -373,6 +374,35 @@ __device__ __forceinline__ double system_sum(const double* __restrict__ part, in
   double v = 0.0;
   for (int32_t w = b0 + threadIdx.x; w < b1; w += kBlock) v += part[w];
   return block_sum<double>(v, sh);
+}
+
+// The partials part[w], w in [b0, b1), that this thread adds in system_sum
+// (w = b0 + threadIdx.x + k kBlock, ascending): the first kPartRegs of them
+// loaded into registers up front (part_load), so that their latency overlaps
+// the kernel's other loads, then added in the same order (part_add), which
+// gives system_sum's bits after block_sum_n.
+constexpr int kPartRegs = 8;  // 2048 workgroups per system in registers
+
+struct PartRegs {
+  double v[kPartRegs];
+};
+
+__device__ __forceinline__ void part_load(const double* __restrict__ part, int32_t b0, int32_t b1, PartRegs& r) {
+#pragma unroll
+  for (int k = 0; k < kPartRegs; ++k) {
+    const int32_t w = b0 + (int32_t)threadIdx.x + k * kBlock;
+    r.v[k] = w < b1 ? part[w] : 0.0;
+  }
+}
+
+__device__ __forceinline__ double part_add(const double* __restrict__ part, int32_t b0, int32_t b1,
+                                           const PartRegs& r) {
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPartRegs; ++k)
+    if (b0 + (int32_t)threadIdx.x + k * kBlock < b1) v += r.v[k];
+  for (int32_t w = b0 + (int32_t)threadIdx.x + kPartRegs * kBlock; w < b1; w += kBlock) v += part[w];
+  return v;
 }
 
 // two system sums in one workgroup reduction
@@ -448,15 +478,26 @@ __global__ void __launch_bounds__(kBlock)
   const int32_t sy = blk_sys[blk];
   const int32_t b0 = sys_blk[sy], b1 = sys_blk[sy + 1];
   double* c = sc + sy * kNS2;
-  // the matrix loads go out before the |r|^2 reduction
+  // the partials, the row's u and the matrix loads all go out before the
+  // |r|^2 reduction
   const int32_t row0 = blk_row0[blk], row1 = blk_row1[blk];
+  const int32_t i = row0 + threadIdx.x;
+  PartRegs pr, pb;
+  part_load(part + 2 * nb, b0, b1, pr);
+  if (first) part_load(part + 4 * nb, b0, b1, pb);
+  const T ui = i < row1 ? u[i] : T(0);
   SpmvHead<T> head;
   spmv_begin<T>(rp, ci, av, row0, row1, head);
   double rr, bb;
   if (first) {
-    system_sum2(part + 2 * nb, part + 4 * nb, b0, b1, sh, rr, bb);
+    double v[2] = {part_add(part + 2 * nb, b0, b1, pr), part_add(part + 4 * nb, b0, b1, pb)};
+    block_sum_n<2>(v, sh);
+    rr = v[0];
+    bb = v[1];
   } else {
-    rr = system_sum(part + 2 * nb, b0, b1, sh);
+    double v[1] = {part_add(part + 2 * nb, b0, b1, pr)};
+    block_sum_n<1>(v, sh);
+    rr = v[0];
     bb = c[5];
   }
   const bool active = rr > tol2 * bb && bb > 0.0;
@@ -468,12 +509,10 @@ __global__ void __launch_bounds__(kBlock)
   if (!active) return;  // uniform per workgroup
   __shared__ T s_prod[kChunk];
   const T wv = spmv_finish<T>(ci, av, u, head, s_prod);
-  const int32_t i = row0 + threadIdx.x;
   double d = 0.0;
   if (i < row1) {
-    const T wi = wv;
-    w[i] = wi;
-    d = (double)wi * (double)u[i];
+    w[i] = wv;
+    d = (double)wv * (double)ui;
   }
   d = block_sum<T>(d, sh);
   if (threadIdx.x == 0) part[3 * nb + blk] = d;
@@ -506,9 +545,13 @@ __global__ void __launch_bounds__(kBlock)
     r0 = r[i];
     d0 = dinv[i];
   }
+  PartRegs pg, pd;
+  part_load(part + Q * nb, b0, b1, pg);
+  part_load(part + 3 * nb, b0, b1, pd);
   if (c[6] == 0.0) return;  // cleared by the preceding S launch; uniform per system
-  double gamma, delta;
-  system_sum2(part + Q * nb, part + 3 * nb, b0, b1, sh, gamma, delta);
+  double gd[2] = {part_add(part + Q * nb, b0, b1, pg), part_add(part + 3 * nb, b0, b1, pd)};
+  block_sum_n<2>(gd, sh);
+  const double gamma = gd[0], delta = gd[1];
   const double* prev = c + 8 + 4 * Q;
   const double iter = prev[2];
   double alpha, beta;

@@ -733,7 +733,7 @@ def batched_bench(n_meshes=256, seed=0):
             "loop_same_api_s_extrapolated": t_loop, "speedup_vs_loop": t_loop / t_b}
 
 
-# the sources that decide what the headline kernel (p1_rows_kernel) reads and
+# the sources that decide what the headline kernel (p1_rows_dest_kernel) reads and
 # writes: the kernel, its headers and the builders of the plans it walks
 ROOFLINE_SOURCES = ("tensormesh_amd/csrc/tm_rows.hip", "tensormesh_amd/csrc/tm_device.h",
                     "tensormesh_amd/csrc/tm_internal.h", "include/tensormesh_amd.h",
@@ -821,7 +821,7 @@ def profile_kernels(reps=5):
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"profiled": ["p1_rows_kernel", "p1_element_kernel", "reduce_stream_kernel",
+    print(json.dumps({"profiled": ["p1_rows_dest_kernel", "p1_element_kernel", "reduce_stream_kernel",
                                    "p1_rows_reduce_kernel", "p1_element_inc_kernel", "cg2_vec_kernel",
                                    "cg2_spmv_kernel", "copy 268435456 B"], "reps": reps}))
 
@@ -966,7 +966,7 @@ def main():
     fb = fused_bytes(E, N, nnz, s=s, c=s)
     traffic = load_traffic() or {}
     # kernels of the path the API call runs (the fastest one, set as default below)
-    kernels = {"fused_rows": [("p1_rows_kernel", fb, f_s)],
+    kernels = {"fused_rows": [("p1_rows_dest_kernel", fb, f_s)],
                "two_stage_segments": [("p1_element_kernel", gb, g_s), ("reduce_stream_kernel", sb, s_s)],
                "two_stage_rows": [("p1_element_kernel", gb, g_s), ("p1_rows_reduce_kernel", s2b, s2_s)]}
     best = min((("fused_rows", dt_fused), ("two_stage_segments", dt_two), ("two_stage_rows", dt_two_rows)),

@@ -212,6 +212,22 @@ __device__ __forceinline__ G abs_g(G v) {
   return v < G(0) ? -v : v;
 }
 
+// 1 / a: the hardware reciprocal estimate refined by Newton-Raphson steps
+// r <- r + r (1 - a r) (two in double, one in float), within about an ulp of
+// the correctly rounded quotient, in place of the IEEE division sequence
+// (about half the instructions).
+template <typename G>
+__device__ __forceinline__ G recip(G a) {
+  if constexpr (sizeof(G) == 8) {
+    double r = __builtin_amdgcn_rcp(a);
+    r = __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
+  } else {
+    const float r = __builtin_amdgcn_rcpf(a);
+    return __builtin_fmaf(r, __builtin_fmaf(-a, r, 1.0f), r);
+  }
+}
+
 // Row 0 of the rotated element's local matrix: K[0][p] = scale g_0 . g_p
 template <int D, int FORM, typename G>
 __device__ __forceinline__ void rot_row(const RotData<D, G>& d, const RowConsts& c, double (&Kr)[D + 1]) {
@@ -232,24 +248,26 @@ __device__ __forceinline__ void rot_row(const RotData<D, G>& d, const RowConsts&
   }
   G coef = (G)c.W;
   if constexpr (FORM == TM_FORM_DIFFUSION_ELEM) coef *= d.k;
-  const G scale = coef / abs_g(det);
+  const G scale = coef * recip(abs_g(det));
+  // sums start from their first term (a 0 + x the compiler must keep, for
+  // the sign of a zero x, costs an instruction each)
   G g0[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    G s = G(0);
+    G s = n[0][j];
 #pragma unroll
-    for (int k = 0; k < D; ++k) s += n[k][j];
+    for (int k = 1; k < D; ++k) s += n[k][j];
     g0[j] = -s;
   }
-  G s0 = G(0);
+  G s0 = g0[0] * g0[0];
 #pragma unroll
-  for (int j = 0; j < D; ++j) s0 += g0[j] * g0[j];
+  for (int j = 1; j < D; ++j) s0 += g0[j] * g0[j];
   Kr[0] = (double)(scale * s0);
 #pragma unroll
   for (int p = 1; p < B; ++p) {
-    G s = G(0);
+    G s = g0[0] * n[p - 1][0];
 #pragma unroll
-    for (int j = 0; j < D; ++j) s += g0[j] * n[p - 1][j];
+    for (int j = 1; j < D; ++j) s += g0[j] * n[p - 1][j];
     Kr[p] = (double)(scale * s);
   }
 }

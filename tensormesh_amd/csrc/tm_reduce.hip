@@ -152,7 +152,10 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
   }
   if (!seg_slot) {
     // contiguous contributions: the run's slots are one contiguous range,
-    // every value of this thread loaded before any is stored
+    // every value of this thread loaded before any is stored.  Each slot is
+    // read once: non-temporal loads leave the caches to the output entries
+    // (the scattered mirror writes merge there), measured 2-5 % faster on
+    // the 3 x 3 block reductions.
     const T* src = slot + (int64_t)cb * HH;
     const int32_t nv = n * HH;
     constexpr int kPerThread = (kCap * HH + kBlock - 1) / kBlock;
@@ -160,7 +163,7 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {
       const int32_t u = (int32_t)threadIdx.x + j * kBlock;
-      v[j] = u < nv ? src[u] : T(0);
+      v[j] = u < nv ? __builtin_nontemporal_load(&src[u]) : T(0);
     }
 #pragma unroll
     for (int j = 0; j < kPerThread; ++j) {

@@ -15,9 +15,10 @@
 // Runs that exceed the LDS capacity (a single very long entry) fall back to a
 // thread-per-entry walk over global memory.
 //
-// Workgroups are remapped so that each XCD gets a contiguous range of runs:
-// neighbouring runs read the same element slot rows, which then stay in one
-// XCD's L2.
+// Workgroups of gathered runs are remapped so that each XCD gets a
+// contiguous range of runs: neighbouring runs read the same element slot
+// rows, which then stay in one XCD's L2.  Contiguous runs (contribution-
+// ordered slots) run in descending order instead (see the kernel).
 //
 // Symmetric segment lists (tm_symmetric_segments): slot ids name packed
 // upper blocks (element kernels of symmetric forms store only the blocks
@@ -110,7 +111,12 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
                                                                T* __restrict__ out) {
   constexpr int kCap = kLdsBytes / (int)(sizeof(T) * HH) < 4096 ? kLdsBytes / (int)(sizeof(T) * HH) : 4096;
   __shared__ T lds[kCap * HH];
-  const int64_t b = xcd_logical(blockIdx.x, n_blocks);
+  // gathered runs: XCD-contiguous ranges (neighbouring runs gather the same
+  // element slots); contiguous runs share nothing and go in descending order:
+  // the element kernels write slots roughly in ascending row order, so the
+  // last runs' slots are the likeliest to still sit in the memory-side cache
+  // (Neo-Hookean f64 406 -> 380 us, hexahedral elasticity 304 -> 287 us)
+  const int64_t b = seg_slot ? xcd_logical(blockIdx.x, n_blocks) : n_blocks - 1 - (int64_t)blockIdx.x;
   const int32_t kb = blk_ptr[b], ke = blk_ptr[b + 1];
   const int32_t cb = seg_ptr[kb], ce = seg_ptr[ke];
   const int32_t n = ce - cb;

@@ -536,12 +536,14 @@ __device__ void walk_entries(const int32_t* __restrict__ conn, const T* __restri
 // phase 2 sums every entry's contiguous range.  Its own kernel so that it
 // allocates only the LDS it uses (the staged values and the rows' node
 // coordinates), which lets more workgroups share a CU.
-// Built for seven workgroups per CU (72 VGPRs) except for the nodal
-// coefficient form, which would spill: the float32 instances went from 52.7
-// to 48.0 us with it; float64 went 59.9 -> 71.5 us while its hot path
-// spilled, and fits without spills since the entry ranges are packed.
+// Built for seven workgroups per CU (72 VGPRs) in float64 and eight (64
+// VGPRs) in float32, except for the nodal coefficient form, which would
+// spill: the float32 instances went from 52.7 to 48.0 us at seven and from
+// 42.4 to 40.2 us at eight; float64 went 59.9 -> 71.5 us while its hot path
+// spilled at seven, fits without spills since the entry ranges are packed,
+// and spills again at eight (55 -> 65 us).
 template <typename T, int D, int FORM>
-__global__ void __launch_bounds__(kBlock, FORM == TM_FORM_DIFFUSION_NODE ? 1 : 7)
+__global__ void __launch_bounds__(kBlock, FORM == TM_FORM_DIFFUSION_NODE ? 1 : sizeof(T) == 4 ? 8 : 7)
     p1_rows_dest_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ coeff,
                         RowConsts c, const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ inc_ptr,
                         const int32_t* __restrict__ inc, const uint32_t* __restrict__ colpos,

@@ -258,12 +258,16 @@ typedef float tm_floatx16 __attribute__((ext_vector_type(16)));
 // BIG (tetra20, hexahedron27: element tangents of 60 x 60 and 81 x 81): one
 // wavefront per workgroup and the rule tables read through the caches instead
 // of LDS, so that a wave needs LDS only for its gradients (2 Q B D doubles).
-template <typename T, int D, int B, bool MFMA, int MQ, bool BIG, int EPW>
+template <typename T, int D, int B, bool MFMA, int MQ, bool BIG, int EPW, bool TAN>
 __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     hyper_wave_kernel(const int32_t* __restrict__ conn, const T* __restrict__ pts, const T* __restrict__ disp,
                       const double* __restrict__ tab, int Q, double lam, double mu, int64_t n_elem, int upper,
                       T* __restrict__ tslot, T* __restrict__ rslot, const int32_t* __restrict__ inc_inv,
                       double* __restrict__ energy_e, const int32_t* __restrict__ slot_dest) {
+  // TAN: the launch forms the tangent (ts == tslot); energy and residual
+  // launches compile without the h arrays, the tangent stage and its
+  // registers
+  T* const ts = TAN ? tslot : nullptr;
   constexpr int kMaxQ = MQ;
   constexpr int kRowVals = B * D * D;  // values of one slot row
   constexpr int kWpb = BIG ? 1 : kWavesPerBlock;  // waves per workgroup
@@ -278,7 +282,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   __shared__ double s_inv[kEpb][kMaxQ][D * D];
   __shared__ double s_jxw[kEpb][kMaxQ];
   __shared__ double s_g[kEpb][kMaxQ * B * D];
-  __shared__ double s_h[kEpb][kMaxQ * B * D];
+  __shared__ double s_h[kEpb][TAN ? kMaxQ * B * D : 1];
   __shared__ double s_P[kEpb][kMaxQ][D * D];
   __shared__ double s_c2[kEpb][kMaxQ];
   __shared__ double s_psi[kEpb][kMaxQ];   // psi * jxw per point (energy launches)
@@ -307,7 +311,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
 #pragma unroll
   for (int r = 0; r < kPairRounds; ++r) {
     const int p = lane + r * L;
-    dst[r] = (!MFMA && slot_dest && tslot && active && p < n_pairs) ? slot_dest[e * n_pairs + p] : 0;
+    dst[r] = (!MFMA && slot_dest && ts && active && p < n_pairs) ? slot_dest[e * n_pairs + p] : 0;
   }
   __syncthreads();
   if (active) {
@@ -440,7 +444,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     for (int q = 0; q < Q; ++q) s += s_psi[wv][q];
     energy_e[e] = s;
   }
-  if (active && tslot) {
+  if (active && ts) {
     // h = F^-T g, read by the tangent only
     for (int t = lane; t < Q * B * D; t += L) {
       const int q = t / (B * D), b = (t / D) % B, a = t % D;
@@ -477,7 +481,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
     // Lane l supplies A[m = l%32][k = l/32] and B[k = l/32][n = l%32]; the
     // accumulator register r of lane l holds row (r/4)*8 + (l/32)*4 + r%4,
     // column l%32.
-    if (tslot) {
+    if (ts) {
       constexpr int DD = D * D;
       constexpr int M = B * D;
       static_assert(M <= 32, "tile holds at most 32 rows");
@@ -513,7 +517,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
         }
       }
     }
-  } else if (tslot && active) {
+  } else if (ts && active) {
     // one lane per node pair (i, j): the whole D x D block from one read of
     // g_i, g_j, h_i, h_j per quadrature point
     constexpr int DD = D * D;
@@ -531,15 +535,15 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
         // exact transpose of (i, j) (the same products, commuted)
         if (slot_dest) {
           const int32_t d = dst[r];
-          out = tslot + (int64_t)(d & INT32_MAX) * DD;
+          out = ts + (int64_t)(d & INT32_MAX) * DD;
           if (d < 0) {
             const int t = i;
             i = j;
             j = t;
           }
         } else {
-          out = inc_inv ? tslot + (int64_t)inc_inv[e * B + i] * kRowVals + j * DD
-                        : tslot + (e * n_pairs + p) * DD;
+          out = inc_inv ? ts + (int64_t)inc_inv[e * B + i] * kRowVals + j * DD
+                        : ts + (e * n_pairs + p) * DD;
         }
       } else {
         out = &s_out[wv][(i * B + j) * DD];
@@ -581,7 +585,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
   }
   if constexpr (kStage) {
     __syncthreads();
-    if (tslot && active) {
+    if (ts && active) {
       // whole slot rows with vector stores: 16 bytes when a row's size allows,
       // else 8 (slot buffers are 256-byte aligned)
       constexpr int kRowBytes = kRowVals * (int)sizeof(T);
@@ -593,7 +597,7 @@ __global__ void __launch_bounds__(kWave * (BIG ? 1 : kWavesPerBlock))
         const int i = v / kPerRow, o = v % kPerRow;
         const int64_t srow = inc_inv ? (int64_t)inc_inv[e * B + i] : e * B + i;
         const vec_t x = reinterpret_cast<const vec_t*>(&s_out[wv][i * kRowVals])[o];
-        reinterpret_cast<vec_t*>(tslot + srow * kRowVals)[o] = x;
+        reinterpret_cast<vec_t*>(ts + srow * kRowVals)[o] = x;
       }
     }
   }
@@ -634,11 +638,19 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   const int upper = (f->flags & TM_FLAG_UPPER) != 0;
   if (upper && (mfma || inc_inv))
     return tmh::fail(TM_ERR_ARG, "TM_FLAG_UPPER needs the element-ordered VALU tangent (no MFMA, no block rows)");
-#define TM_HYPER_V(MF, MQ, BG, EPW)                                                                 \
-  hyper_wave_kernel<T, D, B, MF, MQ, BG, EPW><<<g, blk, 0, stream>>>(                               \
+#define TM_HYPER_T(MF, MQ, BG, EPW, TAN)                                                            \
+  hyper_wave_kernel<T, D, B, MF, MQ, BG, EPW, TAN><<<g, blk, 0, stream>>>(                          \
       m->conn, (const T*)m->points, (const T*)disp, f->tab_dev, Q, lam, mu, m->n_elem, upper,     \
       (T*)tslot, (T*)rslot, inc_inv, energy_e, upper ? slot_dest : nullptr)
-#define TM_HYPER(MF, MQ, EPW) TM_HYPER_V(MF, MQ, false, EPW)
+#define TM_HYPER_V(MF, MQ, BG, EPW) TM_HYPER_T(MF, MQ, BG, EPW, true)
+  // the VALU instances come without the tangent machinery for energy and
+  // residual launches (fewer registers and LDS: more waves per SIMD)
+#define TM_HYPER(MF, MQ, EPW)                                \
+  do {                                                       \
+    if constexpr (MF) TM_HYPER_T(MF, MQ, false, EPW, true);  \
+    else if (tslot) TM_HYPER_T(MF, MQ, false, EPW, true);    \
+    else TM_HYPER_T(MF, MQ, false, EPW, false);              \
+  } while (0)
   if constexpr (kBigCell) {
     if (Q <= 16) TM_HYPER_V(false, 16, true, 1); else TM_HYPER_V(false, 32, true, 1);
   } else if (big) {
@@ -656,6 +668,7 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   }
 #undef TM_HYPER
 #undef TM_HYPER_V
+#undef TM_HYPER_T
   hipError_t err = hipGetLastError();
   if (err != hipSuccess) {
     tmh::set_error("neo-hookean kernel launch: %s", hipGetErrorString(err));

@@ -624,8 +624,10 @@ int launch_hyper(const tm_mesh_view* m, const tm_form_desc* f, const void* disp,
   if (mfma && (Q * D * D) % 2 != 0) return tmh::fail(TM_ERR_UNSUPPORTED, "MFMA path needs an even Q*D*D");
   // two elements per wavefront (32 lanes each) on the VALU path by default:
   // the per-quadrature-point stages, which use Q lanes, then serve two
-  // elements; form->epw asks for 1, 2 or 4 (4: 16 lanes, rules of <= 4 points)
-  int epw = (mfma || big) ? 1 : 2;
+  // elements; four (16 lanes each) without a tangent, whose pair loop wants
+  // no lanes (tetra10 energy + residual 185 -> 146 us per call, f64); form->epw
+  // asks for 1, 2 or 4 (4: 16 lanes, rules of <= 4 points)
+  int epw = (mfma || big) ? 1 : (!tslot && Q <= 4 && B <= 16) ? 4 : 2;
   if (f->epw != 0) {
     const bool ok = f->epw == 1 || (!mfma && !big && (f->epw == 2 || (f->epw == 4 && Q <= 4 && B <= 16)));
     if (!ok) return tmh::fail(TM_ERR_ARG, "elements per wavefront not admitted for this cell / rule / tangent");

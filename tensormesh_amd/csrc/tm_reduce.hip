@@ -147,15 +147,15 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
   const bool groups = HH > 1 && ke - kb <= kMaxPass * kPerPass;
   const int32_t le = (int32_t)threadIdx.x / L, l = (int32_t)threadIdx.x - le * L;
   int32_t ra[kMaxPass], rz[kMaxPass];
-  int64_t d0[kMaxPass], d1[kMaxPass];
+  int32_t d0[kMaxPass], d1[kMaxPass];  // entry ids (< 2^31), -1: none
 #pragma unroll
   for (int ps = 0; ps < kMaxPass; ++ps) {
     const int32_t k = kb + le + ps * kPerPass;
     const bool has = groups && le < kPerPass && k < ke;
     ra[ps] = has ? seg_ptr[k] - cb : 0;
     rz[ps] = has ? seg_ptr[k + 1] - cb : 0;
-    d0[ps] = !has ? -1 : entry ? (int64_t)entry[k] : (int64_t)k;
-    d1[ps] = (has && entry) ? (int64_t)entry_t[k] : -1;
+    d0[ps] = !has ? -1 : entry ? entry[k] : k;
+    d1[ps] = (has && entry) ? entry_t[k] : -1;
   }
   if (!seg_slot) {
     // contiguous contributions: the run's slots are one contiguous range,
@@ -268,11 +268,11 @@ __global__ void __launch_bounds__(kBlock) reduce_stream_kernel(const int32_t* __
         for (int v = 0; v < V; ++v) acc[v] += lds[u * HH + l * V + v];
       }
 #pragma unroll
-      for (int v = 0; v < V; ++v) out[d0[ps] * HH + l * V + v] = acc[v];
+      for (int v = 0; v < V; ++v) out[(int64_t)d0[ps] * HH + l * V + v] = acc[v];
       if (d1[ps] >= 0 && d1[ps] != d0[ps]) {
         // value (l, v) of the block lands at (v, l) of the mirror
 #pragma unroll
-        for (int v = 0; v < V; ++v) out[d1[ps] * HH + (kD > 1 ? v * kD + l : l * V + v)] = acc[v];
+        for (int v = 0; v < V; ++v) out[(int64_t)d1[ps] * HH + (kD > 1 ? v * kD + l : l * V + v)] = acc[v];
       }
     }
   }

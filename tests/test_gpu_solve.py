@@ -100,3 +100,26 @@ def test_variants_agree_on_a_many_workgroup_system():
     s = BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-10)
     x2b, _, _ = s.solve(fi, chunk=32)
     assert torch.equal(x2, x2b)
+
+
+def test_two_launch_beyond_register_partials_matches_one_launch(monkeypatch):
+    """A 531,441-row system spans 2,076 workgroups, more than the 2,048 whose
+    partial sums the two-launch kernels load into registers ahead of use: the
+    rest are added by the remainder loop, in the same order as the one-launch
+    kernels' sums, so the iterates stay bit-identical and reach the tolerance
+    (the library's own choice stops at 2,048 workgroups per system; a C-ABI
+    caller may go further)."""
+    Ki, fi = _system(730, 6)
+    assert Ki.shape[0] > 2048 * 256
+    monkeypatch.setattr(BatchedPCG, "TWO_LAUNCH_MAX_BLOCKS", 4096)
+    out = {}
+    for launches in (1, 2):
+        monkeypatch.setattr(BatchedPCG, "launches", launches)
+        s = BatchedPCG(Ki.row_ptr, Ki.col_indices, Ki.values, [0, Ki.shape[0]], rtol=1e-6)
+        assert s.n_blocks > 2048 and s.variant == {1: 2, 2: 1}[launches]
+        out[launches] = s.solve(fi, chunk=64)
+    (x1, it1, r1), (x2, it2, r2) = out[1], out[2]
+    assert torch.equal(x1, x2) and torch.equal(it1, it2)
+    assert r2[0].item() <= 1e-6
+    res = torch.linalg.norm(Ki @ x2 - fi) / torch.linalg.norm(fi)
+    assert res.item() <= 2e-6
